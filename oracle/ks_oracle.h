@@ -1,0 +1,119 @@
+/*
+ * ks_oracle.h -- FP64 CPU restatement of the reference's 3D real-space
+ * Kohn-Sham path (willandru/DFT, src/dft/*Molecule*.cpp, PZ81.cpp, PBE96.cpp).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP path is
+ * compared against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it; the product library never links or calls it.
+ *
+ * Every function restates the reference's arithmetic in the same operation
+ * order (built with -O2 -ffp-contract=off, like the reference's ISO C++20
+ * build), so results agree bitwise with the compiled reference on the same
+ * libm.  Citations are reference file:line.
+ *
+ * Errors: functions return 0 on success, -1 for an invalid argument (the
+ * reference's std::invalid_argument), -2 for a numerical failure
+ * (std::runtime_error).  og_last_error() holds the message.
+ */
+#ifndef KS_ORACLE_H
+#define KS_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CartesianGrid.cpp:5-159 -- spacing (max-min)/(n-1), x fastest. */
+typedef struct {
+    long nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    double dx, dy, dz;
+} og_grid;
+
+int og_grid_init(og_grid* g, long nx, long ny, long nz, double xmin, double xmax,
+                 double ymin, double ymax, double zmin, double zmax);
+const char* og_last_error(void);
+
+/* KohnShamHamiltonianMolecule.cpp:7-183 */
+int og_hpsi(const og_grid* g, const double* veff, const double* psi, double* out);
+
+/* HartreePotentialMolecule.cpp:37-95 boundary rule; 99-324 Jacobi.
+ * zero_start=1: the reference call (boundary rule, interior 0).
+ * zero_start=0: warm start, interior taken from v on entry (boundary still
+ * set by the rule).  Stops after a sweep whose max|dV| < tol or after
+ * max_sweeps sweeps; v holds the latest iterate. */
+int og_poisson(const og_grid* g, const double* rho, double* v, long max_sweeps,
+               double tol, int zero_start, long* sweeps_done, double* last_maxdiff);
+/* HartreePotentialMolecule.cpp:326-379 */
+double og_hartree_energy(const og_grid* g, const double* rho, const double* vh);
+
+/* XCFunctional.h:5-21 XCInput -> XCResult: in = {na, nb, |grad na|, |grad nb|},
+ * out = {eps_xc, v_a, v_b, c_a, c_b}.  id 1 = PZ81 (PZ81.cpp:437-501),
+ * id 2 = PBE96 (PBE96.cpp:814-966). */
+int og_xc_point(int functional, const double in[4], double out[5]);
+
+/* ExchangeCorrelationMolecule.cpp:576-747 */
+int og_xc_potential(const og_grid* g, int functional, const double* na,
+                    const double* nb, int spin, double* v);
+/* ExchangeCorrelationMolecule.cpp:749-842 */
+double og_xc_energy(const og_grid* g, int functional, const double* na, const double* nb);
+
+/* ElectronDensityMolecule.cpp:5-76 (orbitals stored column-major, N each). */
+int og_spin_density(long n, int norb, const double* psi, const int* electrons, double* out);
+
+/* TotalEnergyMolecule.cpp:13-199 (Neumann-like stencil) */
+double og_kinetic_energy(const og_grid* g, int norb, const double* psi, const int* electrons);
+/* TotalEnergyMolecule.cpp:201-235 */
+double og_external_energy(const og_grid* g, const double* rho, const double* vnuc);
+
+/* NuclearPotential.cpp:6-76; nuclei = {Z, x, y, z} * natoms. */
+int og_nuclear_potential(const og_grid* g, int natoms, const double* nuclei, double* v);
+/* NuclearRepulsion.cpp:6-54 */
+int og_nuclear_repulsion(int natoms, const double* nuclei, double* e);
+
+/* EigenvalueSolverMolecule.cpp:666-1300: single-vector Davidson, orbitals
+ * solved one after another with deflation.  init may be NULL (seed vectors).
+ * psi_out: norb columns of N; eig_out: norb values; happly_out (optional):
+ * H applications per orbital. */
+int og_davidson(const og_grid* g, const double* veff, int norb, const int* occupations,
+                const double* init, long max_happly, double* psi_out, double* eig_out,
+                long* happly_out);
+
+/* SCF driver: SelfConsistentFieldMolecule.cpp:50-561, MolecularSCFIteration.cpp:12-252,
+ * MolecularSCFHelpers.cpp:27-249,515-579, with the knobs a configuration needs. */
+typedef struct {
+    int functional;        /* 1 PZ81, 2 PBE96 */
+    int charge;
+    int multiplicity;
+    int max_iterations;    /* reference: 250 */
+    int poisson_fixed;     /* 0: reference rule (tol 1e-8, cap 10000, zero start);
+                              k>0: exactly k sweeps, warm-started from the last V_H */
+    int stop_rule;         /* 0: reference triple criterion; 1: |dE| < energy_tol only */
+    double energy_tol;     /* used by stop_rule 1 */
+    int init_guess;        /* 0: bare-nucleus orbitals (reference); 1: hydrogenic 1s density */
+    long max_happly;       /* Davidson cap per orbital, reference 5000 */
+} og_scf_config;
+
+typedef struct {
+    int iterations;
+    int converged;
+    double energy[6];      /* kinetic, external, hartree, xc, nuclear, total */
+    double energy_difference;
+    double density_difference;
+    double ks_residual;
+    double eig_alpha0;     /* lowest alpha eigenvalue */
+    double eig_beta0;
+    int first_de_iteration; /* first step with |dE| < energy_tol (0 if none) */
+} og_scf_result;
+
+/* trace (optional): per step {E_total, dE, dRho, KS, mixing}; capacity max_iterations.
+ * eig_out (optional): n_alpha + n_beta final eigenvalues.
+ * rho_out (optional): 2 * N, final output alpha then beta densities. */
+int og_scf(const og_grid* g, int natoms, const double* nuclei, const og_scf_config* cfg,
+           og_scf_result* res, double* trace, double* eig_out, double* rho_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

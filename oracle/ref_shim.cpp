@@ -1,0 +1,235 @@
+// ref_shim.cpp -- extern "C" wrappers around the compiled reference
+// (willandru/DFT src/dft/*.cpp, built from where they lie by oracle/Makefile
+// into oracle/_ref/).  TEST INFRASTRUCTURE ONLY: used to write the golden
+// fixtures under tests/golden and to time the reference CPU path.  Each
+// wrapper calls the reference's public function named in its comment and
+// turns exceptions into status codes (-1 invalid_argument, -2 other).
+#include "CartesianGrid.h"
+#include "DFTData.h"
+#include "EigenvalueSolverMolecule.h"
+#include "ElectronDensityMolecule.h"
+#include "ExchangeCorrelationMolecule.h"
+#include "HartreePotentialMolecule.h"
+#include "KohnShamHamiltonianMolecule.h"
+#include "Molecule.h"
+#include "NuclearPotential.h"
+#include "NuclearRepulsion.h"
+#include "PBE96.h"
+#include "PZ81.h"
+#include "SelfConsistentFieldMolecule.h"
+#include "TotalEnergyMolecule.h"
+
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+std::string g_err;
+
+CartesianGrid make_grid(const long* n, const double* b) {
+    return CartesianGrid(n[0], n[1], n[2], b[0], b[1], b[2], b[3], b[4], b[5]);
+}
+std::vector<double> vec(const double* p, size_t n) { return std::vector<double>(p, p + n); }
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
+}
+
+const XCFunctional& functional(int id) {
+    static PZ81 pz;
+    static PBE96 pbe;
+    if (id == 1) return pz;
+    if (id == 2) return pbe;
+    throw std::invalid_argument("unknown functional id");
+}
+
+Molecule make_molecule(int natoms, const double* nuc, int charge) {
+    Molecule m(charge);
+    for (int a = 0; a < natoms; ++a) m.addNucleus((int)nuc[4 * a], nuc[4 * a + 1], nuc[4 * a + 2], nuc[4 * a + 3]);
+    return m;
+}
+}  // namespace
+
+extern "C" {
+
+const char* rf_last_error() { return g_err.c_str(); }
+
+// grid n = {nx, ny, nz}; bounds b = {xmin, xmax, ymin, ymax, zmin, zmax}
+int rf_grid_spacing(const long* n, const double* b, double* d) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        d[0] = g.getDx(); d[1] = g.getDy(); d[2] = g.getDz();
+    });
+}
+
+// applyMolecularKohnShamHamiltonian
+int rf_hpsi(const long* n, const double* b, const double* v, const double* psi, double* out) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        auto r = applyMolecularKohnShamHamiltonian(g, vec(v, g.getSize()), vec(psi, g.getSize()));
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// calculateHartreePotential / calculateHartreeEnergy
+int rf_poisson(const long* n, const double* b, const double* rho, double* out) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        auto r = calculateHartreePotential(g, vec(rho, g.getSize()));
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+int rf_hartree_energy(const long* n, const double* b, const double* rho, const double* vh, double* e) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        *e = calculateHartreeEnergy(g, vec(rho, g.getSize()), vec(vh, g.getSize()));
+    });
+}
+
+// XCFunctional::evaluate (PZ81 / PBE96) over npts points of {na, nb, ga, gb}
+int rf_xc_points(int id, long npts, const double* in, double* out) {
+    return guarded([&] {
+        const XCFunctional& f = functional(id);
+        for (long p = 0; p < npts; ++p) {
+            XCInput x{in[4 * p], in[4 * p + 1], in[4 * p + 2], in[4 * p + 3]};
+            XCResult r = f.evaluate(x);
+            double* o = out + 5 * p;
+            o[0] = r.energyPerElectron; o[1] = r.potentialAlpha; o[2] = r.potentialBeta;
+            o[3] = r.gradientCoefficientAlpha; o[4] = r.gradientCoefficientBeta;
+        }
+    });
+}
+
+// calculateMolecularSpinExchangeCorrelationPotential / ...Energy
+int rf_xc_potential(const long* n, const double* b, int id, const double* na, const double* nb, int spin, double* out) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        auto r = calculateMolecularSpinExchangeCorrelationPotential(functional(id), g, vec(na, g.getSize()),
+                                                                    vec(nb, g.getSize()), spin);
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+int rf_xc_energy(const long* n, const double* b, int id, const double* na, const double* nb, double* e) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        *e = calculateMolecularExchangeCorrelationEnergy(functional(id), g, vec(na, g.getSize()), vec(nb, g.getSize()));
+    });
+}
+
+// calculateMolecularSpinDensity (alpha orbitals, given electrons)
+int rf_spin_density(long npts, int norb, const double* psi, const int* electrons, double* out) {
+    return guarded([&] {
+        std::vector<MolecularOrbital> orbs(norb);
+        for (int o = 0; o < norb; ++o) {
+            orbs[o].spin = SpinChannel::Alpha;
+            orbs[o].electrons = electrons[o];
+            orbs[o].psi = vec(psi + (size_t)o * npts, npts);
+        }
+        auto r = calculateMolecularSpinDensity(orbs, SpinChannel::Alpha);
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// calculateMolecularKineticEnergy / calculateMolecularExternalEnergy
+int rf_kinetic(const long* n, const double* b, int norb, const double* psi, const int* electrons, double* e) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        std::vector<MolecularOrbital> orbs(norb);
+        for (int o = 0; o < norb; ++o) {
+            orbs[o].electrons = electrons[o];
+            orbs[o].psi = vec(psi + (size_t)o * g.getSize(), g.getSize());
+        }
+        *e = calculateMolecularKineticEnergy(g, orbs);
+    });
+}
+int rf_external(const long* n, const double* b, const double* rho, const double* vnuc, double* e) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        *e = calculateMolecularExternalEnergy(g, vec(rho, g.getSize()), vec(vnuc, g.getSize()));
+    });
+}
+
+// NuclearPotential::calculate / NuclearRepulsion::calculate
+int rf_nuclear_potential(const long* n, const double* b, int natoms, const double* nuc, double* out) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        auto r = NuclearPotential::calculate(make_molecule(natoms, nuc, 0), g);
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+int rf_nuclear_repulsion(int natoms, const double* nuc, double* e) {
+    return guarded([&] { *e = NuclearRepulsion::calculate(make_molecule(natoms, nuc, 0)); });
+}
+
+// solveMolecularOrbitals (alpha, occupations 1); init may be null.
+int rf_davidson(const long* n, const double* b, const double* v, int norb, const double* init, long maxit,
+                double* psi_out, double* eig_out) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        std::vector<MolecularOrbital> initial;
+        if (init) {
+            initial.resize(norb);
+            for (int o = 0; o < norb; ++o) initial[o].psi = vec(init + (size_t)o * g.getSize(), g.getSize());
+        }
+        std::vector<int> occ(norb, 1);
+        std::ostringstream sink;
+        std::streambuf* old = std::cerr.rdbuf(sink.rdbuf());
+        try {
+            auto r = solveMolecularOrbitals(g, vec(v, g.getSize()), norb, occ, SpinChannel::Alpha, initial, maxit);
+            std::cerr.rdbuf(old);
+            for (int o = 0; o < norb; ++o) {
+                std::memcpy(psi_out + (size_t)o * g.getSize(), r[o].psi.data(), g.getSize() * sizeof(double));
+                eig_out[o] = r[o].eigenvalue;
+            }
+        } catch (...) {
+            std::cerr.rdbuf(old);
+            throw;
+        }
+    });
+}
+
+// solveMolecularSelfConsistentField.  out = {iterations, converged, T, Eext,
+// EH, Exc, Enn, Etot, dE, dRho, KS}; eig_out: all orbital eigenvalues (alpha
+// then beta), capacity neig.
+int rf_scf(const long* n, const double* b, int natoms, const double* nuc, int charge, int mult, int id,
+           double* out, double* eig_out, int neig, int quiet) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        Molecule m = make_molecule(natoms, nuc, charge);
+        std::ostringstream sink1, sink2;
+        std::streambuf* o1 = nullptr;
+        std::streambuf* o2 = nullptr;
+        if (quiet) {
+            o1 = std::cout.rdbuf(sink1.rdbuf());
+            o2 = std::cerr.rdbuf(sink2.rdbuf());
+        }
+        try {
+            MolecularResult r = solveMolecularSelfConsistentField(g, m, charge, mult, functional(id));
+            if (quiet) { std::cout.rdbuf(o1); std::cerr.rdbuf(o2); }
+            const auto& s = r.scf;
+            double v[11] = {(double)s.iterations, s.converged ? 1.0 : 0.0, s.energy.kinetic, s.energy.external,
+                            s.energy.hartree, s.energy.exchangeCorrelation, s.energy.nuclearRepulsion,
+                            s.energy.total, s.energyDifference, s.densityDifference, s.maxKSResidual};
+            std::memcpy(out, v, sizeof v);
+            for (int i = 0; i < neig && i < (int)s.molecularOrbitals.size(); ++i) eig_out[i] = s.molecularOrbitals[i].eigenvalue;
+        } catch (...) {
+            if (quiet) { std::cout.rdbuf(o1); std::cerr.rdbuf(o2); }
+            throw;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+"""Pins for the FP64 CPU oracle (oracle/ks_oracle.c).
+
+Each oracle function is checked against (1) the reference's own outputs in
+tests/golden/reference_goldens.npz, written from the compiled reference, and
+(2) what the mathematics fixes: discrete-sine closed forms, symmetry, finite
+differences of the energy expressions, brute-force dense eigensolves and the
+reference's main.cpp SCF known answers.
+"""
+import numpy as np
+import pytest
+
+from conftest import grid_from
+from oracle.oracle import Grid, Reference
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ----------------------------------------------------------------- goldens
+def test_hpsi_golden(oracle, goldens):
+    g = grid_from(goldens["hpsi_grid"])
+    out = oracle.hpsi(g, goldens["hpsi_v"], goldens["hpsi_psi"])
+    np.testing.assert_array_equal(out, goldens["hpsi_out"])
+
+
+@pytest.mark.parametrize("case", ["poisson", "poissonq"])
+def test_poisson_golden(oracle, goldens, case):
+    g = grid_from(goldens[f"{case}_grid"])
+    vh, sweeps, md = oracle.poisson(g, goldens[f"{case}_rho"])
+    np.testing.assert_array_equal(vh, goldens[f"{case}_vh"])
+    assert md < 1e-8 and 0 < sweeps < 10000
+
+
+def test_poisson_boundary_branch(goldens):
+    # the Q/|r| branch is live on this grid: the reference left non-zero faces
+    g = grid_from(goldens["poissonq_grid"])
+    v = goldens["poissonq_vh"].reshape(g.shape)
+    assert np.count_nonzero(v[-1]) > 0 and np.count_nonzero(v[0]) == 0
+
+
+def test_hartree_energy_golden(oracle, goldens):
+    g = grid_from(goldens["poisson_grid"])
+    assert oracle.hartree_energy(g, goldens["poisson_rho"], goldens["poisson_vh"]) == float(goldens["poisson_eh"])
+
+
+@pytest.mark.parametrize("fid,name", [(1, "pz81"), (2, "pbe96")])
+def test_xc_points_golden(oracle, goldens, fid, name):
+    np.testing.assert_array_equal(oracle.xc_points(fid, goldens["xcpt_in"]), goldens[f"xcpt_{name}"])
+
+
+@pytest.mark.parametrize("fid,name", [(1, "pz81"), (2, "pbe96")])
+def test_xc_grid_golden(oracle, goldens, fid, name):
+    g = grid_from(goldens["xc_grid"])
+    na, nb = goldens["xc_na"], goldens["xc_nb"]
+    np.testing.assert_array_equal(oracle.xc_potential(g, fid, na, nb, 0), goldens[f"xc_{name}_va"])
+    np.testing.assert_array_equal(oracle.xc_potential(g, fid, na, nb, 1), goldens[f"xc_{name}_vb"])
+    assert oracle.xc_energy(g, fid, na, nb) == float(goldens[f"xc_{name}_exc"])
+
+
+def test_density_energies_golden(oracle, goldens):
+    g = grid_from(goldens["misc_grid"])
+    psi, el, nuc = goldens["misc_psi"], goldens["misc_el"], goldens["misc_nuc"]
+    np.testing.assert_array_equal(oracle.spin_density(psi, el), goldens["misc_density"])
+    assert oracle.kinetic_energy(g, psi, el) == float(goldens["misc_kinetic"])
+    vn = oracle.nuclear_potential(g, nuc)
+    np.testing.assert_array_equal(vn, goldens["misc_vnuc"])
+    assert oracle.external_energy(g, goldens["misc_rho"], vn) == float(goldens["misc_external"])
+    assert oracle.nuclear_repulsion(nuc) == float(goldens["misc_enn"])
+
+
+def test_davidson_golden(oracle, goldens):
+    g = grid_from(goldens["dav_grid"])
+    eig, psi, _ = oracle.davidson(g, goldens["dav_v"], 4)
+    np.testing.assert_array_equal(eig, goldens["dav_eig"])
+    np.testing.assert_array_equal(psi, goldens["dav_psi"])
+
+
+@pytest.mark.parametrize("name", ["h2", "h2o", "he_pbe"])
+def test_scf_golden(oracle, goldens, name):
+    g = grid_from(goldens[f"scf_{name}_grid"])
+    r = oracle.scf(g, goldens[f"scf_{name}_nuc"], int(goldens[f"scf_{name}_fid"]))
+    ref = goldens[f"scf_{name}_result"]
+    assert r["iterations"] == int(ref[0])
+    np.testing.assert_array_equal(r["energy"], ref[1:7])
+    ne = len(goldens[f"scf_{name}_eig"])
+    np.testing.assert_array_equal(r["eigenvalues"][: ne - 2], goldens[f"scf_{name}_eig"][: ne - 2])
+
+
+def test_scf_main_cpp_known_answers(oracle):
+    """main.cpp:74-174 systems; totals printed by the reference binary."""
+    g = Grid(7, 7, 7, -8.1, 7.9, -8.1, 7.9, -8.1, 7.9)
+    r = oracle.scf(g, [[1, -0.7, 0, 0], [1, 0.7, 0, 0]], 1)
+    assert f"{r['total']:.10e}" == "-2.8564865005e+00" and r["iterations"] == 29
+    g = Grid(9, 9, 9, -8.1, 7.9, -8.1, 7.9, -8.1, 7.9)
+    r = oracle.scf(g, [[8, 0, 0, 0], [1, 1.430, 0, 1.108], [1, -1.430, 0, 1.108]], 1)
+    assert f"{r['total']:.10e}" == "-1.0139828181e+02" and r["iterations"] == 87
+
+
+# ---------------------------------------------------------- mathematics
+def sine_mode(g, m):
+    xs = np.sin(np.pi * m[0] * (np.arange(g.n[0]) + 1) / (g.n[0] + 1))
+    ys = np.sin(np.pi * m[1] * (np.arange(g.n[1]) + 1) / (g.n[1] + 1))
+    zs = np.sin(np.pi * m[2] * (np.arange(g.n[2]) + 1) / (g.n[2] + 1))
+    return np.einsum("k,j,i->kji", zs, ys, xs).ravel()
+
+
+def test_hpsi_sine_modes_are_eigenvectors(oracle):
+    g = Grid(11, 9, 7, -2.0, 3.0, -1.0, 2.2, -1.5, 1.5)
+    for m in [(1, 1, 1), (2, 3, 1), (5, 1, 4)]:
+        psi = sine_mode(g, m)
+        lam = 0.5 * sum((2 - 2 * np.cos(np.pi * m[a] / (g.n[a] + 1))) / h ** 2
+                        for a, h in enumerate((g.dx, g.dy, g.dz)))
+        hp = oracle.hpsi(g, np.full(g.size, 0.7), psi)
+        np.testing.assert_allclose(hp, (lam + 0.7) * psi, rtol=0, atol=1e-12 * lam)
+
+
+def test_hpsi_symmetric(oracle):
+    rng = np.random.default_rng(3)
+    g = Grid(7, 6, 5, -1.0, 1.0, -1.0, 1.5, -2.0, 1.0)
+    v = rng.standard_normal(g.size)
+    a, b = rng.standard_normal(g.size), rng.standard_normal(g.size)
+    assert abs(a @ oracle.hpsi(g, v, b) - b @ oracle.hpsi(g, v, a)) < 1e-10
+
+
+def dense_h(oracle, g, v):
+    eye = np.eye(g.size)
+    return np.stack([oracle.hpsi(g, v, eye[i]) for i in range(g.size)], axis=1)
+
+
+def test_davidson_matches_dense_eigh(oracle):
+    g = Grid.centered(6, 0.6)
+    v = oracle.nuclear_potential(g, [[2, 0.05, 0.1, -0.02]])
+    w = np.linalg.eigvalsh(dense_h(oracle, g, v))
+    eig, psi, _ = oracle.davidson(g, v, 3)
+    np.testing.assert_allclose(eig, w[:3], atol=1e-9)
+    dv = g.dx * g.dy * g.dz
+    np.testing.assert_allclose(psi @ psi.T * dv, np.eye(3), atol=1e-9)
+
+
+def test_poisson_sine_source_closed_form(oracle):
+    # interior points 1..n-2 with V=0 on the faces: sin(pi m i/(n-1)) modes
+    g = Grid(12, 10, 9, -2.0, 2.0, -1.5, 1.8, -1.2, 1.2)
+    m = (1, 2, 1)
+    xs = np.sin(np.pi * m[0] * np.arange(g.n[0]) / (g.n[0] - 1))
+    ys = np.sin(np.pi * m[1] * np.arange(g.n[1]) / (g.n[1] - 1))
+    zs = np.sin(np.pi * m[2] * np.arange(g.n[2]) / (g.n[2] - 1))
+    rho = np.einsum("k,j,i->kji", zs, ys, xs).ravel()
+    lam = sum((2 - 2 * np.cos(np.pi * m[a] / (g.n[a] - 1))) / h ** 2 for a, h in enumerate((g.dx, g.dy, g.dz)))
+    # total charge of this source is not zero, but every boundary point lies on a face here
+    vh, sweeps, md = oracle.poisson(g, rho, max_sweeps=100000, tol=1e-14)
+    np.testing.assert_allclose(vh, 4 * np.pi * rho / lam, atol=1e-11)
+
+
+def test_poisson_fixed_point_residual(oracle, goldens):
+    g = grid_from(goldens["poisson_grid"])
+    rho = goldens["poisson_rho"]
+    vh, _, _ = oracle.poisson(g, rho, max_sweeps=200000, tol=1e-15)
+    V = vh.reshape(g.shape)
+    lap = ((V[1:-1, 1:-1, 2:] + V[1:-1, 1:-1, :-2] - 2 * V[1:-1, 1:-1, 1:-1]) / g.dx ** 2
+           + (V[1:-1, 2:, 1:-1] + V[1:-1, :-2, 1:-1] - 2 * V[1:-1, 1:-1, 1:-1]) / g.dy ** 2
+           + (V[2:, 1:-1, 1:-1] + V[:-2, 1:-1, 1:-1] - 2 * V[1:-1, 1:-1, 1:-1]) / g.dz ** 2)
+    np.testing.assert_allclose(lap, -4 * np.pi * rho.reshape(g.shape)[1:-1, 1:-1, 1:-1], atol=1e-9)
+
+
+def eps_n(oracle, fid, p):
+    o = oracle.xc_points(fid, np.array([p]))[0]
+    return (p[0] + p[1]) * o[0]
+
+
+@pytest.mark.parametrize("fid", [1, 2])
+@pytest.mark.parametrize("p", [[0.3, 0.3, 0.0, 0.0], [0.2, 0.05, 0.0, 0.0], [0.02, 0.011, 0.01, 0.02],
+                               [1.7, 0.4, 0.9, 0.3], [0.004, 0.002, 1e-4, 3e-5]])
+def test_xc_potential_is_energy_derivative(oracle, fid, p):
+    """v_sigma = d(n eps)/dn_sigma and c_sigma = d(n eps)/d|grad n_sigma| (central differences)."""
+    o = oracle.xc_points(fid, np.array([p]))[0]
+    for k in range(2):
+        h = 1e-5 * p[k]
+        pp, pm = list(p), list(p)
+        pp[k] += h
+        pm[k] -= h
+        fd = (eps_n(oracle, fid, pp) - eps_n(oracle, fid, pm)) / (2 * h)
+        assert abs(fd - o[1 + k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, o)
+    if fid == 2:
+        for k in (2, 3):
+            if p[k] == 0:
+                continue
+            h = 1e-5 * p[k]
+            pp, pm = list(p), list(p)
+            pp[k] += h
+            pm[k] -= h
+            fd = (eps_n(oracle, fid, pp) - eps_n(oracle, fid, pm)) / (2 * h)
+            assert abs(fd - o[1 + k]) < 1e-6 * max(1e-3, abs(fd)), (k, fd, o)
+
+
+def pw92_unpolarised(rs):
+    # Perdew & Wang, PRB 45, 13244 (1992), eq. 10 with the zeta=0 parameters of Table I
+    A, a1, b1, b2, b3, b4 = 0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294
+    s = np.sqrt(rs)
+    return -2 * A * (1 + a1 * rs) * np.log(1 + 1 / (2 * A * (b1 * s + b2 * rs + b3 * rs * s + b4 * rs * rs)))
+
+
+@pytest.mark.parametrize("n", [1e-4, 0.01, 0.3, 5.0])
+def test_pbe_zero_gradient_is_lda_exchange_plus_pw92(oracle, n):
+    o = oracle.xc_points(2, np.array([[n / 2, n / 2, 0.0, 0.0]]))[0]
+    ex = -0.75 * (3 / np.pi) ** (1 / 3) * n ** (1 / 3)
+    rs = (3 / (4 * np.pi * n)) ** (1 / 3)
+    # the reference uses A = 0.0310907 (PBE96.cpp:192); the paper's 0.031091 differs by 3e-7 relative
+    assert abs(o[0] - (ex + pw92_unpolarised(rs))) < 5e-6 * abs(o[0])
+    assert o[3] == 0.0 and o[4] == 0.0
+
+
+@pytest.mark.parametrize("n", [1e-3, 0.05, 0.8, 20.0])
+def test_pz81_unpolarised_closed_form(oracle, n):
+    o = oracle.xc_points(1, np.array([[n / 2, n / 2, 0.0, 0.0]]))[0]
+    rs = (3 / (4 * np.pi * n)) ** (1 / 3)
+    ex = -0.7385587663820223 * n ** (1 / 3)
+    if rs < 1:
+        ec = 0.0311 * np.log(rs) - 0.048 + 0.002 * rs * np.log(rs) - 0.0116 * rs
+    else:
+        ec = -0.1423 / (1 + 1.0529 * np.sqrt(rs) + 0.3334 * rs)
+    assert abs(o[0] - (ex + ec)) < 1e-13 * abs(o[0])
+
+
+def test_pz81_fully_polarised_limit(oracle):
+    n = 0.2
+    o = oracle.xc_points(1, np.array([[n, 0.0, 0.0, 0.0]]))[0]
+    rs = (3 / (4 * np.pi * n)) ** (1 / 3)
+    ex = -0.7385587663820223 * n ** (1 / 3) * 2 ** (1 / 3)
+    ec = -0.0843 / (1 + 1.3981 * np.sqrt(rs) + 0.2611 * rs)
+    assert abs(o[0] - (ex + ec)) < 1e-13
+
+
+def test_floor_gives_zero(oracle):
+    for fid in (1, 2):
+        o = oracle.xc_points(fid, np.array([[4e-21, 5e-21, 0.1, 0.1]]))[0]
+        assert not np.any(o)
+
+
+def test_pz81_grid_potential_is_pointwise(oracle, goldens):
+    """PZ81 has c_sigma = 0, so the grid assembly reduces to the pointwise potential."""
+    g = grid_from(goldens["xc_grid"])
+    na, nb = goldens["xc_na"], goldens["xc_nb"]
+    pts = np.stack([na, nb, np.zeros_like(na), np.zeros_like(na)], 1)
+    o = oracle.xc_points(1, pts)
+    mask = na + nb > 1e-20
+    np.testing.assert_array_equal(oracle.xc_potential(g, 1, na, nb, 0)[mask], o[mask, 1])
+
+
+def test_density_closed_form_and_charge(oracle):
+    g = Grid.centered(10, 0.4)
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((3, g.size))
+    dv = g.dx * g.dy * g.dz
+    psi /= np.sqrt((psi ** 2).sum(1, keepdims=True) * dv)
+    n = oracle.spin_density(psi, [1, 1, 2])
+    np.testing.assert_allclose(n, psi[0] ** 2 + psi[1] ** 2 + 2 * psi[2] ** 2, rtol=1e-15)
+    assert abs(n.sum() * dv - 4.0) < 1e-12
+
+
+def test_kinetic_neumann_stencil_by_matrix(oracle):
+    g = Grid(3, 3, 3, -1.0, 1.0, -0.5, 0.7, -0.9, 0.9)
+    rng = np.random.default_rng(8)
+    psi = rng.standard_normal(g.size)
+    L = np.zeros((g.size, g.size))
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                c = (k * 3 + j) * 3 + i
+                for (di, dj, dk), h in (((1, 0, 0), g.dx), ((0, 1, 0), g.dy), ((0, 0, 1), g.dz)):
+                    for s in (-1, 1):
+                        ii, jj, kk = i + s * di, j + s * dj, k + s * dk
+                        if 0 <= ii < 3 and 0 <= jj < 3 and 0 <= kk < 3:
+                            L[c, (kk * 3 + jj) * 3 + ii] += 1 / h ** 2
+                            L[c, c] -= 1 / h ** 2
+    t = -0.5 * psi @ L @ psi * g.dx * g.dy * g.dz
+    assert abs(oracle.kinetic_energy(g, psi, [2]) - 2 * t) < 1e-12 * abs(t)
+
+
+def test_nuclear_terms_closed_form(oracle):
+    nuc = np.array([[6, 0.0, 0.0, 0.0], [8, 2.192, 0.0, 0.0], [8, -2.192, 0.0, 0.0]])
+    assert abs(oracle.nuclear_repulsion(nuc) - (48 / 2.192 * 2 + 64 / 4.384)) < 1e-12
+    g = Grid(4, 4, 4, -3.0, 3.1, -3.0, 3.1, -3.0, 3.1)
+    v = oracle.nuclear_potential(g, nuc).reshape(g.shape)
+    xs, ys, zs = g.coords()
+    r = [np.sqrt((xs[1] - a[1]) ** 2 + (ys[2] - a[2]) ** 2 + (zs[3] - a[3]) ** 2) for a in nuc]
+    assert abs(v[3, 2, 1] + sum(a[0] / d for a, d in zip(nuc, r))) < 1e-13
+
+
+def test_nucleus_on_grid_point_fails(oracle):
+    from oracle.oracle import OracleError
+    g = Grid(5, 5, 5, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+    with pytest.raises(OracleError):
+        oracle.nuclear_potential(g, [[1, 0.0, 0.0, 0.0]])
+
+
+def test_davidson_cap_raises(oracle):
+    from oracle.oracle import OracleError
+    g = Grid.centered(8, 0.5)
+    v = oracle.nuclear_potential(g, [[2, 0.1, 0.0, 0.0]])
+    with pytest.raises(OracleError):
+        oracle.davidson(g, v, 2, max_happly=3)
+
+
+def test_davidson_nan_guess_falls_back_to_seed(oracle):
+    g = Grid.centered(8, 0.5)
+    v = oracle.nuclear_potential(g, [[2, 0.1, 0.0, 0.0]])
+    e0, _, _ = oracle.davidson(g, v, 2)
+    init = np.full((2, g.size), np.nan)
+    e1, _, _ = oracle.davidson(g, v, 2, init=init)
+    np.testing.assert_allclose(e1, e0, atol=1e-10)
+
+
+# ------------------------------------------------ against the live reference
+needs_ref = pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built here")
+
+
+@needs_ref
+def test_against_live_reference_random_grid(oracle):
+    R = Reference()
+    rng = np.random.default_rng(11)
+    g = Grid(9, 7, 8, -2.3, 2.0, -1.1, 2.6, -2.0, 1.9)
+    v, psi = rng.standard_normal(g.size), rng.standard_normal(g.size)
+    np.testing.assert_array_equal(oracle.hpsi(g, v, psi), R.hpsi(g, v, psi))
+    rho = np.abs(rng.standard_normal(g.size))
+    np.testing.assert_array_equal(oracle.poisson(g, rho)[0], R.poisson(g, rho))
+    na, nb = np.exp(-3 * rng.random(g.size)), 0.4 * np.exp(-3 * rng.random(g.size))
+    for fid in (1, 2):
+        for s in (0, 1):
+            np.testing.assert_array_equal(oracle.xc_potential(g, fid, na, nb, s), R.xc_potential(g, fid, na, nb, s))
