@@ -1,0 +1,367 @@
+"""dft_amd -- MI355X-native 3D real-space Kohn-Sham hot path.
+
+The product is native: libdft_hip.so (gfx950 kernels behind the C-ABI in
+include/dft_hip.h) and libdft_core.so (the C++ layer keeping the reference's
+solver interfaces, include/dft_molecule.h and include/dft_core.h).  This
+module loads them through ctypes for tests and benchmarks; it raises if the
+native libraries are missing -- there is no Python or CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+HIP_SO = os.path.join(LIBDIR, "libdft_hip.so")
+CORE_SO = os.path.join(LIBDIR, "libdft_core.so")
+
+F64, F32 = 0, 1
+GRAM_F64, GRAM_F32, GRAM_SPLIT_BF16 = 0, 1, 2
+PZ81, PBE96 = 1, 2
+RED_SUM, RED_DOT, RED_MAXABS, RED_DIFF2 = 0, 1, 2, 3
+
+
+class NativeMissing(RuntimeError):
+    pass
+
+
+class DftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    """dft_grid: CartesianGrid as plain data (CartesianGrid.cpp:5-48)."""
+
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64)] + [
+        (k, C.c_double) for k in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "dx", "dy", "dz")
+    ]
+
+    @classmethod
+    def make(cls, nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax):
+        g = cls()
+        rc = hip_lib().dft_grid_init(C.byref(g), nx, ny, nz, *[C.c_double(v) for v in (xmin, xmax, ymin, ymax, zmin, zmax)])
+        if rc != 0:
+            raise DftError(rc, "invalid grid")
+        return g
+
+    @classmethod
+    def centered(cls, n, h):
+        half = (n - 1) * h / 2.0
+        return cls.make(n, n, n, -half, half, -half, half, -half, half)
+
+    @property
+    def size(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def bounds(self):
+        return (self.xmin, self.xmax, self.ymin, self.ymax, self.zmin, self.zmax)
+
+
+_hip = None
+_core = None
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise NativeMissing(f"{path} is not built; run __graft_entry__.build() (dft_amd/build.py)")
+    return C.CDLL(path)
+
+
+def hip_lib():
+    global _hip
+    if _hip is None:
+        L = _load(HIP_SO)
+        L.dft_last_error.restype = C.c_char_p
+        L.dft_stream.restype = C.c_void_p
+        _hip = L
+    return _hip
+
+
+def core_lib():
+    global _core
+    if _core is None:
+        hip_lib()
+        L = _load(CORE_SO)
+        _core = L
+    return _core
+
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+
+def _prec_of(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return F64
+    if dt == np.float32:
+        return F32
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+class DeviceArray:
+    """A device allocation owned by a Device, with shape/dtype bookkeeping."""
+
+    def __init__(self, dev, shape, dtype):
+        self.dev = dev
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        dev._chk(dev.L.dft_alloc(dev.ctx, C.c_size_t(max(self.nbytes, 1)), C.byref(p)))
+        self.ptr = p.value
+
+    @property
+    def prec(self):
+        return _prec_of(self.dtype)
+
+    def col(self, j, ld=None):
+        """Device address of column j of a column-major block stored as (ncols, ld)."""
+        ld = ld if ld is not None else self.shape[-1]
+        return self.ptr + j * ld * self.dtype.itemsize
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        self.dev._chk(self.dev.L.dft_upload(self.dev.ctx, _vp(self.ptr), a.ctypes.data_as(_vp), C.c_size_t(a.nbytes)))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.dev._chk(self.dev.L.dft_download(self.dev.ctx, out.ctypes.data_as(_vp), _vp(self.ptr), C.c_size_t(self.nbytes)))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.dev.L.dft_free(self.dev.ctx, _vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Device:
+    """One dft_ctx (a HIP stream and its workspaces) on one GPU."""
+
+    def __init__(self, device=0):
+        self.L = hip_lib()
+        ctx = C.c_void_p()
+        rc = self.L.dft_ctx_create(C.c_int(device), C.byref(ctx))
+        if rc != 0:
+            raise DftError(rc, "dft_ctx_create failed (no GPU?)")
+        self.ctx = ctx
+
+    def close(self):
+        if self.ctx:
+            self.L.dft_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise DftError(rc, self.L.dft_last_error(self.ctx).decode())
+
+    def array(self, host=None, shape=None, dtype=np.float64):
+        if host is not None:
+            host = np.asarray(host)
+            a = DeviceArray(self, host.shape, dtype if dtype is not None else host.dtype)
+            return a.upload(host)
+        return DeviceArray(self, shape, dtype)
+
+    def zeros(self, shape, dtype=np.float64):
+        a = DeviceArray(self, shape, dtype)
+        self._chk(self.L.dft_zero(self.ctx, _vp(a.ptr), C.c_size_t(a.nbytes)))
+        return a
+
+    def sync(self):
+        self._chk(self.L.dft_sync(self.ctx))
+
+    # ---- kernels -------------------------------------------------------
+    def hpsi(self, g, veff, psi, out, nb=None, ld=None, ldo=None):
+        n = g.size
+        nb = nb if nb is not None else (psi.shape[0] if len(psi.shape) > 1 else 1)
+        self._chk(self.L.dft_hpsi(self.ctx, C.byref(g), C.c_int(psi.prec), _vp(veff.ptr), _vp(psi.ptr),
+                                  C.c_int64(ld or n), C.c_int(nb), _vp(out.ptr), C.c_int64(ldo or n)))
+
+    def kinetic_energy(self, g, psi, electrons, nb=None):
+        nb = nb if nb is not None else (psi.shape[0] if len(psi.shape) > 1 else 1)
+        el = np.ascontiguousarray(electrons, dtype=np.int32)
+        t = C.c_double()
+        self._chk(self.L.dft_kinetic_energy(self.ctx, C.byref(g), C.c_int(psi.prec), _vp(psi.ptr), C.c_int64(g.size),
+                                            C.c_int(nb), el.ctypes.data_as(C.POINTER(C.c_int)), C.byref(t)))
+        return t.value
+
+    def poisson(self, g, rho, v, max_sweeps=10000, tol=1e-8, zero_start=True, batch=64):
+        sw = C.c_int64()
+        md = C.c_double()
+        self._chk(self.L.dft_poisson(self.ctx, C.byref(g), C.c_int(rho.prec), _vp(rho.ptr), _vp(v.ptr),
+                                     C.c_int64(max_sweeps), C.c_double(tol), C.c_int(1 if zero_start else 0),
+                                     C.c_int(batch), C.byref(sw), C.byref(md)))
+        return sw.value, md.value
+
+    def hartree_energy(self, g, rho, vh):
+        e = C.c_double()
+        self._chk(self.L.dft_hartree_energy(self.ctx, C.byref(g), C.c_int(rho.prec), _vp(rho.ptr), _vp(vh.ptr), C.byref(e)))
+        return e.value
+
+    def xc(self, g, functional, na, nb, va, vb=None, energy=True):
+        e = C.c_double()
+        self._chk(self.L.dft_xc(self.ctx, C.byref(g), C.c_int(na.prec), C.c_int(functional), _vp(na.ptr), _vp(nb.ptr),
+                                _vp(va.ptr), _vp(vb.ptr if vb is not None else None),
+                                C.byref(e) if energy else None))
+        return e.value if energy else None
+
+    def xc_points(self, functional, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
+        din = self.array(pts)
+        dout = self.array(shape=(pts.shape[0], 5))
+        self._chk(self.L.dft_xc_points(self.ctx, C.c_int(functional), _vp(din.ptr), C.c_int64(pts.shape[0]), _vp(dout.ptr)))
+        return dout.download()
+
+    def density(self, psi, occ, out, accumulate=False, nb=None, n=None):
+        nb = nb if nb is not None else psi.shape[0]
+        n = n if n is not None else psi.shape[-1]
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        self._chk(self.L.dft_density(self.ctx, C.c_int(psi.prec), _vp(psi.ptr), C.c_int64(n), C.c_int(nb),
+                                     occ.ctypes.data_as(_dp), C.c_int64(n), _vp(out.ptr), C.c_int(1 if accumulate else 0)))
+
+    def reduce(self, op, a, b=None, n=None):
+        r = C.c_double()
+        n = n if n is not None else int(np.prod(a.shape))
+        self._chk(self.L.dft_reduce(self.ctx, C.c_int(op), C.c_int(a.prec), _vp(a.ptr), _vp(b.ptr if b else None),
+                                    C.c_int64(n), C.byref(r)))
+        return r.value
+
+    def nuclear_potential(self, g, nuclei, out):
+        nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
+        self._chk(self.L.dft_nuclear_potential(self.ctx, C.byref(g), C.c_int(out.prec), C.c_int(nuc.shape[0]),
+                                               nuc.ctypes.data_as(_dp), _vp(out.ptr)))
+
+    def gram(self, mode, a, b, out, n, ma, mb, alpha=1.0, beta=0.0, lda=None, ldb=None, ldo=None):
+        self._chk(self.L.dft_gram(self.ctx, C.c_int(mode), C.c_int64(n), C.c_int(ma), C.c_int(mb), C.c_double(alpha),
+                                  _vp(a.ptr), C.c_int64(lda or n), _vp(b.ptr), C.c_int64(ldb or n), C.c_double(beta),
+                                  _vp(out.ptr), C.c_int64(ldo or ma)))
+
+    def block_update(self, a, cmat, y, n, m, k, alpha=1.0, beta=0.0, lda=None, ldc=None, ldy=None):
+        self._chk(self.L.dft_block_update(self.ctx, C.c_int(a.prec), C.c_int64(n), C.c_int(m), C.c_int(k),
+                                          C.c_double(alpha), _vp(a.ptr), C.c_int64(lda or n), _vp(cmat.ptr),
+                                          C.c_int64(ldc or m), C.c_double(beta), _vp(y.ptr), C.c_int64(ldy or n)))
+
+    def syev(self, a, m):
+        w = self.array(shape=(m,))
+        self._chk(self.L.dft_syev(self.ctx, C.c_int(m), _vp(a.ptr), C.c_int64(m), _vp(w.ptr)))
+        return w
+
+    def chol_inv(self, a, m):
+        self._chk(self.L.dft_chol_inv(self.ctx, C.c_int(m), _vp(a.ptr), C.c_int64(m)))
+
+    def precond_diag(self, g, veff, r, eps, t, nb):
+        self._chk(self.L.dft_precond_diag(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
+                                          C.c_int64(g.size), C.c_int(nb), _vp(eps.ptr), _vp(t.ptr), C.c_int64(g.size)))
+
+    # ---- timing on the context stream ---------------------------------
+    def event(self):
+        e = C.c_void_p()
+        self._chk(self.L.dft_event_create(self.ctx, C.byref(e)))
+        return e
+
+    def record(self, ev):
+        self._chk(self.L.dft_event_record(self.ctx, ev))
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        self._chk(self.L.dft_event_elapsed_ms(self.ctx, a, b, C.byref(ms)))
+        return ms.value
+
+
+# ---------------------------------------------------------------- core layer
+SCF_FP64, SCF_MIXED = 0, 1
+STOP_REFERENCE, STOP_DE = 0, 1
+INIT_BARE, INIT_HYDROGENIC = 0, 1
+PRECOND_DIAG, PRECOND_CNN = 0, 1
+
+
+class ScfConfig(C.Structure):
+    """dft_scf_config (include/dft_core.h)."""
+
+    _fields_ = [("functional", C.c_int), ("charge", C.c_int), ("multiplicity", C.c_int), ("precision", C.c_int),
+                ("nstates", C.c_int), ("max_subspace", C.c_int), ("max_iterations", C.c_int),
+                ("poisson_fixed", C.c_int), ("poisson_batch", C.c_int), ("stop_rule", C.c_int),
+                ("energy_tol", C.c_double), ("init_guess", C.c_int), ("davidson_tol", C.c_double),
+                ("davidson_max_iter", C.c_int), ("refine_iterations", C.c_int), ("precond", C.c_int),
+                ("cnn", C.c_void_p), ("verbose", C.c_int)]
+
+
+class ScfReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("first_de_iteration", C.c_int),
+                ("refine_iterations", C.c_int), ("energy", C.c_double * 6), ("energy_difference", C.c_double),
+                ("density_difference", C.c_double), ("ks_residual", C.c_double), ("seconds_total", C.c_double),
+                ("seconds_to_first_de", C.c_double), ("seconds_poisson", C.c_double), ("seconds_xc", C.c_double),
+                ("seconds_davidson", C.c_double), ("seconds_other", C.c_double), ("happly_columns", C.c_int64),
+                ("poisson_sweeps", C.c_int64), ("davidson_iterations", C.c_int64), ("cnn_applied", C.c_int64),
+                ("cnn_fallbacks", C.c_int64)]
+
+
+def _core_err(rc):
+    L = core_lib()
+    L.dft_core_last_error.restype = C.c_char_p
+    raise DftError(rc, L.dft_core_last_error().decode())
+
+
+def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=SCF_FP64, nstates=0, max_subspace=0,
+            max_iterations=250, poisson_fixed=0, poisson_batch=64, stop_rule=STOP_REFERENCE, energy_tol=1e-5,
+            init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
+            precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False):
+    """Run the device-resident SCF; returns a dict of results."""
+    L = core_lib()
+    nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
+    cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
+                    poisson_fixed, poisson_batch, stop_rule, energy_tol, init_guess, davidson_tol,
+                    davidson_max_iter, refine_iterations, precond, cnn, 1 if verbose else 0)
+    rep = ScfReport()
+    ne = int(round(nuc[:, 0].sum())) - charge
+    eig = np.zeros(ne + 2)
+    cap = max_iterations + max(refine_iterations, 0) + 1
+    trace = np.zeros((cap, 5))
+    rho = np.zeros(2 * grid.size) if want_density else None
+    rc = L.dft_scf_run(C.byref(grid), C.c_int(nuc.shape[0]), nuc.ctypes.data_as(_dp), C.byref(cfg), C.byref(rep),
+                       eig.ctypes.data_as(_dp), trace.ctypes.data_as(_dp), C.c_int(cap),
+                       None if rho is None else rho.ctypes.data_as(_dp))
+    if rc != 0:
+        _core_err(rc)
+    it = rep.iterations
+    out = {k: getattr(rep, k) for k, _ in ScfReport._fields_ if k != "energy"}
+    out["energy"] = list(rep.energy)
+    out["total"] = rep.energy[5]
+    out["converged"] = bool(rep.converged)
+    out["trace"] = trace[:it].copy()
+    na = (ne + multiplicity - 1) // 2
+    nb = ne - na
+    out["eigenvalues"] = eig[: na + nb].copy()
+    if rho is not None:
+        out["rho_alpha"], out["rho_beta"] = rho[: grid.size], rho[grid.size:]
+    return out
+
+
+def davidson_run(grid, veff, nstates, nconv=None, prec=F64, gram_mode=GRAM_F64, max_subspace=0, tol=1e-7,
+                 max_iter=0, want_psi=False):
+    L = core_lib()
+    v = np.ascontiguousarray(veff, dtype=np.float64)
+    nconv = nconv or nstates
+    eig = np.zeros(nstates)
+    res = np.zeros(nstates)
+    its = C.c_int()
+    psi = np.zeros((nconv, grid.size)) if want_psi else None
+    rc = L.dft_davidson_run(C.byref(grid), C.c_int(prec), C.c_int(gram_mode), v.ctypes.data_as(_dp), C.c_int(nstates),
+                            C.c_int(nconv), C.c_int(max_subspace), C.c_double(tol), C.c_int(max_iter),
+                            eig.ctypes.data_as(_dp), None if psi is None else psi.ctypes.data_as(_dp),
+                            res.ctypes.data_as(_dp), C.byref(its))
+    if rc != 0:
+        _core_err(rc)
+    return dict(eigenvalues=eig, residuals=res, iterations=its.value, psi=psi)

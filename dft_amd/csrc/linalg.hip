@@ -1,0 +1,94 @@
+// linalg.hip -- dense FP64 subspace algebra on the device for the block
+// Davidson (Rayleigh-Ritz eigensolve and Cholesky orthonormalisation of
+// m x m matrices, m <= a few hundred), through rocSOLVER on the context
+// stream.
+#include "common.hpp"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+using namespace dft;
+
+namespace {
+// a (upper, i <= j) <- tmp(j, i) from the lower triangle of tmp; zeros below
+__global__ __launch_bounds__(256) void k_lower_t(int m, const double* __restrict__ tmp, double* __restrict__ a, int64_t lda) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * m) return;
+    const int i = (int)(e % m), j = (int)(e / m);
+    a[(int64_t)j * lda + i] = i <= j ? tmp[(int64_t)i * m + j] : 0.0;
+}
+__global__ __launch_bounds__(256) void k_copy_sq(int m, const double* __restrict__ a, int64_t lda, double* __restrict__ t) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * m) return;
+    const int i = (int)(e % m), j = (int)(e / m);
+    t[(int64_t)j * m + i] = a[(int64_t)j * lda + i];
+}
+}  // namespace
+
+namespace dft {
+
+void release_blas(dft_ctx* c) {
+    if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
+    if (c->dev_info) hipFree(c->dev_info);
+    c->blas = nullptr;
+    c->dev_info = nullptr;
+}
+
+static int blas(dft_ctx* c, rocblas_handle* h) {
+    if (!c->blas) {
+        rocblas_handle hh;
+        if (rocblas_create_handle(&hh) != rocblas_status_success) return set_err(c, DFT_EHIP, "rocblas_create_handle failed");
+        c->blas = (void*)hh;
+        DFT_HIP(c, hipMalloc((void**)&c->dev_info, 4 * sizeof(int)));
+    }
+    if (rocblas_set_stream((rocblas_handle)c->blas, c->stream) != rocblas_status_success)
+        return set_err(c, DFT_EHIP, "rocblas_set_stream failed");
+    *h = (rocblas_handle)c->blas;
+    return DFT_OK;
+}
+
+}  // namespace dft
+
+extern "C" {
+
+int dft_syev(dft_ctx* c, int m, double* a, int64_t lda, double* w) {
+    DFT_REQUIRE(c, a && w && m > 0 && lda >= m && m <= 4096, "dft_syev: bad arguments");
+    rocblas_handle h;
+    if (int rc = blas(c, &h)) return rc;
+    if (int rc = ensure_scratch(c, sizeof(double) * (size_t)m)) return rc;
+    rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, m, a, (rocblas_int)lda, w,
+                                         (double*)c->scratch, c->dev_info);
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "rocsolver_dsyevd failed");
+    int info = 0;
+    DFT_HIP(c, hipMemcpyAsync(&info, c->dev_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    DFT_HIP(c, hipStreamSynchronize(c->stream));
+    if (info != 0) return set_err(c, DFT_ENUMERIC, "dft_syev: eigensolver did not converge");
+    return DFT_OK;
+}
+
+int dft_chol_inv(dft_ctx* c, int m, double* a, int64_t lda) {
+    DFT_REQUIRE(c, a && m > 0 && lda >= m && m <= 4096, "dft_chol_inv: bad arguments");
+    rocblas_handle h;
+    if (int rc = blas(c, &h)) return rc;
+    rocblas_status st = rocsolver_dpotrf(h, rocblas_fill_lower, m, a, (rocblas_int)lda, c->dev_info);
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "rocsolver_dpotrf failed");
+    int info = 0;
+    DFT_HIP(c, hipMemcpyAsync(&info, c->dev_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    DFT_HIP(c, hipStreamSynchronize(c->stream));
+    if (info != 0) return set_err(c, DFT_ENUMERIC, "dft_chol_inv: matrix is not positive definite");
+    st = rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, m, a, (rocblas_int)lda, c->dev_info);
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "rocsolver_dtrtri failed");
+    DFT_HIP(c, hipMemcpyAsync(&info, c->dev_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    DFT_HIP(c, hipStreamSynchronize(c->stream));
+    if (info != 0) return set_err(c, DFT_ENUMERIC, "dft_chol_inv: singular triangular factor");
+    // a <- (L^{-1})^T: upper triangular, zero below the diagonal
+    if (int rc = ensure_scratch(c, sizeof(double) * (size_t)m * m)) return rc;
+    double* t = (double*)c->scratch;
+    const unsigned gb = (unsigned)(((int64_t)m * m + 255) / 256);
+    k_copy_sq<<<gb, 256, 0, c->stream>>>(m, a, lda, t);
+    k_lower_t<<<gb, 256, 0, c->stream>>>(m, t, a, lda);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+}  // extern "C"

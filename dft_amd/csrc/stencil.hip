@@ -1,0 +1,263 @@
+// stencil.hip -- (a) the 7-point Kohn-Sham operator on blocks of orbitals,
+// the reference's Neumann-stencil kinetic energy, and the Davidson diagonal
+// preconditioner.
+//
+// H psi layout: a workgroup owns a 64 (x) x 4*TY (y) column of the grid over
+// a z-chunk and streams through it plane by plane.  Lane = x, so a wave is
+// one unit-stride 64-point row segment (coalesced); each lane keeps TY
+// consecutive y-rows and the z-1 / z / z+1 planes of NB orbitals in
+// registers.  x-neighbours come from the adjacent lane by a DPP wave shift
+// (no LDS, no extra memory traffic), y-neighbours from the lane's own rows
+// plus one halo row above and below, z-neighbours from the register queue.
+// V_eff is loaded once per plane and reused by the NB orbitals of the group.
+#include "common.hpp"
+
+#include <algorithm>
+#include <vector>
+
+using namespace dft;
+
+namespace {
+
+// lane i <- lane i-1 (wave_shr:1); lane 0 <- fill
+__device__ inline float shr1(float v, float fill) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+// lane i <- lane i+1 (wave_shl:1); lane 63 <- fill
+__device__ inline float shl1(float v, float fill) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ inline double shr1(double v, double fill) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double shl1(double v, double fill) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <typename T>
+struct Coef {
+    T cx, cy, cz, cc;  // 1/dx^2, 1/dy^2, 1/dz^2, -2(1/dx^2+1/dy^2+1/dz^2)
+};
+
+template <typename T, int TY, int NB>
+__global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, int nzc, Coef<T> co,
+                                              const T* __restrict__ V, const T* __restrict__ psi, int64_t ld,
+                                              T* __restrict__ out, int64_t ldo) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 64, x = x0 + lane;
+    const int ybase = blockIdx.y * (4 * TY) + w * TY;
+    const int chunk = blockIdx.z % nzc, grp = blockIdx.z / nzc;
+    const int k0 = chunk * zc, k1 = min(nz, k0 + zc);
+    const int64_t plane = (int64_t)nx * ny;
+    const bool xin = x < nx;
+    const bool has_xm = x0 > 0, has_xp = x0 + 64 < nx;
+    const T* P[NB];
+    T* O[NB];
+#pragma unroll
+    for (int o = 0; o < NB; ++o) {
+        P[o] = psi + (int64_t)(grp * NB + o) * ld;
+        O[o] = out + (int64_t)(grp * NB + o) * ldo;
+    }
+    bool yin[TY];
+    int64_t rowoff[TY];
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+        yin[r] = (ybase + r) < ny;
+        rowoff[r] = (int64_t)(ybase + r) * nx;
+    }
+    const bool ym_in = ybase > 0 && ybase - 1 < ny, yp_in = ybase + TY < ny;
+    const int64_t ymoff = (int64_t)(ybase - 1) * nx, ypoff = (int64_t)(ybase + TY) * nx;
+
+    T prev[NB][TY], cur[NB][TY], nxt[NB][TY];
+#pragma unroll
+    for (int o = 0; o < NB; ++o)
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            const bool ok = xin && yin[r];
+            prev[o][r] = (ok && k0 > 0) ? P[o][(int64_t)(k0 - 1) * plane + rowoff[r] + x] : T(0);
+            cur[o][r] = ok ? P[o][(int64_t)k0 * plane + rowoff[r] + x] : T(0);
+        }
+
+    for (int k = k0; k < k1; ++k) {
+        const int64_t pk = (int64_t)k * plane;
+        const bool has_next = k + 1 < nz;
+        T v[TY];
+#pragma unroll
+        for (int r = 0; r < TY; ++r) v[r] = (xin && yin[r]) ? V[pk + rowoff[r] + x] : T(0);
+#pragma unroll
+        for (int o = 0; o < NB; ++o) {
+            const T* p = P[o] + pk;
+#pragma unroll
+            for (int r = 0; r < TY; ++r) nxt[o][r] = (xin && yin[r] && has_next) ? p[plane + rowoff[r] + x] : T(0);
+            const T ym = (xin && ym_in) ? p[ymoff + x] : T(0);
+            const T yp = (xin && yp_in) ? p[ypoff + x] : T(0);
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                T hm = T(0), hp = T(0);
+                if (lane == 0 && has_xm && yin[r]) hm = p[rowoff[r] + x0 - 1];
+                if (lane == 63 && has_xp && yin[r]) hp = p[rowoff[r] + x0 + 64];
+                const T c = cur[o][r];
+                const T xm = shr1(c, hm);
+                const T xp = shl1(c, hp);
+                const T ymv = r > 0 ? cur[o][r > 0 ? r - 1 : 0] : ym;
+                const T ypv = r < TY - 1 ? cur[o][r < TY - 1 ? r + 1 : 0] : yp;
+                // KohnShamHamiltonianMolecule.cpp:88-187 order: centre, x-, x+, y-, y+, z-, z+
+                T lap = co.cc * c;
+                lap += xm * co.cx;
+                lap += xp * co.cx;
+                lap += ymv * co.cy;
+                lap += ypv * co.cy;
+                lap += prev[o][r] * co.cz;
+                lap += nxt[o][r] * co.cz;
+                const T kin = T(-0.5) * lap;
+                const T pot = v[r] * c;
+                if (xin && yin[r]) O[o][pk + rowoff[r] + x] = kin + pot;
+            }
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
+                prev[o][r] = cur[o][r];
+                cur[o][r] = nxt[o][r];
+            }
+        }
+    }
+}
+
+template <typename T, int TY, int NB>
+int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int ngroups, T* out,
+                int64_t ldo) {
+    const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
+    Coef<T> co;
+    const double ix2 = 1.0 / (g->dx * g->dx), iy2 = 1.0 / (g->dy * g->dy), iz2 = 1.0 / (g->dz * g->dz);
+    co.cx = (T)ix2;
+    co.cy = (T)iy2;
+    co.cz = (T)iz2;
+    co.cc = (T)(-2.0 * (ix2 + iy2 + iz2));
+    const int gx = (nx + 63) / 64, gy = (ny + 4 * TY - 1) / (4 * TY);
+    const int64_t base = (int64_t)gx * gy * ngroups;
+    const int64_t want = 8LL * c->num_cus;
+    int nzc = (int)std::max<int64_t>(1, std::min<int64_t>((want + base - 1) / base, std::max(1, nz / 8)));
+    const int zc = (nz + nzc - 1) / nzc;
+    nzc = (nz + zc - 1) / zc;
+    DFT_REQUIRE(c, (int64_t)nzc * ngroups <= 65535 && gy <= 65535, "dft_hpsi: launch grid too large");
+    dim3 grid(gx, gy, nzc * ngroups);
+    k_hpsi<T, TY, NB><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+template <typename T>
+int hpsi_t(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int nb, T* out, int64_t ldo) {
+    constexpr int TY = 4;
+    const int n4 = nb / 4;
+    if (n4 > 0)
+        if (int rc = launch_hpsi<T, TY, 4>(c, g, V, psi, ld, n4, out, ldo)) return rc;
+    for (int o = n4 * 4; o < nb; ++o)
+        if (int rc = launch_hpsi<T, TY, 1>(c, g, V, psi + (int64_t)o * ld, ld, 1, out + (int64_t)o * ldo, ldo)) return rc;
+    return DFT_OK;
+}
+
+// Kinetic energy with the reference's face rule (TotalEnergyMolecule.cpp:80-186):
+// only existing neighbours contribute (psi_nb - psi)/h^2.
+template <typename T>
+__global__ __launch_bounds__(256) void k_kinetic(int nx, int ny, int nz, double dx2, double dy2, double dz2,
+                                                 const T* __restrict__ psi, int64_t ld, int nb,
+                                                 const double* __restrict__ el, double* __restrict__ partials) {
+    __shared__ double lds[4];
+    const int64_t n = (int64_t)nx * ny * nz, plane = (int64_t)nx * ny;
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += stride) {
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / plane);
+        for (int o = 0; o < nb; ++o) {
+            if (el[o] <= 0.0) continue;
+            const T* q = psi + (int64_t)o * ld;
+            const double c = (double)q[p];
+            double lap = 0.0;
+            if (i > 0) lap += ((double)q[p - 1] - c) / dx2;
+            if (i + 1 < nx) lap += ((double)q[p + 1] - c) / dx2;
+            if (j > 0) lap += ((double)q[p - nx] - c) / dy2;
+            if (j + 1 < ny) lap += ((double)q[p + nx] - c) / dy2;
+            if (k > 0) lap += ((double)q[p - plane] - c) / dz2;
+            if (k + 1 < nz) lap += ((double)q[p + plane] - c) / dz2;
+            s += el[o] * (c * (-0.5 * lap));
+        }
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_precond_diag(int64_t n, double kin, const T* __restrict__ V,
+                                                      const T* __restrict__ r, int64_t ldr,
+                                                      const double* __restrict__ eps, T* __restrict__ t,
+                                                      int64_t ldt) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int o = blockIdx.y;
+    if (p >= n) return;
+    // EigenvalueSolverMolecule.cpp:500-554
+    const double d = eps[o] - (kin + (double)V[p]);
+    const double rv = (double)r[(int64_t)o * ldr + p];
+    t[(int64_t)o * ldt + p] = (T)((!isfinite(d) || fabs(d) < 1.0e-10) ? rv : rv / d);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dft_hpsi(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* psi, int64_t ld, int nb, void* out,
+             int64_t ldo) {
+    DFT_REQUIRE(c, valid_grid(g), "dft_hpsi: invalid grid");
+    const int64_t n = grid_points(g);
+    DFT_REQUIRE(c, veff && psi && out && nb >= 0 && ld >= n && ldo >= n, "dft_hpsi: bad arguments");
+    DFT_REQUIRE(c, g->ny < (1 << 30) && g->nx < (1 << 30), "dft_hpsi: grid too large");
+    if (nb == 0) return DFT_OK;
+    if (prec == DFT_F64) return hpsi_t<double>(c, g, (const double*)veff, (const double*)psi, ld, nb, (double*)out, ldo);
+    if (prec == DFT_F32) return hpsi_t<float>(c, g, (const float*)veff, (const float*)psi, ld, nb, (float*)out, ldo);
+    return set_err(c, DFT_EINVAL, "dft_hpsi: bad precision");
+}
+
+int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
+                       const int* electrons_host, double* t_host) {
+    DFT_REQUIRE(c, valid_grid(g) && psi && t_host && nb >= 0 && (nb == 0 || electrons_host), "dft_kinetic_energy: bad arguments");
+    const int64_t n = grid_points(g);
+    DFT_REQUIRE(c, ld >= n, "dft_kinetic_energy: ld < grid size");
+    if (nb == 0) { *t_host = 0.0; return DFT_OK; }
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4 * (int64_t)c->num_cus);
+    if (int rc = ensure_partials(c, (size_t)blocks + 1 + (size_t)nb)) return rc;
+    double* el = c->partials + blocks + 1;
+    std::vector<double> elh(nb);
+    for (int o = 0; o < nb; ++o) elh[o] = (double)electrons_host[o];
+    DFT_HIP(c, hipMemcpyAsync(el, elh.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    const double dx2 = g->dx * g->dx, dy2 = g->dy * g->dy, dz2 = g->dz * g->dz;
+    if (prec == DFT_F64)
+        k_kinetic<double><<<blocks, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const double*)psi, ld, nb, el, c->partials);
+    else
+        k_kinetic<float><<<blocks, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const float*)psi, ld, nb, el, c->partials);
+    DFT_CHECK_LAUNCH(c);
+    double s = 0.0;
+    if (int rc = finish_sum(c, blocks, &s)) return rc;
+    *t_host = s * (g->dx * g->dy * g->dz);
+    return DFT_OK;
+}
+
+int dft_precond_diag(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr, int nb,
+                     const double* eps, void* t, int64_t ldt) {
+    DFT_REQUIRE(c, valid_grid(g) && veff && r && eps && t && nb >= 0 && nb < 65536, "dft_precond_diag: bad arguments");
+    const int64_t n = grid_points(g);
+    DFT_REQUIRE(c, ldr >= n && ldt >= n, "dft_precond_diag: leading dimension < grid size");
+    if (nb == 0) return DFT_OK;
+    const double kin = 1.0 / (g->dx * g->dx) + 1.0 / (g->dy * g->dy) + 1.0 / (g->dz * g->dz);
+    dim3 grid((unsigned)((n + 255) / 256), nb);
+    if (prec == DFT_F64)
+        k_precond_diag<double><<<grid, 256, 0, c->stream>>>(n, kin, (const double*)veff, (const double*)r, ldr, eps, (double*)t, ldt);
+    else
+        k_precond_diag<float><<<grid, 256, 0, c->stream>>>(n, kin, (const float*)veff, (const float*)r, ldr, eps, (float*)t, ldt);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+}  // extern "C"

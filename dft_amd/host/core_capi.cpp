@@ -1,0 +1,88 @@
+// core_capi.cpp -- extern "C" entry points of libdft_core.so (dft_core.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "davidson.hpp"
+#include "dft_core.h"
+#include "scf.hpp"
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return DFT_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return DFT_EINVAL;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return DFT_ENUMERIC;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* dft_core_last_error(void) { return g_err.c_str(); }
+
+int dft_scf_run(const dft_grid* g, int natoms, const double* nuclei, const dft_scf_config* cfg, dft_scf_report* rep,
+                double* eig, double* trace, int trace_cap, double* rho) {
+    return guarded([&] {
+        if (!g || !cfg || !rep || (natoms > 0 && !nuclei)) throw std::invalid_argument("dft_scf_run: null argument");
+        dftc::Ctx ctx(0);
+        dftc::ScfDriver drv(ctx, *g, natoms, nuclei, *cfg);
+        dftc::ScfOutputs out;
+        out.want_rho = rho != nullptr;
+        drv.run(*rep, out);
+        if (eig) std::copy(out.eig.begin(), out.eig.end(), eig);
+        if (trace && trace_cap > 0) {
+            const size_t nv = std::min(out.trace.size(), (size_t)trace_cap * 5);
+            std::copy(out.trace.begin(), out.trace.begin() + nv, trace);
+        }
+        if (rho) std::copy(out.rho.begin(), out.rho.end(), rho);
+    });
+}
+
+int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
+                     int max_subspace, double tol, int max_iter, double* eig, double* psi, double* resid,
+                     int* iterations) {
+    return guarded([&] {
+        if (!g || !veff_host || !eig || nstates <= 0) throw std::invalid_argument("dft_davidson_run: bad arguments");
+        dftc::Ctx ctx(0);
+        const int64_t n = g->nx * g->ny * g->nz;
+        dftc::Buf v64(ctx, (size_t)n * 8), v(ctx, (size_t)n * dftc::esize(prec));
+        v64.upload(veff_host, (size_t)n * 8);
+        dftc::check(ctx, dft_convert(ctx, v64.get(), DFT_F64, v.get(), prec, n), "dft_convert");
+        dftc::BlockDavidson D(ctx, *g, prec, nstates, max_subspace > 0 ? max_subspace : 3 * nstates);
+        dftc::DiagonalPreconditioner P;
+        dftc::DavidsonOptions o;
+        o.nconv = nconv > 0 ? nconv : nstates;
+        o.tol = tol > 0 ? tol : 1.0e-7;
+        o.max_iter = max_iter > 0 ? max_iter : 200;
+        o.gram_mode = gram_mode;
+        dftc::DavidsonStats st = D.solve(v.get(), false, o, P);
+        std::copy(D.eigenvalues().begin(), D.eigenvalues().end(), eig);
+        if (resid) std::copy(D.residuals().begin(), D.residuals().end(), resid);
+        if (iterations) *iterations = st.iterations;
+        if (psi) {
+            // reference normalisation: sum psi^2 dV = 1
+            const int nc = std::min(o.nconv, nstates);
+            dftc::Buf out(ctx, (size_t)n * nc * 8);
+            dftc::check(ctx, dft_convert(ctx, D.X(), prec, out.get(), DFT_F64, n * (int64_t)nc), "dft_convert");
+            out.download(psi, (size_t)n * nc * 8);
+            const double s = 1.0 / std::sqrt(g->dx * g->dy * g->dz);
+            for (int64_t i = 0; i < n * nc; ++i) psi[i] *= s;
+        }
+        if (!st.converged) throw std::runtime_error("Davidson did not converge");
+    });
+}
+
+}  // extern "C"

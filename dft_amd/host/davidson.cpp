@@ -1,0 +1,164 @@
+// davidson.cpp -- block Davidson (see davidson.hpp).
+#include "davidson.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+namespace dftc {
+
+BlockDavidson::BlockDavidson(dft_ctx* c, const dft_grid& g, int prec, int nstates, int max_subspace)
+    : c_(c), g_(g), prec_(prec), k_(nstates), mmax_(max_subspace) {
+    if (k_ <= 0) throw std::invalid_argument("El numero de orbitales debe ser mayor que cero.");
+    if (mmax_ < 2 * k_) mmax_ = 2 * k_;
+    n_ = g.nx * g.ny * g.nz;
+    es_ = esize(prec_);
+    V_ = Buf(c, (size_t)n_ * mmax_ * es_);
+    W_ = Buf(c, (size_t)n_ * mmax_ * es_);
+    X_ = Buf(c, (size_t)n_ * k_ * es_);
+    HX_ = Buf(c, (size_t)n_ * k_ * es_);
+    R_ = Buf(c, (size_t)n_ * k_ * es_);
+    T_ = Buf(c, (size_t)n_ * k_ * es_);
+    G_ = Buf(c, sizeof(double) * (size_t)mmax_ * mmax_);
+    Q_ = Buf(c, sizeof(double) * (size_t)mmax_ * mmax_);
+    P_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
+    S_ = Buf(c, sizeof(double) * (size_t)k_ * k_);
+    theta_ = Buf(c, sizeof(double) * (size_t)mmax_);
+    rn_ = Buf(c, sizeof(double) * (size_t)k_);
+    G_.zero();
+}
+
+void BlockDavidson::hpsi(const void* veff, const void* in, void* out, int ncols) {
+    check(c_, dft_hpsi(c_, &g_, prec_, veff, in, n_, ncols, out, n_), "dft_hpsi");
+}
+
+void BlockDavidson::gram(int ma, int mb, const void* A, const void* B, double* out, int64_t ldo, double beta) {
+    const int mode = prec_ == DFT_F64 ? DFT_GRAM_F64 : gram_mode_;
+    check(c_, dft_gram(c_, mode, n_, ma, mb, 1.0, A, n_, B, n_, beta, out, ldo), "dft_gram");
+}
+
+void BlockDavidson::load_X(const void* src, int src_prec, int ncols) {
+    ncols = std::min(ncols, k_);
+    check(c_, dft_convert(c_, src, src_prec, X_.get(), prec_, n_ * (int64_t)ncols), "dft_convert");
+}
+
+int BlockDavidson::orthonormalise(void* src, int cnt, void* dst, void* tmp) {
+    // two CholQR passes: src -> tmp -> dst
+    void* in = src;
+    for (int pass = 0; pass < 2; ++pass) {
+        void* out = pass == 0 ? tmp : dst;
+        double* S = S_.as<double>();
+        gram(cnt, cnt, in, in, S, cnt);
+        const int rc = dft_chol_inv(c_, cnt, S, cnt);
+        if (rc == DFT_OK) {
+            check(c_, dft_block_update(c_, prec_, n_, cnt, cnt, 1.0, in, n_, S, cnt, 0.0, out, n_), "dft_block_update");
+        } else if (rc == DFT_ENUMERIC) {
+            // rank-deficient block: keep eigen-directions above a relative floor
+            gram(cnt, cnt, in, in, S, cnt);
+            check(c_, dft_syev(c_, cnt, S, cnt, theta_.as<double>() /* scratch */), "dft_syev");
+            std::vector<double> lam(cnt), U((size_t)cnt * cnt);
+            check(c_, dft_download(c_, lam.data(), theta_.get(), sizeof(double) * cnt), "dft_download");
+            check(c_, dft_download(c_, U.data(), S, sizeof(double) * cnt * cnt), "dft_download");
+            const double lmax = std::max(lam.back(), 0.0);
+            const double floor = (prec_ == DFT_F64 ? 1e-24 : 1e-12) * std::max(lmax, 1e-300);
+            int first = 0;
+            while (first < cnt && !(lam[first] > floor)) ++first;
+            const int kept = cnt - first;
+            if (kept == 0) return 0;
+            std::vector<double> C((size_t)cnt * kept);
+            for (int j = 0; j < kept; ++j) {
+                const double s = 1.0 / std::sqrt(lam[first + j]);
+                for (int i = 0; i < cnt; ++i) C[(size_t)j * cnt + i] = U[(size_t)(first + j) * cnt + i] * s;
+            }
+            check(c_, dft_upload(c_, S, C.data(), sizeof(double) * C.size()), "dft_upload");
+            check(c_, dft_block_update(c_, prec_, n_, cnt, kept, 1.0, in, n_, S, cnt, 0.0, out, n_), "dft_block_update");
+            cnt = kept;
+        } else {
+            check(c_, rc, "dft_chol_inv");
+        }
+        in = out;
+    }
+    return cnt;
+}
+
+DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOptions& opt, Preconditioner& P) {
+    DavidsonStats st;
+    gram_mode_ = opt.gram_mode;
+    const int k = k_;
+    const size_t colb = (size_t)n_ * es_;
+    char* V = static_cast<char*>(V_.get());
+    char* W = static_cast<char*>(W_.get());
+    double* G = G_.as<double>();
+    double* Q = Q_.as<double>();
+    double* Pm = P_.as<double>();
+    double* theta = theta_.as<double>();
+    if (!warm) check(c_, dft_seed_vectors(c_, &g_, prec_, 0, k, X_.get(), n_), "dft_seed_vectors");
+    int m = orthonormalise(X_.get(), k, V, T_.get());
+    if (m < k) throw std::runtime_error("No se pudo construir el vector inicial.");
+    hpsi(veff, V, W, k);
+    st.happly_columns += k;
+    gram(k, k, V, W, G, mmax_);
+    const int nconv = std::min(std::max(opt.nconv, 1), k);
+    theta_h_.assign(k, 0.0);
+    rn_h_.assign(k, INFINITY);
+    std::vector<double> prev_rn;
+    for (int it = 1; it <= opt.max_iter; ++it) {
+        st.iterations = it;
+        check(c_, dft_copy(c_, Q, G, sizeof(double) * (size_t)mmax_ * mmax_), "dft_copy");
+        check(c_, dft_syev(c_, m, Q, mmax_, theta), "dft_syev");
+        check(c_, dft_block_update(c_, prec_, n_, m, k, 1.0, V, n_, Q, mmax_, 0.0, X_.get(), n_), "dft_block_update");
+        check(c_, dft_block_update(c_, prec_, n_, m, k, 1.0, W, n_, Q, mmax_, 0.0, HX_.get(), n_), "dft_block_update");
+        check(c_, dft_residual(c_, prec_, n_, k, HX_.get(), n_, X_.get(), n_, theta, R_.get(), n_), "dft_residual");
+        check(c_, dft_column_norms2(c_, prec_, n_, k, R_.get(), n_, rn_.as<double>()), "dft_column_norms2");
+        check(c_, dft_download(c_, theta_h_.data(), theta, sizeof(double) * k), "dft_download");
+        check(c_, dft_download(c_, rn_h_.data(), rn_.get(), sizeof(double) * k), "dft_download");
+        double maxres = 0.0;
+        for (int j = 0; j < k; ++j) {
+            rn_h_[j] = std::sqrt(std::max(rn_h_[j], 0.0));
+            if (!std::isfinite(rn_h_[j])) throw std::runtime_error("Davidson: non-finite residual");
+            if (j < nconv) maxres = std::max(maxres, rn_h_[j]);
+        }
+        if (!prev_rn.empty()) P.feedback(prev_rn, rn_h_);
+        prev_rn = rn_h_;
+        st.max_residual = maxres;
+        if (maxres < opt.tol) {
+            st.converged = true;
+            break;
+        }
+        if (it == opt.max_iter) break;
+        // corrections for every unconverged column, in contiguous runs
+        int nt = 0;
+        for (int j = 0; j < k;) {
+            if (rn_h_[j] < opt.tol) { ++j; continue; }
+            int j1 = j;
+            while (j1 < k && rn_h_[j1] >= opt.tol) ++j1;
+            P.apply(c_, g_, prec_, veff, static_cast<char*>(R_.get()) + j * colb, n_, j1 - j, theta + j,
+                    static_cast<char*>(T_.get()) + (size_t)nt * colb, n_);
+            nt += j1 - j;
+            j = j1;
+        }
+        // block Gram-Schmidt against the basis, twice
+        for (int pass = 0; pass < 2; ++pass) {
+            gram(m, nt, V, T_.get(), Pm, mmax_);
+            check(c_, dft_block_update(c_, prec_, n_, m, nt, -1.0, V, n_, Pm, mmax_, 1.0, T_.get(), n_), "dft_block_update");
+        }
+        if (m + nt > mmax_) {  // restart on the Ritz block
+            check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
+            check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
+            m = k;
+            gram(k, k, V, W, G, mmax_);
+            ++st.restarts;
+        }
+        const int kept = orthonormalise(T_.get(), nt, V + (size_t)m * colb, R_.get());
+        if (kept == 0) break;  // no new direction: stagnation
+        hpsi(veff, V + (size_t)m * colb, W + (size_t)m * colb, kept);
+        st.happly_columns += kept;
+        gram(m + kept, kept, V, W + (size_t)m * colb, G + (size_t)m * mmax_, mmax_);
+        check(c_, dft_transpose(c_, m, kept, G + (size_t)m * mmax_, mmax_, G + m, mmax_), "dft_transpose");
+        m += kept;
+    }
+    return st;
+}
+
+}  // namespace dftc

@@ -1,0 +1,90 @@
+// davidson.hpp -- block Davidson eigensolver on device-resident orbital
+// blocks.  Replaces the reference's one-orbital-at-a-time Davidson
+// (EigenvalueSolverMolecule.cpp:666-1300) with a block method whose dense
+// products are the MFMA Gram kernels:
+//   basis V (N x m, orthonormal), W = H V, G = V^T W (FP64, m x m)
+//   Rayleigh-Ritz: G = Q diag(theta) Q^T (FP64 on the device)
+//   X = V Q_k, HX = W Q_k, R = HX - X theta
+//   T = P(R) for unconverged columns (P = diagonal or learned preconditioner)
+//   T <- T - V (V^T T) twice, then CholQR twice, appended to V;
+//   restart to V = X when m would exceed the subspace limit.
+// Vectors are kept at unit 2-norm; with psi = u / sqrt(dV) the reference's
+// residual ||H psi - eps psi|| sqrt(dV) equals ||H u - eps u||.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "gpu.hpp"
+
+namespace dftc {
+
+// A correction operator T = P(R) on `ncols` residual columns.
+struct Preconditioner {
+    virtual ~Preconditioner() = default;
+    // Writes T columns; returns how many of the columns used the fallback.
+    virtual int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* R, int64_t ldr, int ncols,
+                      const double* theta_dev, void* T, int64_t ldt) = 0;
+    // Told after each iteration whether residual norms went down (per column index in the block).
+    virtual void feedback(const std::vector<double>& /*before*/, const std::vector<double>& /*after*/) {}
+};
+
+struct DiagonalPreconditioner : Preconditioner {
+    int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* R, int64_t ldr, int ncols,
+              const double* theta_dev, void* T, int64_t ldt) override {
+        check(c, dft_precond_diag(c, &g, prec, veff, R, ldr, ncols, theta_dev, T, ldt), "dft_precond_diag");
+        return 0;
+    }
+};
+
+struct DavidsonOptions {
+    int nconv = 1;         // leading states that must reach tol
+    double tol = 1.0e-7;   // EigenvalueSolverMolecule.cpp:19
+    int max_iter = 200;
+    int gram_mode = DFT_GRAM_F64;
+};
+
+struct DavidsonStats {
+    int iterations = 0;
+    int restarts = 0;
+    int64_t happly_columns = 0;
+    double max_residual = 0.0;
+    bool converged = false;
+};
+
+class BlockDavidson {
+   public:
+    BlockDavidson(dft_ctx* c, const dft_grid& g, int prec, int nstates, int max_subspace);
+    // Solve for the lowest nstates of H = -1/2 Lap + veff.  When `warm` is
+    // true the current X is the starting block, otherwise seed vectors
+    // (buildInitialVector) are used.
+    DavidsonStats solve(const void* veff, bool warm, const DavidsonOptions& opt, Preconditioner& P);
+
+    void* X() const { return X_.get(); }          // N x k Ritz vectors (unit 2-norm)
+    int64_t ld() const { return n_; }
+    int nstates() const { return k_; }
+    int prec() const { return prec_; }
+    const std::vector<double>& eigenvalues() const { return theta_h_; }
+    const std::vector<double>& residuals() const { return rn_h_; }
+    // Replace X with a block given in another precision (refinement hand-over).
+    void load_X(const void* src, int src_prec, int ncols);
+
+   private:
+    void hpsi(const void* veff, const void* in, void* out, int ncols);
+    void gram(int ma, int mb, const void* A, const void* B, double* out, int64_t ldo, double beta = 0.0);
+    // Orthonormalise `cnt` columns at src into dst (CholQR twice, with an
+    // eigenvalue filter if the block is rank deficient); returns kept count.
+    int orthonormalise(void* src, int cnt, void* dst, void* tmp);
+
+    dft_ctx* c_;
+    dft_grid g_;
+    int prec_, k_, mmax_;
+    int64_t n_;
+    size_t es_;
+    Buf V_, W_, X_, HX_, R_, T_;
+    Buf G_, Q_, P_, S_, theta_, rn_;
+    std::vector<double> theta_h_, rn_h_;
+    int gram_mode_ = DFT_GRAM_F64;
+};
+
+}  // namespace dftc

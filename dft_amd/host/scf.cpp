@@ -1,0 +1,289 @@
+// scf.cpp -- device-resident SCF driver (see scf.hpp).
+#include "scf.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+namespace dftc {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// DFTConstants.h:14-19 and SelfConsistentFieldMolecule.cpp:27-31
+constexpr double kMixing0 = 0.25, kMinMix = 0.10, kMaxMix = 0.50, kMixUp = 1.10, kMixDown = 0.75, kOsc = 1.05;
+constexpr double kEnergyTol = 1.0e-11, kDensityTol = 1.0e-9, kKsTol = 1.0e-7;
+
+}  // namespace
+
+struct ScfDriver::Fields {
+    int prec;
+    int64_t n;
+    Buf vnuc, ra, rb, rho, vh, vhl, vxa, vxb, va, vb, ora, orb, orho, ovh, ta, tb;
+    std::unique_ptr<BlockDavidson> da, db;
+    Fields(dft_ctx* c, const dft_grid& g, int p, int k, int mmax, bool closed) : prec(p) {
+        n = g.nx * g.ny * g.nz;
+        const size_t b = (size_t)n * esize(p);
+        for (Buf* f : {&vnuc, &ra, &rb, &rho, &vh, &vhl, &vxa, &vxb, &va, &vb, &ora, &orb, &orho, &ovh, &ta, &tb}) *f = Buf(c, b);
+        vhl.zero();
+        da.reset(new BlockDavidson(c, g, p, k, mmax));
+        if (!closed) db.reset(new BlockDavidson(c, g, p, k, mmax));
+    }
+};
+
+ScfDriver::ScfDriver(dft_ctx* c, const dft_grid& g, int natoms, const double* nuclei, const dft_scf_config& cfg)
+    : c_(c), g_(g), nuc_(nuclei, nuclei + 4 * (size_t)natoms), cfg_(cfg) {}
+
+void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
+    rep = dft_scf_report{};
+    const int natoms = (int)(nuc_.size() / 4);
+    // SelfConsistentFieldMolecule.cpp:58-127
+    if (natoms == 0) throw std::invalid_argument("La molecula debe contener al menos un nucleo.");
+    if (cfg_.multiplicity < 1) throw std::invalid_argument("La multiplicidad electronica debe ser al menos 1.");
+    if (cfg_.functional != DFT_XC_PZ81 && cfg_.functional != DFT_XC_PBE96) throw std::invalid_argument("unknown functional");
+    int zsum = 0;
+    for (int a = 0; a < natoms; ++a) zsum += (int)nuc_[4 * a];
+    const int ne = zsum - cfg_.charge;
+    if (ne < 0) throw std::invalid_argument("La carga molecular produce un numero negativo de electrones.");
+    if (ne == 0) throw std::invalid_argument("El SCF molecular requiere al menos un electron.");
+    const int sd = cfg_.multiplicity - 1;
+    if (sd > ne) throw std::invalid_argument("La multiplicidad electronica es incompatible con el numero de electrones.");
+    if ((ne - sd) % 2 != 0)
+        throw std::invalid_argument("La multiplicidad electronica es incompatible con la paridad del numero de electrones.");
+    const int nal = (ne + sd) / 2, nbe = (ne - sd) / 2;
+    const bool closed = cfg_.multiplicity == 1;
+    if (nbe <= 0) throw std::invalid_argument("El numero de orbitales debe ser mayor que cero.");
+    const int nocc_max = std::max(nal, nbe);
+    const int k = std::max(cfg_.nstates > 0 ? cfg_.nstates : nocc_max, nocc_max);
+    const int mmax = cfg_.max_subspace > 0 ? cfg_.max_subspace : 3 * k;
+    const int maxit = cfg_.max_iterations > 0 ? cfg_.max_iterations : 250;
+    const int batch = cfg_.poisson_batch > 0 ? cfg_.poisson_batch : 64;
+    const double dv = g_.dx * g_.dy * g_.dz;
+    const int64_t n = g_.nx * g_.ny * g_.nz;
+    double enn = 0.0;  // NuclearRepulsion.cpp:6-54
+    for (int a = 0; a < natoms; ++a)
+        for (int b = a + 1; b < natoms; ++b) {
+            const double dx = nuc_[4 * a + 1] - nuc_[4 * b + 1], dy = nuc_[4 * a + 2] - nuc_[4 * b + 2],
+                         dz = nuc_[4 * a + 3] - nuc_[4 * b + 3];
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 == 0.0) throw std::runtime_error("NuclearRepulsion is undefined for coincident nuclei.");
+            enn += nuc_[4 * a] * nuc_[4 * b] / std::sqrt(d2);
+        }
+
+    std::unique_ptr<Preconditioner> precond;
+    if (cfg_.precond == DFT_PRECOND_CNN) {
+        precond = make_cnn_preconditioner(c_, cfg_.cnn, g_);
+        if (!precond) throw std::invalid_argument("learned preconditioner requested without a model");
+    } else {
+        precond.reset(new DiagonalPreconditioner());
+    }
+
+    const bool mixed = cfg_.precision == DFT_SCF_MIXED;
+    std::unique_ptr<Fields> F(new Fields(c_, g_, mixed ? DFT_F32 : DFT_F64, k, mmax, closed));
+    const auto t0 = Clock::now();
+    double t_pois = 0, t_xc = 0, t_dav = 0;
+    auto tick = [&](double& acc, Clock::time_point& t) {
+        check(c_, dft_sync(c_), "dft_sync");
+        const auto now = Clock::now();
+        acc += secs(t, now);
+        t = now;
+    };
+
+    std::vector<double> occ_a(nal, 1.0 / dv), occ_b(nbe, 1.0 / dv);  // unit-norm vectors -> psi^2 = u^2/dV
+    std::vector<int> el_a(nal, 1), el_b(nbe, 1);
+    DavidsonOptions dop;
+    dop.tol = cfg_.davidson_tol > 0 ? cfg_.davidson_tol : 1.0e-7;
+    dop.max_iter = cfg_.davidson_max_iter > 0 ? cfg_.davidson_max_iter : 200;
+    bool have_prev = false;
+
+    auto davidson = [&](Fields& f, BlockDavidson& D, const void* veff, int nocc) {
+        DavidsonOptions o = dop;
+        o.nconv = nocc;
+        if (f.prec == DFT_F32) {
+            o.gram_mode = DFT_GRAM_SPLIT_BF16;
+            o.tol = std::max(o.tol, 5.0e-5);  // FP32 residual floor
+        }
+        DavidsonStats st = D.solve(veff, have_prev, o, *precond);
+        rep.happly_columns += st.happly_columns;
+        rep.davidson_iterations += st.iterations;
+        if (!st.converged && f.prec == DFT_F64 && st.max_residual > 1e3 * o.tol)
+            throw std::runtime_error("El orbital molecular no convergio. residual = " + std::to_string(st.max_residual));
+        return st;
+    };
+
+    auto poisson = [&](Fields& f, const void* rho, void* v) {
+        int64_t sw = 0;
+        double md = 0;
+        if (cfg_.poisson_fixed > 0) {
+            check(c_, dft_copy(c_, v, f.vhl.get(), (size_t)n * esize(f.prec)), "dft_copy");
+            check(c_, dft_poisson(c_, &g_, f.prec, rho, v, cfg_.poisson_fixed, 0.0, 0, batch, &sw, &md), "dft_poisson");
+            check(c_, dft_copy(c_, f.vhl.get(), v, (size_t)n * esize(f.prec)), "dft_copy");
+        } else {
+            // HartreePotentialMolecule.cpp:14-15; FP32 cannot resolve 1e-8 on V ~ 1
+            const double tol = f.prec == DFT_F64 ? 1.0e-8 : 1.0e-6;
+            check(c_, dft_poisson(c_, &g_, f.prec, rho, v, 10000, tol, 1, batch, &sw, &md), "dft_poisson");
+        }
+        rep.poisson_sweeps += sw;
+    };
+
+    auto ddiff = [&](int p, const void* oldd, const void* newd) {  // MolecularSCFHelpers.cpp:46-95
+        double d2 = 0, n2 = 0;
+        check(c_, dft_reduce(c_, DFT_RED_DIFF2, p, newd, oldd, n, &d2), "dft_reduce");
+        check(c_, dft_reduce(c_, DFT_RED_DOT, p, newd, newd, n, &n2), "dft_reduce");
+        const double dn = std::sqrt(d2 * dv), nn = std::sqrt(n2 * dv);
+        return nn <= 1.0e-14 ? dn : dn / nn;
+    };
+
+    // ---- initial densities (MolecularSCFHelpers.cpp:515-579) ----
+    {
+        Fields& f = *F;
+        check(c_, dft_nuclear_potential(c_, &g_, f.prec, natoms, nuc_.data(), f.vnuc.get()), "dft_nuclear_potential");
+        if (cfg_.init_guess == DFT_INIT_HYDROGENIC) {
+            check(c_, dft_hydrogenic_density(c_, &g_, f.prec, natoms, nuc_.data(), nal, f.ra.get()), "dft_hydrogenic_density");
+            check(c_, dft_hydrogenic_density(c_, &g_, f.prec, natoms, nuc_.data(), nbe, f.rb.get()), "dft_hydrogenic_density");
+        } else {
+            auto t = Clock::now();
+            davidson(f, *f.da, f.vnuc.get(), nal);
+            if (!closed) davidson(f, *f.db, f.vnuc.get(), nbe);
+            tick(t_dav, t);
+            check(c_, dft_density(c_, f.prec, f.da->X(), n, nal, occ_a.data(), n, f.ra.get(), 0), "dft_density");
+            if (closed) check(c_, dft_copy(c_, f.rb.get(), f.ra.get(), (size_t)n * esize(f.prec)), "dft_copy");
+            else check(c_, dft_density(c_, f.prec, f.db->X(), n, nbe, occ_b.data(), n, f.rb.get(), 0), "dft_density");
+            have_prev = true;
+        }
+    }
+
+    double prevE = std::numeric_limits<double>::infinity(), prevDD = prevE, mixing = kMixing0;
+    bool refining = false;
+    int refine_steps = 0;
+    const double etol = cfg_.energy_tol > 0 ? cfg_.energy_tol : 1.0e-5;
+    for (int it = 1; it <= maxit + (mixed ? std::max(cfg_.refine_iterations, 0) : 0); ++it) {
+        if (!refining && it > maxit) break;
+        Fields& f = *F;
+        const int p = f.prec;
+        const size_t fb = (size_t)n * esize(p);
+        const double imix = mixing;
+        auto t = Clock::now();
+        // MolecularSCFIteration.cpp:48-66
+        check(c_, dft_add3(c_, p, n, f.ra.get(), f.rb.get(), nullptr, f.rho.get()), "dft_add3");
+        poisson(f, f.rho.get(), f.vh.get());
+        tick(t_pois, t);
+        // :68-90
+        check(c_, dft_xc(c_, &g_, p, cfg_.functional, f.ra.get(), f.rb.get(), f.vxa.get(), closed ? nullptr : f.vxb.get(),
+                         nullptr), "dft_xc");
+        tick(t_xc, t);
+        // :92-115
+        check(c_, dft_add3(c_, p, n, f.vnuc.get(), f.vh.get(), f.vxa.get(), f.va.get()), "dft_add3");
+        if (!closed) check(c_, dft_add3(c_, p, n, f.vnuc.get(), f.vh.get(), f.vxb.get(), f.vb.get()), "dft_add3");
+        // :117-144
+        DavidsonStats sa = davidson(f, *f.da, f.va.get(), nal), sb;
+        if (!closed) sb = davidson(f, *f.db, f.vb.get(), nbe);
+        have_prev = true;
+        tick(t_dav, t);
+        // :163-193
+        check(c_, dft_density(c_, p, f.da->X(), n, nal, occ_a.data(), n, f.ora.get(), 0), "dft_density");
+        if (closed) check(c_, dft_copy(c_, f.orb.get(), f.ora.get(), fb), "dft_copy");
+        else check(c_, dft_density(c_, p, f.db->X(), n, nbe, occ_b.data(), n, f.orb.get(), 0), "dft_density");
+        check(c_, dft_add3(c_, p, n, f.ora.get(), f.orb.get(), nullptr, f.orho.get()), "dft_add3");
+        poisson(f, f.orho.get(), f.ovh.get());
+        tick(t_pois, t);
+        // :195-205 -> TotalEnergyMolecule.cpp:237-333
+        double E[6];
+        {
+            double ta = 0, tb = 0;
+            check(c_, dft_kinetic_energy(c_, &g_, p, f.da->X(), n, nal, el_a.data(), &ta), "dft_kinetic_energy");
+            if (closed) tb = ta;
+            else check(c_, dft_kinetic_energy(c_, &g_, p, f.db->X(), n, nbe, el_b.data(), &tb), "dft_kinetic_energy");
+            E[0] = (ta + tb) / dv;
+            double s = 0;
+            check(c_, dft_reduce(c_, DFT_RED_DOT, p, f.orho.get(), f.vnuc.get(), n, &s), "dft_reduce");
+            E[1] = s * dv;
+            check(c_, dft_hartree_energy(c_, &g_, p, f.orho.get(), f.ovh.get(), &E[2]), "dft_hartree_energy");
+            check(c_, dft_xc(c_, &g_, p, cfg_.functional, f.ora.get(), f.orb.get(), f.ta.get(), f.tb.get(), &E[3]), "dft_xc");
+            E[4] = enn;
+            E[5] = E[0] + E[1] + E[2] + E[3] + E[4];
+        }
+        // :207-241
+        const double ddt = ddiff(p, f.rho.get(), f.orho.get());
+        const double dds = std::max(ddiff(p, f.ra.get(), f.ora.get()), ddiff(p, f.rb.get(), f.orb.get()));
+        const double dd = std::max(ddt, dds);
+        // :243-249: max KS residual over occupied orbitals (Davidson's final residuals)
+        double ks = 0.0;
+        for (int j = 0; j < nal; ++j) ks = std::max(ks, f.da->residuals()[j]);
+        if (!closed)
+            for (int j = 0; j < nbe; ++j) ks = std::max(ks, f.db->residuals()[j]);
+        // SelfConsistentFieldMolecule.cpp:297-400
+        const double de = std::isfinite(prevE) ? std::fabs(E[5] - prevE) : std::numeric_limits<double>::infinity();
+        rep.iterations = it;
+        std::copy(E, E + 6, rep.energy);
+        rep.energy_difference = de;
+        rep.density_difference = dd;
+        rep.ks_residual = ks;
+        out.trace.insert(out.trace.end(), {E[5], de, dd, ks, imix});
+        out.eig.assign(f.da->eigenvalues().begin(), f.da->eigenvalues().begin() + nal);
+        const auto& eb = closed ? f.da->eigenvalues() : f.db->eigenvalues();
+        out.eig.insert(out.eig.end(), eb.begin(), eb.begin() + nbe);
+        tick(rep.seconds_other, t);
+        if (!rep.first_de_iteration && de < etol) {
+            rep.first_de_iteration = it;
+            rep.seconds_to_first_de = secs(t0, Clock::now());
+        }
+        if (cfg_.verbose)
+            std::fprintf(stderr, "[scf] %3d %s E=%.12f dE=%.3e dRho=%.3e KS=%.3e mix=%.3f eps0=%.8f dav=%d/%d\n", it,
+                         p == DFT_F64 ? "f64" : "f32", E[5], de, dd, ks, imix, out.eig[0], sa.iterations,
+                         closed ? 0 : sb.iterations);
+        bool conv = cfg_.stop_rule == DFT_STOP_DE ? de < etol : (de < kEnergyTol && dd < kDensityTol && ks < kKsTol);
+        if (refining) ++refine_steps;
+        const bool hand_over = conv && mixed && !refining && cfg_.refine_iterations > 0;
+        if (conv && !hand_over) {
+            rep.converged = 1;
+            break;
+        }
+        // :447-484 lagged adaptive mixing
+        if (std::isfinite(prevDD)) {
+            if (dd > prevDD * kOsc) mixing = std::max(kMinMix, mixing * kMixDown);
+            else if (dd < prevDD) mixing = std::min(kMaxMix, mixing * kMixUp);
+        }
+        check(c_, dft_axpby(c_, p, n, imix, f.ora.get(), 1.0 - imix, f.ra.get()), "dft_axpby");
+        check(c_, dft_axpby(c_, p, n, imix, f.orb.get(), 1.0 - imix, f.rb.get()), "dft_axpby");
+        prevDD = dd;
+        prevE = E[5];
+        if (hand_over) {
+            // FP64 refinement: carry densities, V_H warm start and orbitals over
+            std::unique_ptr<Fields> G(new Fields(c_, g_, DFT_F64, k, mmax, closed));
+            check(c_, dft_convert(c_, f.ra.get(), DFT_F32, G->ra.get(), DFT_F64, n), "dft_convert");
+            check(c_, dft_convert(c_, f.rb.get(), DFT_F32, G->rb.get(), DFT_F64, n), "dft_convert");
+            check(c_, dft_convert(c_, f.vhl.get(), DFT_F32, G->vhl.get(), DFT_F64, n), "dft_convert");
+            check(c_, dft_nuclear_potential(c_, &g_, DFT_F64, natoms, nuc_.data(), G->vnuc.get()), "dft_nuclear_potential");
+            G->da->load_X(f.da->X(), DFT_F32, k);
+            if (!closed) G->db->load_X(f.db->X(), DFT_F32, k);
+            F = std::move(G);
+            refining = true;
+            prevE = std::numeric_limits<double>::infinity();
+            prevDD = prevE;
+        }
+    }
+    check(c_, dft_sync(c_), "dft_sync");
+    rep.seconds_total = secs(t0, Clock::now());
+    rep.seconds_poisson = t_pois;
+    rep.seconds_xc = t_xc;
+    rep.seconds_davidson = t_dav;
+    rep.refine_iterations = refine_steps;
+    if (out.want_rho) {
+        Fields& f = *F;
+        out.rho.assign(2 * (size_t)n, 0.0);
+        Buf tmp(c_, (size_t)n * 8);
+        check(c_, dft_convert(c_, f.ora.get(), f.prec, tmp.get(), DFT_F64, n), "dft_convert");
+        tmp.download(out.rho.data(), (size_t)n * 8);
+        check(c_, dft_convert(c_, f.orb.get(), f.prec, tmp.get(), DFT_F64, n), "dft_convert");
+        tmp.download(out.rho.data() + n, (size_t)n * 8);
+    }
+}
+
+}  // namespace dftc

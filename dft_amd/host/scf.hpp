@@ -1,0 +1,44 @@
+// scf.hpp -- device-resident Kohn-Sham SCF driver.
+//
+// Restates SelfConsistentFieldMolecule.cpp:50-561 and
+// MolecularSCFIteration.cpp:12-252 step for step (densities -> V_H -> V_xc
+// -> V_eff -> orbitals -> output densities -> output V_H -> energies ->
+// dRho -> KS residual -> stop test -> lagged adaptive linear mixing) with
+// every field on the GPU and the orbital solve done by the block Davidson.
+// Mixed mode runs the loop in FP32 (split-BF16 Gram products) until the
+// stop rule holds, then continues in FP64 until it holds again.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "davidson.hpp"
+#include "dft_core.h"
+#include "gpu.hpp"
+
+namespace dftc {
+
+struct ScfOutputs {
+    std::vector<double> eig;    // occupied alpha then beta eigenvalues
+    std::vector<double> trace;  // 5 per step
+    std::vector<double> rho;    // output alpha then beta density (FP64), optional
+    bool want_rho = false;
+};
+
+// Factory for the learned preconditioner (cnn.cpp); null when unavailable.
+std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g);
+
+class ScfDriver {
+   public:
+    ScfDriver(dft_ctx* c, const dft_grid& g, int natoms, const double* nuclei, const dft_scf_config& cfg);
+    void run(dft_scf_report& rep, ScfOutputs& out);
+
+   private:
+    struct Fields;
+    dft_ctx* c_;
+    dft_grid g_;
+    std::vector<double> nuc_;
+    dft_scf_config cfg_;
+};
+
+}  // namespace dftc

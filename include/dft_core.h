@@ -1,0 +1,90 @@
+/*
+ * dft_core.h -- C-ABI of the device-resident solver layer (libdft_core.so).
+ *
+ * The reference's drop-in C++ interface (applyMolecularKohnShamHamiltonian,
+ * solveMolecularOrbitals, calculateHartreePotential, ..., the SCF driver)
+ * is declared in dft_molecule.h.  The entry points here run the same SCF
+ * (SelfConsistentFieldMolecule.cpp:50-561, MolecularSCFIteration.cpp:12-252)
+ * with every field resident on the GPU, plus the configuration knobs a
+ * workload needs (Poisson mode, stop rule, initial guess, precision mode,
+ * Davidson block size, preconditioner).  Host arrays only cross the
+ * boundary at entry (geometry) and exit (results).
+ * Status codes: 0 ok, -1 invalid argument, -2 numerical failure or
+ * non-convergence, -3 GPU error; dft_core_last_error() holds the message.
+ */
+#ifndef DFT_CORE_H
+#define DFT_CORE_H
+
+#include <stdint.h>
+
+#include "dft_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { DFT_SCF_FP64 = 0, DFT_SCF_MIXED = 1 };
+enum { DFT_STOP_REFERENCE = 0, DFT_STOP_DE = 1 };
+enum { DFT_INIT_BARE = 0, DFT_INIT_HYDROGENIC = 1 };
+enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1 };
+
+typedef struct {
+    int functional;        /* DFT_XC_PZ81 / DFT_XC_PBE96 */
+    int charge;
+    int multiplicity;      /* 1: closed shell (beta copies alpha, MolecularSCFIteration.cpp:77-90,127-133) */
+    int precision;         /* DFT_SCF_FP64 or DFT_SCF_MIXED */
+    int nstates;           /* Davidson block per spin (>= occupied); 0: occupied count */
+    int max_subspace;      /* 0: 3 * block */
+    int max_iterations;    /* SCF steps; 0: 250 (DFTConstants.h:15) */
+    int poisson_fixed;     /* 0: reference rule (tol 1e-8, cap 10000, zero start); k > 0: k warm sweeps */
+    int poisson_batch;     /* sweeps between host checks; 0: 64 */
+    int stop_rule;         /* DFT_STOP_REFERENCE (dE, dRho, KS triple) or DFT_STOP_DE */
+    double energy_tol;     /* |dE| threshold for DFT_STOP_DE */
+    int init_guess;        /* DFT_INIT_BARE (reference) or DFT_INIT_HYDROGENIC */
+    double davidson_tol;   /* residual threshold; 0: 1e-7 (EigenvalueSolverMolecule.cpp:19) */
+    int davidson_max_iter; /* 0: 200 */
+    int refine_iterations; /* mixed mode: max FP64 refinement steps after the FP32 phase stops */
+    int precond;           /* DFT_PRECOND_DIAG or DFT_PRECOND_CNN */
+    const void* cnn;       /* trained network (dft_cnn_model) for DFT_PRECOND_CNN */
+    int verbose;           /* 1: one line per SCF step on stderr */
+} dft_scf_config;
+
+typedef struct {
+    int iterations;            /* SCF steps taken (all phases) */
+    int converged;
+    int first_de_iteration;    /* first step with |dE| < energy_tol */
+    int refine_iterations;     /* FP64 steps after the mixed phase */
+    double energy[6];          /* kinetic, external, hartree, xc, nuclear, total (EnergyComponents, DFTData.h:47-54) */
+    double energy_difference;
+    double density_difference;
+    double ks_residual;
+    double seconds_total;      /* wall time of the SCF loop, GPU synchronised */
+    double seconds_to_first_de;
+    double seconds_poisson, seconds_xc, seconds_davidson, seconds_other;
+    int64_t happly_columns;    /* orbital H applications */
+    int64_t poisson_sweeps;
+    int64_t davidson_iterations;
+    int64_t cnn_applied, cnn_fallbacks;
+} dft_scf_report;
+
+const char* dft_core_last_error(void);
+
+/* Run the SCF.  nuclei: {Z, x, y, z} * natoms.  Optional outputs (NULL to
+ * skip): eig (n_alpha + n_beta occupied eigenvalues), trace (5 doubles per
+ * step: E, dE, dRho, KS, mixing; capacity max_iterations + refine), rho
+ * (2*N: output alpha then beta density). */
+int dft_scf_run(const dft_grid* g, int natoms, const double* nuclei, const dft_scf_config* cfg, dft_scf_report* rep,
+                double* eig, double* trace, int trace_cap, double* rho);
+
+/* Block Davidson on a given potential (tests and benchmarks).  veff_host: N
+ * FP64 values.  prec/gram_mode select the arithmetic.  eig: nstates values;
+ * psi (optional): nconv orbitals, N each, normalised sum psi^2 dV = 1 like
+ * the reference's MolecularOrbital::psi; resid (optional): nstates residuals. */
+int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
+                     int max_subspace, double tol, int max_iter, double* eig, double* psi, double* resid,
+                     int* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,207 @@
+/*
+ * dft_hip.h -- C-ABI of the gfx950 kernel library (libdft_hip.so).
+ *
+ * The reference (willandru/DFT) has no FFI: its 3D Kohn-Sham hot path is a
+ * set of free C++ functions on std::vector<double>.  Each entry point below
+ * replaces the loop nest of one of those functions; the C++ layer in
+ * include/dft_molecule.h keeps the reference's own names and signatures on
+ * top of these.  Conventions:
+ *   - every function returns an int status: 0 ok, -1 invalid argument,
+ *     -2 numerical failure / non-convergence, -3 HIP error;
+ *     dft_last_error(ctx) holds the message;
+ *   - field/orbital pointers are DEVICE pointers from dft_alloc (or any
+ *     device allocation on the context's device); host pointers are named
+ *     *_host;
+ *   - fields are nx*ny*nz values with x fastest (index (k*ny+j)*nx+i,
+ *     CartesianGrid.cpp:145-159); an orbital block is column-major, column
+ *     o at psi + o*ld, ld >= nx*ny*nz;
+ *   - all work is queued on the context's stream; functions that return a
+ *     host scalar synchronise that stream;
+ *   - one context per host thread; calls on one context are not re-entrant.
+ */
+#ifndef DFT_HIP_H
+#define DFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dft_ctx dft_ctx;
+
+enum { DFT_OK = 0, DFT_EINVAL = -1, DFT_ENUMERIC = -2, DFT_EHIP = -3 };
+/* Arithmetic/storage precision of a field or orbital block. */
+enum { DFT_F64 = 0, DFT_F32 = 1 };
+/* Gram product modes (dft_gram). */
+enum { DFT_GRAM_F64 = 0, DFT_GRAM_F32 = 1, DFT_GRAM_SPLIT_BF16 = 2 };
+/* XCFunctional subclasses (XCFunctional.h:15-21, PZ81.h, PBE96.h). */
+enum { DFT_XC_PZ81 = 1, DFT_XC_PBE96 = 2 };
+/* Reductions (dft_reduce). */
+enum { DFT_RED_SUM = 0, DFT_RED_DOT = 1, DFT_RED_MAXABS = 2, DFT_RED_DIFF2 = 3 };
+
+/* CartesianGrid (CartesianGrid.h:5-61) as plain data; dft_grid_init sets
+ * dx = (xmax-xmin)/(nx-1) exactly as CartesianGrid.cpp:45-47. */
+typedef struct {
+    int64_t nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    double dx, dy, dz;
+} dft_grid;
+
+int dft_grid_init(dft_grid* g, int64_t nx, int64_t ny, int64_t nz, double xmin, double xmax, double ymin,
+                  double ymax, double zmin, double zmax);
+
+/* ---- context, memory, stream ---------------------------------------- */
+int dft_ctx_create(int device, dft_ctx** out);
+int dft_ctx_destroy(dft_ctx* ctx);
+const char* dft_last_error(const dft_ctx* ctx);
+int dft_sync(dft_ctx* ctx);
+/* the context's hipStream_t, for callers that interleave their own work */
+void* dft_stream(dft_ctx* ctx);
+int dft_alloc(dft_ctx* ctx, size_t bytes, void** dev_ptr);
+int dft_free(dft_ctx* ctx, void* dev_ptr);
+int dft_upload(dft_ctx* ctx, void* dev_dst, const void* host_src, size_t bytes);
+int dft_download(dft_ctx* ctx, void* host_dst, const void* dev_src, size_t bytes);
+int dft_copy(dft_ctx* ctx, void* dev_dst, const void* dev_src, size_t bytes);
+int dft_zero(dft_ctx* ctx, void* dev_ptr, size_t bytes);
+/* y[i] = (prec_dst) x[i], n elements */
+int dft_convert(dft_ctx* ctx, const void* x, int prec_src, void* y, int prec_dst, int64_t n);
+/* Timing on the context stream: events are opaque. */
+int dft_event_create(dft_ctx* ctx, void** ev);
+int dft_event_record(dft_ctx* ctx, void* ev);
+int dft_event_elapsed_ms(dft_ctx* ctx, void* start, void* stop, float* ms);
+int dft_event_destroy(dft_ctx* ctx, void* ev);
+
+/* ---- (a) H psi ---------------------------------------------------------
+ * Replaces applyMolecularKohnShamHamiltonian (KohnShamHamiltonianMolecule.cpp:7-183):
+ * out[:,o] = -1/2 Lap_h psi[:,o] + veff .* psi[:,o] for nb orbitals, zero
+ * ghosts outside the box.  All three arrays in precision prec. */
+int dft_hpsi(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const void* psi, int64_t ld, int nb,
+             void* out, int64_t ldo);
+
+/* Kinetic energy sum_o e_o <psi_o| -1/2 Lap_Neumann |psi_o> dV with the
+ * reference's face rule (TotalEnergyMolecule.cpp:13-199).  electrons_host:
+ * nb ints.  Result in FP64 on the host. */
+int dft_kinetic_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
+                       const int* electrons_host, double* t_host);
+
+/* Davidson diagonal preconditioner (EigenvalueSolverMolecule.cpp:483-555):
+ * t[:,o] = r[:,o] / (eps_o - (sum 1/h^2 + veff)), or r where |denominator| < 1e-10
+ * or not finite.  eps_dev: nb FP64 values on the device. */
+int dft_precond_diag(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr,
+                     int nb, const double* eps_dev, void* t, int64_t ldt);
+
+/* ---- (b) Poisson ---------------------------------------------------------
+ * Replaces calculateHartreePotential (HartreePotentialMolecule.cpp:99-324).
+ * Boundary values follow the reference rule (:37-95: 0 on a face, else
+ * Q/|r|).  zero_start=1 is the reference call (interior starts at 0);
+ * zero_start=0 warm-starts from v on entry.  Sweeps stop after the first
+ * sweep whose max|dV| over the interior is < tol (tol<=0: never), or after
+ * max_sweeps; v holds that sweep's iterate (the exact reference stop, found
+ * by replaying a batch that overshoots).  sweeps_per_launch >=1 groups
+ * launches between host checks.  prec F32 runs the sweeps in FP32 on FP32
+ * rho/v. */
+int dft_poisson(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, void* v, int64_t max_sweeps, double tol,
+                int zero_start, int sweeps_per_launch, int64_t* sweeps_done, double* last_maxdiff);
+/* Hartree energy 1/2 sum rho V dV (HartreePotentialMolecule.cpp:326-379). */
+int dft_hartree_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, const void* vh, double* e_host);
+
+/* ---- (c) exchange-correlation -------------------------------------------
+ * Replaces calculateMolecularSpinExchangeCorrelationPotential
+ * (ExchangeCorrelationMolecule.cpp:576-747) for both spins in one call:
+ * va/vb receive v_alpha/v_beta (vb may be NULL to compute alpha only, the
+ * reference's closed-shell call).  exc_host (may be NULL) receives the
+ * energy of calculateMolecularExchangeCorrelationEnergy (:749-842). */
+int dft_xc(dft_ctx* ctx, const dft_grid* g, int prec, int functional, const void* na, const void* nb, void* va,
+           void* vb, double* exc_host);
+/* Pointwise XCFunctional::evaluate for npts inputs {na, nb, |grad na|,
+ * |grad nb|} (FP64, device) -> {eps, v_a, v_b, c_a, c_b}. */
+int dft_xc_points(dft_ctx* ctx, int functional, const double* in_dev, int64_t npts, double* out_dev);
+
+/* ---- (d) density and reductions ------------------------------------------
+ * calculateMolecularSpinDensity (ElectronDensityMolecule.cpp:5-76):
+ * n = (accumulate ? n : 0) + sum_o occ_o psi_o^2; occ_host: nb doubles. */
+int dft_density(dft_ctx* ctx, int prec, const void* psi, int64_t ld, int nb, const double* occ_host, int64_t n_pts,
+                void* n_out, int accumulate);
+/* FP64 reductions over n elements of a (and b): SUM a, DOT a.b, MAXABS a,
+ * DIFF2 sum (a-b)^2.  Result on the host. */
+int dft_reduce(dft_ctx* ctx, int op, int prec, const void* a, const void* b, int64_t n, double* result_host);
+/* Elementwise: y = alpha*x + beta*y */
+int dft_axpby(dft_ctx* ctx, int prec, int64_t n, double alpha, const void* x, double beta, void* y);
+/* out = a + b + c (c may be NULL) -- V_eff assembly (MolecularSCFIteration.cpp:92-115) */
+int dft_add3(dft_ctx* ctx, int prec, int64_t n, const void* a, const void* b, const void* c, void* out);
+/* NuclearPotential::calculate (NuclearPotential.cpp:6-76): nuclei_host =
+ * {Z,x,y,z} per atom.  Returns -2 if a nucleus sits exactly on a grid point. */
+int dft_nuclear_potential(dft_ctx* ctx, const dft_grid* g, int prec, int natoms, const double* nuclei_host, void* v);
+
+/* ---- (e) Gram / Rayleigh-Ritz products ------------------------------------
+ * out (device, FP64, column-major ma x mb, leading dim ldo) =
+ *   alpha * A^T B  (+ beta * out), A: N x ma (lda), B: N x mb (ldb).
+ * DFT_GRAM_F64: FP64 operands on FP64 MFMA.  DFT_GRAM_F32: FP32 operands,
+ * FP32 FMA.  DFT_GRAM_SPLIT_BF16: FP32 operands split x = hi + lo (bf16),
+ * A^T B ~ hi.hi + hi.lo + lo.hi on bf16 MFMA with FP32 accumulation,
+ * partial sums combined in FP64. */
+int dft_gram(dft_ctx* ctx, int mode, int64_t n, int ma, int mb, double alpha, const void* a, int64_t lda,
+             const void* b, int64_t ldb, double beta, double* out, int64_t ldo);
+/* Block update Y = alpha * A C + beta * Y: A N x m (prec), C m x k FP64
+ * device column-major (ldc), Y N x k (prec). */
+int dft_block_update(dft_ctx* ctx, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
+                     const double* c, int64_t ldc, double beta, void* y, int64_t ldy);
+/* Per-column scaling y[:,j] = x[:,j] * s[j] (s device FP64, nb values);
+ * and residual r[:,j] = hx[:,j] - theta[j] x[:,j]. */
+int dft_scale_columns(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, int64_t ldx, const double* s,
+                      void* y, int64_t ldy);
+int dft_residual(dft_ctx* ctx, int prec, int64_t n, int nb, const void* hx, int64_t ldh, const void* x, int64_t ldx,
+                 const double* theta, void* r, int64_t ldr);
+/* Column norms^2 (FP64 device out, nb values) */
+int dft_column_norms2(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, int64_t ldx, double* out);
+
+/* ---- dense FP64 subspace algebra on the device (m <= 1024) --------------- */
+/* Symmetric eigensolve of an m x m FP64 device matrix (column-major, ld):
+ * eigenvalues ascending into w (device), eigenvectors overwrite a. */
+int dft_syev(dft_ctx* ctx, int m, double* a, int64_t lda, double* w);
+/* Cholesky S = L L^T of an SPD m x m device matrix, then a <- L^{-T}
+ * (upper triangular, zeros below): for S = T^T T, T * a has orthonormal
+ * columns (CholQR).  Returns -2 if S is not positive definite. */
+int dft_chol_inv(dft_ctx* ctx, int m, double* a, int64_t lda);
+
+/* Small FP64 device transpose: dst(j,i) = src(i,j), src rows x cols. */
+int dft_transpose(dft_ctx* ctx, int rows, int cols, const double* src, int64_t lds, double* dst, int64_t ldd);
+
+/* ---- setup fields -------------------------------------------------------- */
+/* Davidson seed vectors buildInitialVector (EigenvalueSolverMolecule.cpp:177-283)
+ * for indices first_index .. first_index+count-1 into out columns. */
+int dft_seed_vectors(dft_ctx* ctx, const dft_grid* g, int prec, int first_index, int count, void* out, int64_t ld);
+/* Initial density guess: sum over atoms of Z * hydrogenic 1s density
+ * (Z^3/pi) exp(-2 Z r), normalised on the grid to `electrons`. */
+int dft_hydrogenic_density(dft_ctx* ctx, const dft_grid* g, int prec, int natoms, const double* nuclei_host,
+                           double electrons, void* out);
+
+/* ---- learned preconditioner: 3D CNN (BF16 weights, FP32 accumulation) ---- */
+/* Network: 4 layers of 3x3x3 convolutions, channels 2 -> C -> C -> C -> 1,
+ * SiLU after layers 1-3, zero padding, input {residual, V_eff} on a D^3
+ * patch (or a whole grid for inference).  Parameters are one FP32 master
+ * array (dft_cnn_param_count values) plus its BF16 copy used by the
+ * kernels. */
+int dft_cnn_param_count(int channels, int64_t* count);
+/* Forward over nbatch patches of nx*ny*nz; stores activations in act
+ * (dft_cnn_act_count floats) for backward when act != NULL. */
+int dft_cnn_act_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int64_t* count);
+int dft_cnn_forward(dft_ctx* ctx, int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch,
+                    const uint16_t* w_bf16, const float* in, float* out, float* act);
+/* Mean-squared-error loss vs target and its gradient wrt the parameters
+ * (FP32, accumulated into grad). loss_host may be NULL. */
+int dft_cnn_backward(dft_ctx* ctx, int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch,
+                     const uint16_t* w_bf16, const float* in, const float* act, const float* out,
+                     const float* target, float* grad, float* scratch, double* loss_host);
+int dft_cnn_scratch_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int64_t* count);
+/* Adam step on the FP32 master weights; refreshes the BF16 copy. */
+int dft_cnn_adam(dft_ctx* ctx, int64_t count, float* w, uint16_t* w_bf16, const float* grad, float* m, float* v,
+                 int step, float lr, float beta1, float beta2, float eps);
+int dft_cnn_to_bf16(dft_ctx* ctx, int64_t count, const float* w, uint16_t* w_bf16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

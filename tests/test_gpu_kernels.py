@@ -1,0 +1,262 @@
+"""GPU parity of the HIP kernels (through the C-ABI) against the FP64 oracle.
+
+Tolerances: FP64 kernels (H psi, V_H, V_xc) within relative 1e-12 of the
+oracle (normwise: max|gpu - ref| / max|ref|); energies within 1e-12
+relative; FP32 kernels within the FP32 bounds stated per test.
+"""
+import numpy as np
+import pytest
+
+from conftest import grid_from
+
+pytestmark = pytest.mark.gpu
+
+REL64 = 1e-12
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import dft_amd
+    d = dft_amd.Device(0)
+    yield d
+    d.close()
+
+
+def dgrid(og):
+    import dft_amd
+    return dft_amd.Grid.make(*og.n, *og.bounds)
+
+
+def test_hpsi_golden_f64(dev, goldens, oracle):
+    og = grid_from(goldens["hpsi_grid"])
+    g = dgrid(og)
+    v = dev.array(goldens["hpsi_v"])
+    p = dev.array(goldens["hpsi_psi"].reshape(1, -1))
+    out = dev.array(shape=(1, og.size))
+    dev.hpsi(g, v, p, out)
+    got = out.download()[0]
+    print("hpsi golden rel err", rel(got, goldens["hpsi_out"]))
+    assert rel(got, goldens["hpsi_out"]) <= REL64
+
+
+@pytest.mark.parametrize("shape", [(70, 37, 21), (130, 9, 12), (5, 66, 3)])
+def test_hpsi_block_f64_and_f32(dev, oracle, shape):
+    from oracle.oracle import Grid
+    og = Grid(*shape, -3.0, 3.3, -2.0, 2.5, -1.1, 1.9)
+    g = dgrid(og)
+    rng = np.random.default_rng(7)
+    nb = 6  # one group of four plus two single-orbital launches
+    v = rng.standard_normal(og.size)
+    psi = rng.standard_normal((nb, og.size))
+    ref = np.stack([oracle.hpsi(og, v, psi[o]) for o in range(nb)])
+    dv, dp, do = dev.array(v), dev.array(psi), dev.array(shape=(nb, og.size))
+    dev.hpsi(g, dv, dp, do)
+    got = do.download()
+    print("hpsi f64 rel", rel(got, ref), "bitwise", np.array_equal(got, ref))
+    assert rel(got, ref) <= REL64
+    dv32, dp32 = dev.array(v, dtype=np.float32), dev.array(psi, dtype=np.float32)
+    do32 = dev.array(shape=(nb, og.size), dtype=np.float32)
+    dev.hpsi(g, dv32, dp32, do32)
+    got32 = do32.download().astype(np.float64)
+    # FP32 inputs are rounded (2^-24) and the stencil sums ~8 terms
+    assert rel(got32, ref) <= 2e-6
+
+
+@pytest.mark.parametrize("case", ["poisson", "poissonq"])
+def test_poisson_golden_f64(dev, goldens, oracle, case):
+    og = grid_from(goldens[f"{case}_grid"])
+    g = dgrid(og)
+    rho = dev.array(goldens[f"{case}_rho"])
+    v = dev.zeros((og.size,))
+    sweeps, md = dev.poisson(g, rho, v, max_sweeps=10000, tol=1e-8, zero_start=True, batch=64)
+    _, osw, omd = oracle.poisson(og, goldens[f"{case}_rho"])
+    got = v.download()
+    print(case, "sweeps", sweeps, osw, "rel", rel(got, goldens[f"{case}_vh"]))
+    assert sweeps == osw
+    assert rel(got, goldens[f"{case}_vh"]) <= REL64
+    eh = dev.hartree_energy(g, rho, v)
+    assert abs(eh - oracle.hartree_energy(og, goldens[f"{case}_rho"], got)) <= 1e-12 * abs(eh)
+
+
+def test_poisson_fixed_warm_f64_and_f32(dev, oracle):
+    from oracle.oracle import Grid
+    og = Grid.centered(40, 0.3)
+    xs, ys, zs = og.coords()
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    rho = (np.exp(-1.2 * (X ** 2 + Y ** 2 + (Z - 0.3) ** 2)) * 0.9).ravel()
+    v0 = np.sin(X).ravel() * 0.01
+    ref, osw, _ = oracle.poisson(og, rho, v0=v0, max_sweeps=200, tol=0.0)
+    g = dgrid(og)
+    d_rho, d_v = dev.array(rho), dev.array(v0)
+    sweeps, _ = dev.poisson(g, d_rho, d_v, max_sweeps=200, tol=0.0, zero_start=False, batch=50)
+    assert sweeps == 200 == osw
+    assert rel(d_v.download(), ref) <= REL64
+    r32, v32 = dev.array(rho, dtype=np.float32), dev.array(v0, dtype=np.float32)
+    dev.poisson(g, r32, v32, max_sweeps=200, tol=0.0, zero_start=False, batch=200)
+    assert rel(v32.download(), ref) <= 1e-5
+
+
+def ulp_sensitivity(oracle, fid, pts):
+    """Largest relative change of each output under a 1-ulp change of any input
+    (computed with the CPU oracle): the conditioning floor of a pointwise test."""
+    base = oracle.xc_points(fid, pts)
+    sens = np.zeros_like(base)
+    for k in range(4):
+        q = pts.copy()
+        q[:, k] = np.nextafter(q[:, k], np.inf)
+        o = oracle.xc_points(fid, q)
+        sens = np.maximum(sens, np.abs(o - base) / np.maximum(np.abs(base), 1e-300))
+    return sens
+
+
+def test_xc_points_golden(dev, goldens, oracle):
+    """Relative 1e-12, except where the value itself is the residue of a
+    cancellation: there the bound is twice the oracle's own 1-ulp input
+    sensitivity (e.g. v_alpha ~ 5e-14 at n_alpha = 0, n_beta = 1e-6).  The
+    potential of an exactly empty spin channel (zeta = +-1, PBE96's ETA
+    regularisation, PBE96.cpp:700-713) is a cancellation of terms of the
+    size of the occupied channel's potential; it is bounded by 1e-10 of
+    max(|eps|, |v_a|, |v_b|) at that point."""
+    pts = goldens["xcpt_in"]
+    for fid, name in ((1, "pz81"), (2, "pbe96")):
+        got = dev.xc_points(fid, pts)
+        ref = goldens[f"xcpt_{name}"]
+        sens = ulp_sensitivity(oracle, fid, pts)
+        tol = (1e-12 + 2.0 * sens) * np.abs(ref)
+        scale = np.max(np.abs(ref[:, :3]), axis=1)
+        empty_a, empty_b = pts[:, 0] <= 0.0, pts[:, 1] <= 0.0
+        tol[empty_a, 1] = np.maximum(tol[empty_a, 1], 1e-10 * scale[empty_a])
+        tol[empty_b, 2] = np.maximum(tol[empty_b, 2], 1e-10 * scale[empty_b])
+        bad = np.abs(got - ref) > tol
+        assert not bad.any(), (name, np.argwhere(bad)[:5], got[bad][:5], ref[bad][:5])
+        well = (sens < 1e-13) & ~np.stack([np.zeros_like(empty_a), empty_a, empty_b, empty_a | empty_b,
+                                           empty_a | empty_b], 1)
+        assert np.max(np.abs(got - ref)[well] / np.maximum(np.abs(ref[well]), 1e-300)) <= 1e-12
+
+
+@pytest.mark.parametrize("fid,name", [(1, "pz81"), (2, "pbe96")])
+def test_xc_grid_golden_f64(dev, goldens, oracle, fid, name):
+    og = grid_from(goldens["xc_grid"])
+    g = dgrid(og)
+    na, nb = dev.array(goldens["xc_na"]), dev.array(goldens["xc_nb"])
+    va, vb = dev.array(shape=(og.size,)), dev.array(shape=(og.size,))
+    exc = dev.xc(g, fid, na, nb, va, vb)
+    ea, eb = rel(va.download(), goldens[f"xc_{name}_va"]), rel(vb.download(), goldens[f"xc_{name}_vb"])
+    print(name, "vxc rel", ea, eb, "exc", exc, float(goldens[f"xc_{name}_exc"]))
+    assert ea <= REL64 and eb <= REL64
+    assert abs(exc - float(goldens[f"xc_{name}_exc"])) <= 1e-12 * abs(exc)
+
+
+@pytest.mark.parametrize("fid", [1, 2])
+def test_xc_f32_vs_oracle(dev, oracle, fid):
+    from oracle.oracle import Grid
+    og = Grid.centered(32, 0.3)
+    xs, ys, zs = og.coords()
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    r2 = X ** 2 + Y ** 2 + Z ** 2
+    na = (2.0 * np.exp(-2.0 * np.sqrt(r2 + 0.01))).ravel()
+    nb = (1.5 * np.exp(-1.7 * np.sqrt(r2 + 0.02))).ravel()
+    ref_a = oracle.xc_potential(og, fid, na, nb, 0)
+    ref_e = oracle.xc_energy(og, fid, na, nb)
+    g = dgrid(og)
+    a32, b32 = dev.array(na, dtype=np.float32), dev.array(nb, dtype=np.float32)
+    va, vb = dev.array(shape=(og.size,), dtype=np.float32), dev.array(shape=(og.size,), dtype=np.float32)
+    e = dev.xc(g, fid, a32, b32, va, vb)
+    got = va.download().astype(np.float64)
+    assert np.all(np.isfinite(got))
+    print("xc f32", fid, rel(got, ref_a), abs(e - ref_e) / abs(ref_e))
+    assert rel(got, ref_a) <= 2e-4
+    assert abs(e - ref_e) <= 1e-5 * abs(ref_e)
+
+
+def test_density_and_energies(dev, goldens, oracle):
+    og = grid_from(goldens["misc_grid"])
+    g = dgrid(og)
+    psi, el = goldens["misc_psi"], goldens["misc_el"]
+    dp = dev.array(psi)
+    out = dev.array(shape=(og.size,))
+    dev.density(dp, el.astype(float), out)
+    assert rel(out.download(), goldens["misc_density"]) <= REL64
+    t = dev.kinetic_energy(g, dp, el)
+    assert abs(t - float(goldens["misc_kinetic"])) <= 1e-12 * abs(t)
+    vn = dev.array(shape=(og.size,))
+    dev.nuclear_potential(g, goldens["misc_nuc"], vn)
+    assert rel(vn.download(), goldens["misc_vnuc"]) <= REL64
+    rho = dev.array(goldens["misc_rho"])
+    import dft_amd
+    eext = dev.reduce(dft_amd.RED_DOT, rho, vn) * og.dx * og.dy * og.dz
+    assert abs(eext - float(goldens["misc_external"])) <= 1e-12 * abs(eext)
+
+
+def test_nuclear_potential_singular(dev):
+    import dft_amd
+    g = dft_amd.Grid.make(5, 5, 5, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+    out = dev.array(shape=(g.size,))
+    with pytest.raises(dft_amd.DftError):
+        dev.nuclear_potential(g, [[1, 0.0, 0.0, 0.0]], out)
+
+
+@pytest.mark.parametrize("n,ma,mb", [(5000, 8, 8), (70001, 45, 17), (300000, 96, 96)])
+def test_gram_modes(dev, n, ma, mb):
+    import dft_amd
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((ma, n))
+    B = rng.standard_normal((mb, n))
+    ref = A @ B.T
+    out = dev.zeros((mb, ma))  # column-major ma x mb == row-major (mb, ma)
+    dev.gram(dft_amd.GRAM_F64, dev.array(A), dev.array(B), out, n, ma, mb)
+    got = out.download().T
+    print("gram f64", rel(got, ref))
+    assert rel(got, ref) <= 1e-13
+    a32, b32 = A.astype(np.float32), B.astype(np.float32)
+    ref32 = a32.astype(np.float64) @ b32.astype(np.float64).T
+    for mode, tol in ((dft_amd.GRAM_F32, 1e-5), (dft_amd.GRAM_SPLIT_BF16, 1e-4)):
+        dev.gram(mode, dev.array(a32, dtype=np.float32), dev.array(b32, dtype=np.float32), out, n, ma, mb)
+        got = out.download().T
+        print("gram mode", mode, rel(got, ref32))
+        assert rel(got, ref32) <= tol
+
+
+@pytest.mark.parametrize("n,m,k", [(1000, 8, 8), (50001, 37, 70), (200000, 192, 96)])
+def test_block_update(dev, n, m, k):
+    rng = np.random.default_rng(m)
+    A = rng.standard_normal((m, n))
+    Cm = rng.standard_normal((k, m))  # column-major m x k
+    Y0 = rng.standard_normal((k, n))
+    ref = 0.5 * (Cm @ A) + 2.0 * Y0
+    dy = dev.array(Y0)
+    dev.block_update(dev.array(A), dev.array(Cm), dy, n, m, k, alpha=0.5, beta=2.0)
+    assert rel(dy.download(), ref) <= 1e-13
+    dy32 = dev.array(Y0, dtype=np.float32)
+    dev.block_update(dev.array(A, dtype=np.float32), dev.array(Cm), dy32, n, m, k, alpha=0.5, beta=2.0)
+    assert rel(dy32.download(), ref) <= 1e-5
+
+
+def test_syev_and_cholinv(dev):
+    rng = np.random.default_rng(2)
+    m = 40
+    M = rng.standard_normal((m, m))
+    S = M @ M.T + m * np.eye(m)
+    da = dev.array(S)
+    w = dev.syev(da, m)
+    wr = np.linalg.eigvalsh(S)
+    assert np.max(np.abs(w.download() - wr)) <= 1e-10 * wr.max()
+    V = da.download().T  # column-major eigenvectors
+    assert np.max(np.abs(S @ V - V * w.download())) <= 1e-9 * wr.max()
+    dl = dev.array(S)
+    dev.chol_inv(dl, m)
+    C = dl.download().T  # column-major -> C = L^{-T}, upper triangular
+    assert np.all(np.tril(C, -1) == 0.0)
+    L = np.linalg.cholesky(S)
+    assert np.max(np.abs(C.T @ L - np.eye(m))) <= 1e-12
+    # CholQR: T C has orthonormal columns when S = T^T T
+    T = rng.standard_normal((500, m))
+    dt = dev.array(T.T @ T)
+    dev.chol_inv(dt, m)
+    Q = T @ dt.download().T
+    assert np.max(np.abs(Q.T @ Q - np.eye(m))) <= 1e-10

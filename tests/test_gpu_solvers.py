@@ -1,0 +1,84 @@
+"""GPU block Davidson and device-resident SCF against the oracle and the
+reference's known answers.  Bars: occupied eigenvalues within 1e-8 Ha and
+total energies within 1e-7 Ha of the FP64 CPU path (reference rules)."""
+import numpy as np
+import pytest
+
+from conftest import grid_from
+
+pytestmark = pytest.mark.gpu
+
+
+def dgrid(og):
+    import dft_amd
+    return dft_amd.Grid.make(*og.n, *og.bounds)
+
+
+def test_davidson_golden_eigenvalues(goldens):
+    import dft_amd
+    og = grid_from(goldens["dav_grid"])
+    r = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4, want_psi=True)
+    print("davidson", r["eigenvalues"][:4], goldens["dav_eig"], r["iterations"])
+    assert np.max(np.abs(r["eigenvalues"][:4] - goldens["dav_eig"])) <= 1e-8
+    assert np.all(r["residuals"][:4] < 1e-7)
+    dv = og.dx * og.dy * og.dz
+    psi = r["psi"]
+    np.testing.assert_allclose(psi @ psi.T * dv, np.eye(4), atol=1e-9)
+    # same subspace as the reference's orbitals: projector distance
+    ref = goldens["dav_psi"]
+    Pg = psi.T @ psi * dv
+    Pr = ref.T @ ref * dv
+    assert np.max(np.abs(Pg - Pr)) <= 1e-6
+
+
+def test_davidson_vs_dense(oracle):
+    import dft_amd
+    from oracle.oracle import Grid
+    og = Grid.centered(6, 0.6)
+    v = oracle.nuclear_potential(og, [[2, 0.05, 0.1, -0.02]])
+    eye = np.eye(og.size)
+    H = np.stack([oracle.hpsi(og, v, eye[i]) for i in range(og.size)], 1)
+    w = np.linalg.eigvalsh(H)
+    r = dft_amd.davidson_run(dgrid(og), v, nstates=5, nconv=3)
+    assert np.max(np.abs(r["eigenvalues"][:3] - w[:3])) <= 1e-9
+
+
+@pytest.mark.parametrize("gram", ["f32", "split"])
+def test_davidson_mixed_precision(goldens, gram):
+    import dft_amd
+    og = grid_from(goldens["dav_grid"])
+    mode = dft_amd.GRAM_F32 if gram == "f32" else dft_amd.GRAM_SPLIT_BF16
+    r = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4, prec=dft_amd.F32, gram_mode=mode,
+                             tol=5e-5)
+    err = np.max(np.abs(r["eigenvalues"][:4] - goldens["dav_eig"]))
+    print("mixed davidson", gram, err, r["iterations"])
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("name,total,steps", [("h2", -2.8564865005, 29), ("h2o", -101.39828181, 87)])
+def test_scf_main_cpp_known_answers(goldens, name, total, steps):
+    import dft_amd
+    og = grid_from(goldens[f"scf_{name}_grid"])
+    r = dft_amd.scf_run(dgrid(og), goldens[f"scf_{name}_nuc"], functional=int(goldens[f"scf_{name}_fid"]))
+    ref = goldens[f"scf_{name}_result"]
+    print(name, r["total"], ref[6], r["iterations"], steps)
+    assert r["converged"]
+    assert abs(r["total"] - ref[6]) <= 1e-7
+    assert abs(r["total"] - total) <= 1e-7 + 5e-11 * abs(total)
+    # Two SCF runs that each satisfy the reference's stop rule (dE < 1e-11,
+    # dRho < 1e-9, KS < 1e-7) differ by up to ~1e-7 in eigenvalues (the stop
+    # rule bounds the step, not the distance to the fixed point); the 1e-8
+    # eigenvalue bar is asserted on identical inputs in test_scf_step_parity.
+    ne = len(r["eigenvalues"])
+    assert np.max(np.abs(r["eigenvalues"] - goldens[f"scf_{name}_eig"][:ne])) <= 1e-7
+
+
+def test_scf_he_pbe_golden(goldens):
+    import dft_amd
+    og = grid_from(goldens["scf_he_pbe_grid"])
+    r = dft_amd.scf_run(dgrid(og), goldens["scf_he_pbe_nuc"], functional=2)
+    ref = goldens["scf_he_pbe_result"]
+    print("he pbe", r["total"], ref[6], r["energy"], ref[1:6])
+    assert r["converged"]
+    np.testing.assert_allclose(r["energy"], ref[1:7], atol=1e-7)
+    assert abs(r["eigenvalues"][0] - goldens["scf_he_pbe_eig"][0]) <= 1e-8
