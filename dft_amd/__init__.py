@@ -8,6 +8,7 @@ native libraries are missing -- there is no Python or CPU fallback.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -114,6 +115,7 @@ class DeviceArray:
         p = C.c_void_p()
         dev._chk(dev.L.dft_alloc(dev.ctx, C.c_size_t(max(self.nbytes, 1)), C.byref(p)))
         self.ptr = p.value
+        dev._live.add(self)
 
     @property
     def prec(self):
@@ -136,9 +138,9 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.dev.ctx:
             self.dev.L.dft_free(self.dev.ctx, _vp(self.ptr))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -157,11 +159,21 @@ class Device:
         if rc != 0:
             raise DftError(rc, "dft_ctx_create failed (no GPU?)")
         self.ctx = ctx
+        self._live = weakref.WeakSet()
 
     def close(self):
+        """Free every array still alive on this context, then destroy it."""
         if self.ctx:
+            for a in list(self._live):
+                a.free()
             self.L.dft_ctx_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def _chk(self, rc):
         if rc != 0:
@@ -260,6 +272,10 @@ class Device:
 
     def chol_inv(self, a, m):
         self._chk(self.L.dft_chol_inv(self.ctx, C.c_int(m), _vp(a.ptr), C.c_int64(m)))
+
+    def seed_vectors(self, g, out, first, count):
+        self._chk(self.L.dft_seed_vectors(self.ctx, C.byref(g), C.c_int(out.prec), C.c_int(first), C.c_int(count),
+                                          _vp(out.ptr), C.c_int64(g.size)))
 
     def precond_diag(self, g, veff, r, eps, t, nb):
         self._chk(self.L.dft_precond_diag(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
@@ -365,3 +381,33 @@ def davidson_run(grid, veff, nstates, nconv=None, prec=F64, gram_mode=GRAM_F64, 
     if rc != 0:
         _core_err(rc)
     return dict(eigenvalues=eig, residuals=res, iterations=its.value, psi=psi)
+
+
+class ScfFields(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_double)) for k in ("vh", "vxc_alpha", "vxc_beta", "rho_alpha", "rho_beta", "vh_out")]
+
+
+def scf_step(grid, nuclei, rho_alpha, rho_beta, functional=PZ81, multiplicity=1, precision=SCF_FP64, nstates=0,
+             poisson_fixed=0, davidson_tol=1e-7):
+    """One SCF iteration on given input densities; returns fields and scalars."""
+    L = core_lib()
+    nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
+    n = grid.size
+    ra = np.ascontiguousarray(rho_alpha, dtype=np.float64)
+    rb = np.ascontiguousarray(rho_beta, dtype=np.float64)
+    cfg = ScfConfig(functional, 0, multiplicity, precision, nstates, 0, 1, poisson_fixed, 64, STOP_DE, 0.0,
+                    INIT_BARE, davidson_tol, 0, 0, PRECOND_DIAG, None, 0)
+    rep = ScfReport()
+    out = {k: np.zeros(n) for k in ("vh", "vxc_alpha", "vxc_beta", "rho_alpha", "rho_beta", "vh_out")}
+    f = ScfFields(*[out[k].ctypes.data_as(_dp) for k in ("vh", "vxc_alpha", "vxc_beta", "rho_alpha", "rho_beta", "vh_out")])
+    ne = int(round(nuc[:, 0].sum()))
+    eig = np.zeros(ne + 2)
+    rc = L.dft_scf_step(C.byref(grid), C.c_int(nuc.shape[0]), nuc.ctypes.data_as(_dp), C.byref(cfg),
+                        ra.ctypes.data_as(_dp), rb.ctypes.data_as(_dp), C.byref(rep), eig.ctypes.data_as(_dp), C.byref(f))
+    if rc != 0:
+        _core_err(rc)
+    out["energy"] = np.array(rep.energy)
+    out["eigenvalues"] = eig[:ne]
+    out["dRho"] = rep.density_difference
+    out["ks"] = rep.ks_residual
+    return out

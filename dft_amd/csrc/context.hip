@@ -165,14 +165,14 @@ int dft_ctx_create(int device, dft_ctx** out) {
 
 int dft_ctx_destroy(dft_ctx* c) {
     if (!c) return DFT_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->partials) hipFree(c->partials);
-    if (c->scratch) hipFree(c->scratch);
-    if (c->sweep_max) hipFree(c->sweep_max);
-    if (c->host_scalar) hipHostFree(c->host_scalar);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->sweep_max) (void)hipFree(c->sweep_max);
+    if (c->host_scalar) (void)hipHostFree(c->host_scalar);
     dft::release_blas(c);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return DFT_OK;
 }
@@ -180,6 +180,7 @@ int dft_ctx_destroy(dft_ctx* c) {
 const char* dft_last_error(const dft_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int dft_sync(dft_ctx* c) {
+    if (!c) return DFT_EINVAL;
     DFT_HIP(c, hipStreamSynchronize(c->stream));
     return DFT_OK;
 }
@@ -187,6 +188,7 @@ int dft_sync(dft_ctx* c) {
 void* dft_stream(dft_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int dft_alloc(dft_ctx* c, size_t bytes, void** p) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, p != nullptr, "dft_alloc: null out pointer");
     *p = nullptr;
     if (bytes == 0) return DFT_OK;
@@ -195,6 +197,7 @@ int dft_alloc(dft_ctx* c, size_t bytes, void** p) {
 }
 
 int dft_free(dft_ctx* c, void* p) {
+    if (!c) return DFT_EINVAL;
     if (!p) return DFT_OK;
     DFT_HIP(c, hipStreamSynchronize(c->stream));
     DFT_HIP(c, hipFree(p));
@@ -202,6 +205,7 @@ int dft_free(dft_ctx* c, void* p) {
 }
 
 int dft_upload(dft_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return DFT_EINVAL;
     if (!bytes) return DFT_OK;
     DFT_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     DFT_HIP(c, hipStreamSynchronize(c->stream));
@@ -209,6 +213,7 @@ int dft_upload(dft_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 int dft_download(dft_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return DFT_EINVAL;
     if (!bytes) return DFT_OK;
     DFT_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     DFT_HIP(c, hipStreamSynchronize(c->stream));
@@ -216,18 +221,21 @@ int dft_download(dft_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 int dft_copy(dft_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c) return DFT_EINVAL;
     if (!bytes) return DFT_OK;
     DFT_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
     return DFT_OK;
 }
 
 int dft_zero(dft_ctx* c, void* p, size_t bytes) {
+    if (!c) return DFT_EINVAL;
     if (!bytes) return DFT_OK;
     DFT_HIP(c, hipMemsetAsync(p, 0, bytes, c->stream));
     return DFT_OK;
 }
 
 int dft_convert(dft_ctx* c, const void* x, int ps, void* y, int pd, int64_t n) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, n >= 0 && (ps == DFT_F64 || ps == DFT_F32) && (pd == DFT_F64 || pd == DFT_F32), "dft_convert: bad args");
     if (!n) return DFT_OK;
     if (ps == DFT_F64 && pd == DFT_F32) k_convert<double, float><<<grid1d(n), 256, 0, c->stream>>>((const double*)x, (float*)y, n);
@@ -238,26 +246,31 @@ int dft_convert(dft_ctx* c, const void* x, int ps, void* y, int pd, int64_t n) {
 }
 
 int dft_event_create(dft_ctx* c, void** ev) {
+    if (!c) return DFT_EINVAL;
     hipEvent_t e;
     DFT_HIP(c, hipEventCreate(&e));
     *ev = (void*)e;
     return DFT_OK;
 }
 int dft_event_record(dft_ctx* c, void* ev) {
+    if (!c) return DFT_EINVAL;
     DFT_HIP(c, hipEventRecord((hipEvent_t)ev, c->stream));
     return DFT_OK;
 }
 int dft_event_elapsed_ms(dft_ctx* c, void* a, void* b, float* ms) {
+    if (!c) return DFT_EINVAL;
     DFT_HIP(c, hipEventSynchronize((hipEvent_t)b));
     DFT_HIP(c, hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
     return DFT_OK;
 }
 int dft_event_destroy(dft_ctx* c, void* ev) {
+    if (!c) return DFT_EINVAL;
     DFT_HIP(c, hipEventDestroy((hipEvent_t)ev));
     return DFT_OK;
 }
 
 int dft_reduce(dft_ctx* c, int op, int prec, const void* a, const void* b, int64_t n, double* out) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, out && a && n >= 0 && op >= 0 && op <= 3, "dft_reduce: bad args");
     DFT_REQUIRE(c, (op != DFT_RED_DOT && op != DFT_RED_DIFF2) || b, "dft_reduce: second operand required");
     if (n == 0) { *out = 0.0; return DFT_OK; }
@@ -285,6 +298,7 @@ int dft_reduce(dft_ctx* c, int op, int prec, const void* a, const void* b, int64
 }
 
 int dft_axpby(dft_ctx* c, int prec, int64_t n, double alpha, const void* x, double beta, void* y) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, n >= 0 && x && y, "dft_axpby: bad args");
     if (!n) return DFT_OK;
     if (prec == DFT_F64) k_axpby<double><<<grid1d(n), 256, 0, c->stream>>>(n, alpha, (const double*)x, beta, (double*)y);
@@ -294,6 +308,7 @@ int dft_axpby(dft_ctx* c, int prec, int64_t n, double alpha, const void* x, doub
 }
 
 int dft_add3(dft_ctx* c, int prec, int64_t n, const void* a, const void* b, const void* d, void* out) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, n >= 0 && a && b && out, "dft_add3: bad args");
     if (!n) return DFT_OK;
     if (prec == DFT_F64) k_add3<double><<<grid1d(n), 256, 0, c->stream>>>(n, (const double*)a, (const double*)b, (const double*)d, (double*)out);

@@ -93,6 +93,7 @@ extern "C" {
 
 int dft_density(dft_ctx* c, int prec, const void* psi, int64_t ld, int nb, const double* occ_host, int64_t n, void* out,
                 int accumulate) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, psi && out && n > 0 && ld >= n && nb >= 0 && (nb == 0 || occ_host), "dft_density: bad arguments");
     if (int rc = ensure_partials(c, (size_t)nb + 8)) return rc;
     double* occ = c->partials;
@@ -107,6 +108,7 @@ int dft_density(dft_ctx* c, int prec, const void* psi, int64_t ld, int nb, const
 }
 
 int dft_nuclear_potential(dft_ctx* c, const dft_grid* g, int prec, int natoms, const double* nuclei_host, void* v) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && v && natoms >= 0 && (natoms == 0 || nuclei_host), "dft_nuclear_potential: bad arguments");
     const int64_t n = grid_points(g);
     if (int rc = ensure_partials(c, (size_t)4 * natoms + 16)) return rc;
@@ -129,6 +131,7 @@ int dft_nuclear_potential(dft_ctx* c, const dft_grid* g, int prec, int natoms, c
 
 int dft_scale_columns(dft_ctx* c, int prec, int64_t n, int nb, const void* x, int64_t ldx, const double* s, void* y,
                       int64_t ldy) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, x && y && s && n > 0 && nb >= 0 && nb <= 65535 && ldx >= n && ldy >= n, "dft_scale_columns: bad arguments");
     if (!nb) return DFT_OK;
     dim3 g((unsigned)((n + 255) / 256), nb);
@@ -140,6 +143,7 @@ int dft_scale_columns(dft_ctx* c, int prec, int64_t n, int nb, const void* x, in
 
 int dft_residual(dft_ctx* c, int prec, int64_t n, int nb, const void* hx, int64_t ldh, const void* x, int64_t ldx,
                  const double* th, void* r, int64_t ldr) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, hx && x && th && r && n > 0 && nb >= 0 && nb <= 65535, "dft_residual: bad arguments");
     if (!nb) return DFT_OK;
     dim3 g((unsigned)((n + 255) / 256), nb);
@@ -150,6 +154,7 @@ int dft_residual(dft_ctx* c, int prec, int64_t n, int nb, const void* hx, int64_
 }
 
 int dft_column_norms2(dft_ctx* c, int prec, int64_t n, int nb, const void* x, int64_t ldx, double* out) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, x && out && n > 0 && nb >= 0 && nb <= 65535 && ldx >= n, "dft_column_norms2: bad arguments");
     if (!nb) return DFT_OK;
     const int nblk = (int)std::min<int64_t>((n + 255) / 256, std::max<int64_t>(8, (4 * (int64_t)c->num_cus + nb - 1) / nb));

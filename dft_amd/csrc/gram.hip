@@ -1,5 +1,5 @@
-// gram.hip -- (e) the Davidson Gram / Rayleigh-Ritz products A^T B and the
-// tall-skinny block update Y = A C, on the CDNA4 matrix cores.
+// gram.hip -- (e) the Davidson Gram / Rayleigh-Ritz products A^T B on the
+// CDNA4 matrix cores.
 //
 // A^T B with A: N x ma, B: N x mb, N (grid points) >> ma, mb (<= a few
 // hundred).  The long dimension N is split across workgroups; each
@@ -59,95 +59,162 @@ struct GramT<DFT_GRAM_SPLIT_BF16> {
     static constexpr int KSTEP = 32;
 };
 
+// Workgroup = 4 waves in 2 x 2, each wave 3 x 3 MFMA 16x16 tiles: a 96 x 96
+// output tile over one K-chunk.  Each K-slab (KS rows of the 96 A columns and
+// 96 B columns) is read with 16-byte loads (8 lanes cover a 128-byte column
+// segment) and stored in LDS in operand order; for SPLIT_BF16 the FP32 values
+// are split into hi/lo bf16 on the way in.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm, const typename GramT<MODE>::In* __restrict__ A,
-                                              int64_t lda, const typename GramT<MODE>::In* __restrict__ B, int64_t ldb,
-                                              int64_t kchunk, double* __restrict__ part) {
-    using Tr = GramT<MODE>;
-    using In = typename Tr::In;
-    using Acc = typename Tr::Acc;
-    constexpr int KS = Tr::KSTEP;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tile = blockIdx.x, ks = blockIdx.y;
-    const int i0 = (tile % ntm) * 64 + (w >> 1) * 32;
-    const int j0 = (tile / ntm) * 64 + (w & 1) * 32;
-    const int64_t kbeg = (int64_t)ks * kchunk, kend = min(n, kbeg + kchunk);
-    const int r16 = lane & 15, kq = lane >> 4;
-    Acc acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = Acc{0, 0, 0, 0};
-    const int ia[2] = {i0 + r16, i0 + 16 + r16};
-    const int jb[2] = {j0 + r16, j0 + 16 + r16};
-    const bool va[2] = {ia[0] < ma, ia[1] < ma};
-    const bool vb[2] = {jb[0] < mb, jb[1] < mb};
-    const In* pa[2] = {A + (int64_t)(va[0] ? ia[0] : 0) * lda, A + (int64_t)(va[1] ? ia[1] : 0) * lda};
-    const In* pb[2] = {B + (int64_t)(vb[0] ? jb[0] : 0) * ldb, B + (int64_t)(vb[1] ? jb[1] : 0) * ldb};
+struct Slab;
+template <>
+struct Slab<DFT_GRAM_F64> {
+    static constexpr int KS = 16, PAD = 17;  // doubles per column in LDS
+};
+template <>
+struct Slab<DFT_GRAM_F32> {
+    static constexpr int KS = 16, PAD = 17;  // floats per column
+};
+template <>
+struct Slab<DFT_GRAM_SPLIT_BF16> {
+    static constexpr int KS = 32, PAD = 40;  // bf16 per column (80-byte stride, 16-byte aligned)
+};
 
-    if constexpr (MODE != DFT_GRAM_SPLIT_BF16) {
-        for (int64_t k = kbeg; k < kend; k += KS) {
-            const int64_t kk = k + kq;
-            const bool kin = kk < kend;
-            In xa[2], xb[2];
+constexpr int GT = 96;  // output tile edge
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm, int ntiles, int nks,
+                                              const typename GramT<MODE>::In* __restrict__ A, int64_t lda,
+                                              const typename GramT<MODE>::In* __restrict__ B, int64_t ldb,
+                                              int64_t kchunk, double* __restrict__ part) {
+    using In = typename GramT<MODE>::In;
+    using Acc = typename GramT<MODE>::Acc;
+    constexpr int KS = Slab<MODE>::KS, PAD = Slab<MODE>::PAD;
+    constexpr bool SPLIT = MODE == DFT_GRAM_SPLIT_BF16;
+    using Ld = typename std::conditional<SPLIT, __bf16, In>::type;
+    // LDS: [2 operands][GT columns][PAD]; SPLIT keeps hi and lo planes
+    __shared__ __attribute__((aligned(16))) Ld lds[(SPLIT ? 2 : 1) * 2 * GT * PAD];
+    Ld* ldsA = lds;
+    Ld* ldsB = lds + GT * PAD;
+    Ld* ldsAl = lds + 2 * GT * PAD;  // SPLIT lo planes
+    Ld* ldsBl = lds + 3 * GT * PAD;
+
+    // XCD-aware mapping: the ntiles tiles of one K-chunk run on one XCD
+    const int b = blockIdx.x;
+    const int xcd = b & 7, slot = b >> 3;
+    const int tile = slot % ntiles;
+    const int ks = (slot / ntiles) * 8 + xcd;
+    const int i0 = (tile % ntm) * GT, j0 = (tile / ntm) * GT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wi = (w >> 1) * 48, wj = (w & 1) * 48;  // wave's 48 x 48 block inside the tile
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int64_t kbeg = (int64_t)ks * kchunk, kend = min(n, kbeg + kchunk);
+
+    Acc acc[3][3];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) xa[a] = (va[a] && kin) ? pa[a][kk] : In(0);
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) xb[b] = (vb[b] && kin) ? pb[b][kk] : In(0);
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    if constexpr (MODE == DFT_GRAM_F64)
-                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a], xb[b], acc[a][b], 0, 0, 0);
-                    else
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[a], xb[b], acc[a][b], 0, 0, 0);
+        for (int c = 0; c < 3; ++c) acc[a][c] = Acc{0, 0, 0, 0};
+
+    // slab loader: 16-byte pieces; EPV elements per piece, PPC pieces per column
+    constexpr int EPV = 16 / sizeof(In);
+    constexpr int PPC = KS / EPV;
+    constexpr int COLS_PER_PASS = 256 / PPC;
+    const int lcol = threadIdx.x / PPC, lpiece = threadIdx.x % PPC;
+
+    for (int64_t k0 = kbeg; k0 < kend; k0 += KS) {
+        // ---- global -> LDS
+        for (int cbase = 0; cbase < 2 * GT; cbase += COLS_PER_PASS) {
+            const int cc = cbase + lcol;  // 0..2*GT-1: A columns then B columns
+            const bool isA = cc < GT;
+            const int col = isA ? i0 + cc : j0 + (cc - GT);
+            const bool cval = isA ? col < ma : col < mb;
+            const In* src = isA ? A + (int64_t)(cval ? col : 0) * lda : B + (int64_t)(cval ? col : 0) * ldb;
+            const int64_t kk = k0 + lpiece * EPV;
+            In v[EPV];
+            if (cval && kk + EPV <= kend && ((((uintptr_t)(src + kk)) & 15) == 0)) {
+                if constexpr (sizeof(In) == 8) {
+                    const double2 q = *reinterpret_cast<const double2*>(src + kk);
+                    v[0] = q.x; v[1] = q.y;
+                } else {
+                    const float4 q = *reinterpret_cast<const float4*>(src + kk);
+                    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
                 }
+            } else {
+#pragma unroll
+                for (int e = 0; e < EPV; ++e) v[e] = (cval && kk + e < kend) ? src[kk + e] : In(0);
+            }
+            const int lc = isA ? cc : cc - GT;
+            Ld* dst = (isA ? ldsA : ldsB) + lc * PAD + lpiece * EPV;
+            if constexpr (SPLIT) {
+                Ld* dstl = (isA ? ldsAl : ldsBl) + lc * PAD + lpiece * EPV;
+#pragma unroll
+                for (int e = 0; e < EPV; ++e) {
+                    const __bf16 h = (__bf16)v[e];
+                    dst[e] = h;
+                    dstl[e] = (__bf16)(v[e] - (float)h);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < EPV; ++e) dst[e] = v[e];
+            }
         }
-    } else {
-        for (int64_t k = kbeg; k < kend; k += KS) {
-            const int64_t kk = k + 8 * kq;
-            bf8 ah[2], al[2], bh[2], bl[2];
+        __syncthreads();
+        // ---- MFMA over the slab
+        if constexpr (SPLIT) {
+            bf8 ah[3], al[3], bh[3], bl[3];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 3; ++a) {
+                const int col = wi + 16 * a + r16;
+                ah[a] = *reinterpret_cast<const bf8*>(ldsA + col * PAD + 8 * kq);
+                al[a] = *reinterpret_cast<const bf8*>(ldsAl + col * PAD + 8 * kq);
+            }
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float x = (va[a] && kk + e < kend) ? pa[a][kk + e] : 0.0f;
-                    const __bf16 h = to_bf16(x);
-                    ah[a][e] = h;
-                    al[a][e] = to_bf16(x - (float)h);
+            for (int c = 0; c < 3; ++c) {
+                const int col = wj + 16 * c + r16;
+                bh[c] = *reinterpret_cast<const bf8*>(ldsB + col * PAD + 8 * kq);
+                bl[c] = *reinterpret_cast<const bf8*>(ldsBl + col * PAD + 8 * kq);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
                 }
+        } else {
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int k4 = 0; k4 < KS; k4 += 4) {
+                In xa[3], xb[3];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float x = (vb[b] && kk + e < kend) ? pb[b][kk + e] : 0.0f;
-                    const __bf16 h = to_bf16(x);
-                    bh[b][e] = h;
-                    bl[b][e] = to_bf16(x - (float)h);
-                }
+                for (int a = 0; a < 3; ++a) xa[a] = ldsA[(wi + 16 * a + r16) * PAD + k4 + kq];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+                for (int c = 0; c < 3; ++c) xb[c] = ldsB[(wj + 16 * c + r16) * PAD + k4 + kq];
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-                }
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if constexpr (MODE == DFT_GRAM_F64)
+                            acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a], xb[c], acc[a][c], 0, 0, 0);
+                        else
+                            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[a], xb[c], acc[a][c], 0, 0, 0);
+                    }
+            }
         }
+        __syncthreads();
     }
-    // partial tile -> part[ks][j*ma + i]
+    // partial tile -> part[ks][j*ma + i] (K-chunks past the end write zeros)
     double* P = part + (int64_t)ks * ma * mb;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int j = j0 + 16 * b + r16;
+        for (int c = 0; c < 3; ++c) {
+            const int j = j0 + wj + 16 * c + r16;
             if (j >= mb) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + 16 * a + res_row<MODE == DFT_GRAM_F64>(kq, r);
-                if (i < ma) P[(int64_t)j * ma + i] = (double)acc[a][b][r];
+                const int i = i0 + wi + 16 * a + res_row<MODE == DFT_GRAM_F64>(kq, r);
+                if (i < ma) P[(int64_t)j * ma + i] = (double)acc[a][c][r];
             }
         }
 }
@@ -164,118 +231,39 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ 
     *o = beta == 0.0 ? alpha * s : alpha * s + beta * *o;
 }
 
-// ------------------------------------------------------- block update Y = a A C + b Y
-// A: N x m (column-major), C: m x k FP64, Y: N x k.  Workgroup = 4 waves x
-// 16 rows = 64 rows of Y and WN*16 columns; the A operand (16 rows x 4 K)
-// is shared by the WN MFMA tiles of the wave.
-template <typename T, int WN>
-__global__ __launch_bounds__(256) void k_update(int64_t n, int m, int k, double alpha, const T* __restrict__ A,
-                                                int64_t lda, const double* __restrict__ C, int64_t ldc, double beta,
-                                                T* __restrict__ Y, int64_t ldy) {
-    using Acc = typename std::conditional<sizeof(T) == 8, d4, f4>::type;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * 64 + w * 16;
-    const int j0 = blockIdx.y * (16 * WN);
-    const int r16 = lane & 15, kq = lane >> 4;
-    const int64_t row = r0 + r16;
-    const bool rin = row < n;
-    Acc acc[WN];
-#pragma unroll
-    for (int b = 0; b < WN; ++b) acc[b] = Acc{0, 0, 0, 0};
-    int jc[WN];
-    bool jv[WN];
-#pragma unroll
-    for (int b = 0; b < WN; ++b) {
-        jc[b] = j0 + 16 * b + r16;
-        jv[b] = jc[b] < k;
-    }
-    for (int kk = 0; kk < m; kk += 4) {
-        const int q = kk + kq;
-        const bool qin = q < m;
-        const T a = (rin && qin) ? A[(int64_t)q * lda + row] : T(0);
-#pragma unroll
-        for (int b = 0; b < WN; ++b) {
-            const T cv = (jv[b] && qin) ? (T)C[(int64_t)jc[b] * ldc + q] : T(0);
-            if constexpr (sizeof(T) == 8)
-                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, cv, acc[b], 0, 0, 0);
-            else
-                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, cv, acc[b], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < WN; ++b) {
-        const int j = j0 + 16 * b + r16;
-        if (j >= k) continue;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t i = r0 + res_row<sizeof(T) == 8>(kq, r);
-            if (i >= n) continue;
-            T* y = Y + (int64_t)j * ldy + i;
-            const double v = alpha * (double)acc[b][r];
-            *y = beta == 0.0 ? (T)v : (T)(v + beta * (double)*y);
-        }
-    }
-}
-
-template <typename T>
-int update_t(dft_ctx* c, int64_t n, int m, int k, double alpha, const T* A, int64_t lda, const double* C, int64_t ldc,
-             double beta, T* Y, int64_t ldy) {
-    const unsigned gx = (unsigned)((n + 63) / 64);
-    // the widest column block up to 128 columns; A is read once per block
-    if (k <= 16) {
-        k_update<T, 1><<<dim3(gx, (k + 15) / 16), 256, 0, c->stream>>>(n, m, k, alpha, A, lda, C, ldc, beta, Y, ldy);
-    } else if (k <= 32) {
-        k_update<T, 2><<<dim3(gx, (k + 31) / 32), 256, 0, c->stream>>>(n, m, k, alpha, A, lda, C, ldc, beta, Y, ldy);
-    } else if (k <= 64) {
-        k_update<T, 4><<<dim3(gx, (k + 63) / 64), 256, 0, c->stream>>>(n, m, k, alpha, A, lda, C, ldc, beta, Y, ldy);
-    } else {
-        k_update<T, 8><<<dim3(gx, (k + 127) / 128), 256, 0, c->stream>>>(n, m, k, alpha, A, lda, C, ldc, beta, Y, ldy);
-    }
-    DFT_CHECK_LAUNCH(c);
-    return DFT_OK;
-}
-
 }  // namespace
 
 extern "C" {
 
 int dft_gram(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, const void* a, int64_t lda, const void* b,
              int64_t ldb, double beta, double* out, int64_t ldo) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, a && b && out && n > 0 && ma > 0 && mb > 0 && lda >= n && ldb >= n && ldo >= ma, "dft_gram: bad arguments");
     DFT_REQUIRE(c, mode >= DFT_GRAM_F64 && mode <= DFT_GRAM_SPLIT_BF16, "dft_gram: bad mode");
-    const int ntm = (ma + 63) / 64, ntn = (mb + 63) / 64;
+    const int ntm = (ma + GT - 1) / GT, ntn = (mb + GT - 1) / GT;
     const int ntiles = ntm * ntn;
-    const int kstep = mode == DFT_GRAM_SPLIT_BF16 ? 32 : 4;
-    // K-split: enough workgroups to fill the chip twice over, K-chunks a multiple of the MFMA K step
-    int64_t nks = std::max<int64_t>(1, (8LL * c->num_cus + ntiles - 1) / ntiles);
-    int64_t kchunk = (n + nks - 1) / nks;
-    kchunk = std::max<int64_t>(kchunk, 512);
-    kchunk = (kchunk + kstep - 1) / kstep * kstep;
+    const int ks = mode == DFT_GRAM_SPLIT_BF16 ? 32 : 16;
+    // K-split: about four workgroups per CU in total, a multiple of 8 chunks
+    // (one group of chunks per XCD), chunks a multiple of the slab depth
+    int64_t nks = std::max<int64_t>(1, (4LL * c->num_cus + ntiles - 1) / ntiles);
+    int64_t kchunk = std::max<int64_t>((n + nks - 1) / nks, 4 * ks);
+    kchunk = (kchunk + ks - 1) / ks * ks;
     nks = (n + kchunk - 1) / kchunk;
-    DFT_REQUIRE(c, nks <= 65535, "dft_gram: too many K splits");
+    nks = (nks + 7) / 8 * 8;
+    DFT_REQUIRE(c, nks * ntiles < (1LL << 31), "dft_gram: too many workgroups");
     if (int rc = ensure_partials(c, (size_t)nks * ma * mb + 8)) return rc;
-    dim3 grid(ntiles, (unsigned)nks);
+    const unsigned nblk = (unsigned)(nks * ntiles);
     if (mode == DFT_GRAM_F64)
-        k_gram<DFT_GRAM_F64><<<grid, 256, 0, c->stream>>>(n, ma, mb, ntm, (const double*)a, lda, (const double*)b, ldb, kchunk, c->partials);
+        k_gram<DFT_GRAM_F64><<<nblk, 256, 0, c->stream>>>(n, ma, mb, ntm, ntiles, (int)nks, (const double*)a, lda, (const double*)b, ldb, kchunk, c->partials);
     else if (mode == DFT_GRAM_F32)
-        k_gram<DFT_GRAM_F32><<<grid, 256, 0, c->stream>>>(n, ma, mb, ntm, (const float*)a, lda, (const float*)b, ldb, kchunk, c->partials);
+        k_gram<DFT_GRAM_F32><<<nblk, 256, 0, c->stream>>>(n, ma, mb, ntm, ntiles, (int)nks, (const float*)a, lda, (const float*)b, ldb, kchunk, c->partials);
     else
-        k_gram<DFT_GRAM_SPLIT_BF16><<<grid, 256, 0, c->stream>>>(n, ma, mb, ntm, (const float*)a, lda, (const float*)b, ldb, kchunk, c->partials);
+        k_gram<DFT_GRAM_SPLIT_BF16><<<nblk, 256, 0, c->stream>>>(n, ma, mb, ntm, ntiles, (int)nks, (const float*)a, lda, (const float*)b, ldb, kchunk, c->partials);
     DFT_CHECK_LAUNCH(c);
     const int64_t tot = (int64_t)ma * mb;
     k_gram_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(c->partials, (int)nks, ma, mb, alpha, beta, out, ldo);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
-}
-
-int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
-                     const double* cm, int64_t ldc, double beta, void* y, int64_t ldy) {
-    DFT_REQUIRE(c, a && cm && y && n > 0 && m >= 0 && k >= 0 && lda >= n && ldy >= n && ldc >= m, "dft_block_update: bad arguments");
-    DFT_REQUIRE(c, (n + 63) / 64 < (1LL << 31), "dft_block_update: n too large");
-    if (k == 0) return DFT_OK;
-    if (prec == DFT_F64) return update_t<double>(c, n, m, k, alpha, (const double*)a, lda, cm, ldc, beta, (double*)y, ldy);
-    if (prec == DFT_F32) return update_t<float>(c, n, m, k, alpha, (const float*)a, lda, cm, ldc, beta, (float*)y, ldy);
-    return set_err(c, DFT_EINVAL, "dft_block_update: bad precision");
 }
 
 }  // extern "C"

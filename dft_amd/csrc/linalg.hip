@@ -10,6 +10,12 @@
 using namespace dft;
 
 namespace {
+__global__ __launch_bounds__(256) void k_to_f32(int m, int k, const double* __restrict__ c, int64_t ldc, float* __restrict__ o) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * k) return;
+    const int i = (int)(e % m), j = (int)(e / m);
+    o[e] = (float)c[(int64_t)j * ldc + i];
+}
 // a (upper, i <= j) <- tmp(j, i) from the lower triangle of tmp; zeros below
 __global__ __launch_bounds__(256) void k_lower_t(int m, const double* __restrict__ tmp, double* __restrict__ a, int64_t lda) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -29,7 +35,7 @@ namespace dft {
 
 void release_blas(dft_ctx* c) {
     if (c->blas) rocblas_destroy_handle((rocblas_handle)c->blas);
-    if (c->dev_info) hipFree(c->dev_info);
+    if (c->dev_info) (void)hipFree(c->dev_info);
     c->blas = nullptr;
     c->dev_info = nullptr;
 }
@@ -51,7 +57,37 @@ static int blas(dft_ctx* c, rocblas_handle* h) {
 
 extern "C" {
 
+// Block update Y = alpha A C + beta Y (a plain tall-skinny GEMM, N x m times
+// m x k): rocBLAS' 64-bit-index GEMM on the context stream; FP32 blocks use
+// an FP32 copy of the small coefficient matrix.
+int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
+                     const double* cm, int64_t ldc, double beta, void* y, int64_t ldy) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, a && cm && y && n > 0 && m > 0 && k >= 0 && lda >= n && ldy >= n && ldc >= m, "dft_block_update: bad arguments");
+    if (k == 0) return DFT_OK;
+    rocblas_handle h;
+    if (int rc = blas(c, &h)) return rc;
+    rocblas_status st;
+    if (prec == DFT_F64) {
+        st = rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &alpha, (const double*)a, lda,
+                              cm, ldc, &beta, (double*)y, ldy);
+    } else if (prec == DFT_F32) {
+        if (int rc = ensure_scratch(c, sizeof(float) * (size_t)m * k)) return rc;
+        float* c32 = (float*)c->scratch;
+        k_to_f32<<<(unsigned)(((int64_t)m * k + 255) / 256), 256, 0, c->stream>>>(m, k, cm, ldc, c32);
+        DFT_CHECK_LAUNCH(c);
+        const float al = (float)alpha, be = (float)beta;
+        st = rocblas_sgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &al, (const float*)a, lda, c32,
+                              m, &be, (float*)y, ldy);
+    } else {
+        return set_err(c, DFT_EINVAL, "dft_block_update: bad precision");
+    }
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "rocblas gemm failed");
+    return DFT_OK;
+}
+
 int dft_syev(dft_ctx* c, int m, double* a, int64_t lda, double* w) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, a && w && m > 0 && lda >= m && m <= 4096, "dft_syev: bad arguments");
     rocblas_handle h;
     if (int rc = blas(c, &h)) return rc;
@@ -67,6 +103,7 @@ int dft_syev(dft_ctx* c, int m, double* a, int64_t lda, double* w) {
 }
 
 int dft_chol_inv(dft_ctx* c, int m, double* a, int64_t lda) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, a && m > 0 && lda >= m && m <= 4096, "dft_chol_inv: bad arguments");
     rocblas_handle h;
     if (int rc = blas(c, &h)) return rc;

@@ -10,6 +10,8 @@
 // host finds the exact sweep at which the reference would have stopped.
 #include "common.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -43,63 +45,65 @@ struct JCoef {
     T cx, cy, cz, den, fourpi;
 };
 
+// One sweep over the tile (bx, by, bz): 64 x-points, 4*TY y-rows, zc planes.
+// Returns this thread's max|V_new - V_old| over the interior points it updated.
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_jacobi(int nx, int ny, int nz, int zc, JCoef<T> co, const T* __restrict__ rho,
-                                                const T* __restrict__ cur_in, T* __restrict__ out,
-                                                unsigned long long* __restrict__ slot) {
-    __shared__ double red[4];
+__device__ inline T jacobi_tile(int nx, int ny, int nz, int zc, const JCoef<T>& co, const T* __restrict__ rho,
+                                const T* __restrict__ cur_in, T* __restrict__ out, int bx, int by, int bz) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 64, x = x0 + lane;
-    const int ybase = blockIdx.y * (4 * TY) + w * TY;
-    const int k0 = blockIdx.z * zc, k1 = min(nz, k0 + zc);
-    const int64_t plane = (int64_t)nx * ny;
+    const int x0 = bx * 64, x = x0 + lane;
+    const int ybase = by * (4 * TY) + w * TY;
+    const int k0 = bz * zc, k1 = min(nz, k0 + zc);
+    const int plane = nx * ny;  // dft_poisson guarantees nx*ny*nz < 2^31: 32-bit offsets
     const bool xin = x < nx;
     const bool xint = x > 0 && x + 1 < nx;
-    const bool has_xm = x0 > 0, has_xp = x0 + 64 < nx;
+    // x-halo: lane 0 reads x0-1, lane 63 reads x0+64; other lanes stay masked off
+    int hx = x;
+    bool hin = false;
+    if (lane == 0) { hx = x0 - 1; hin = x0 > 0; }
+    if (lane == 63) { hx = x0 + 64; hin = x0 + 64 < nx; }
     bool yin[TY], yint[TY];
-    int64_t rowoff[TY];
+    int roff[TY], hoff[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
         yin[r] = (ybase + r) < ny;
         yint[r] = (ybase + r) > 0 && (ybase + r) + 1 < ny;
-        rowoff[r] = (int64_t)(ybase + r) * nx;
+        roff[r] = (ybase + r) * nx + x;
+        hoff[r] = (ybase + r) * nx + hx;
     }
     const bool ym_in = ybase > 0 && ybase - 1 < ny, yp_in = ybase + TY < ny;
-    const int64_t ymoff = (int64_t)(ybase - 1) * nx, ypoff = (int64_t)(ybase + TY) * nx;
+    const int ymo = (ybase - 1) * nx + x, ypo = (ybase + TY) * nx + x;
     T prev[TY], cur[TY], nxt[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
         const bool ok = xin && yin[r];
-        prev[r] = (ok && k0 > 0) ? cur_in[(int64_t)(k0 - 1) * plane + rowoff[r] + x] : T(0);
-        cur[r] = ok ? cur_in[(int64_t)k0 * plane + rowoff[r] + x] : T(0);
+        prev[r] = (ok && k0 > 0) ? cur_in[(k0 - 1) * plane + roff[r]] : T(0);
+        cur[r] = ok ? cur_in[k0 * plane + roff[r]] : T(0);
     }
     T md = T(0);
     for (int k = k0; k < k1; ++k) {
-        const int64_t pk = (int64_t)k * plane;
+        const int pk = k * plane;
         const bool has_next = k + 1 < nz;
         const bool kint = k > 0 && k + 1 < nz;
-        const T* p = cur_in + pk;
 #pragma unroll
-        for (int r = 0; r < TY; ++r) nxt[r] = (xin && yin[r] && has_next) ? p[plane + rowoff[r] + x] : T(0);
-        const T ym = (xin && ym_in) ? p[ymoff + x] : T(0);
-        const T yp = (xin && yp_in) ? p[ypoff + x] : T(0);
+        for (int r = 0; r < TY; ++r) nxt[r] = (xin && yin[r] && has_next) ? cur_in[pk + plane + roff[r]] : T(0);
+        const T ym = (xin && ym_in) ? cur_in[pk + ymo] : T(0);
+        const T yp = (xin && yp_in) ? cur_in[pk + ypo] : T(0);
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
-            T hm = T(0), hp = T(0);
-            if (lane == 0 && has_xm && yin[r]) hm = p[rowoff[r] + x0 - 1];
-            if (lane == 63 && has_xp && yin[r]) hp = p[rowoff[r] + x0 + 64];
+            const T h = (hin && yin[r]) ? cur_in[pk + hoff[r]] : T(0);
             const T c = cur[r];
-            const T xm = shr1(c, hm);
-            const T xp = shl1(c, hp);
+            const T xm = shr1(c, h);
+            const T xp = shl1(c, h);
             const T ymv = r > 0 ? cur[r > 0 ? r - 1 : 0] : ym;
             const T ypv = r < TY - 1 ? cur[r < TY - 1 ? r + 1 : 0] : yp;
             if (xint && yint[r] && kint) {
                 // HartreePotentialMolecule.cpp:295-322
-                const T src = co.fourpi * rho[pk + rowoff[r] + x];
+                const T src = co.fourpi * rho[pk + roff[r]];
                 const T nv = (co.cx * (xm + xp) + co.cy * (ymv + ypv) + co.cz * (prev[r] + nxt[r]) + src) / co.den;
                 const T d = nv > c ? nv - c : c - nv;
                 md = md < d ? d : md;
-                out[pk + rowoff[r] + x] = nv;
+                out[pk + roff[r]] = nv;
             }
         }
 #pragma unroll
@@ -108,13 +112,53 @@ __global__ __launch_bounds__(256) void k_jacobi(int nx, int ny, int nz, int zc, 
             cur[r] = nxt[r];
         }
     }
-    double m = wave_max((double)md);
+    return md;
+}
+
+// workgroup max of a per-thread value into *slot (non-negative doubles)
+__device__ inline void block_max_to(double m, double* red, unsigned long long* slot) {
+    m = wave_max(m);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) red[w] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = red[0];
         for (int i = 1; i < 4; ++i) b = b < red[i] ? red[i] : b;
         if (b > 0.0) atomicMax(slot, dbits(b));
+    }
+    __syncthreads();
+}
+
+// one sweep per launch: one workgroup per tile
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k_jacobi(int nx, int ny, int nz, int zc, JCoef<T> co, const T* __restrict__ rho,
+                                                const T* __restrict__ cur_in, T* __restrict__ out,
+                                                unsigned long long* __restrict__ slot) {
+    __shared__ double red[4];
+    const T md = jacobi_tile<T, TY>(nx, ny, nz, zc, co, rho, cur_in, out, blockIdx.x, blockIdx.y, blockIdx.z);
+    block_max_to((double)md, red, slot);
+}
+
+// Several sweeps per launch: a cooperative grid of resident workgroups walks
+// the tiles of each sweep, then meets at a grid-wide barrier before the next
+// sweep reads what this one wrote.  Sweep s writes its max|dV| to slot[s].
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void k_jacobi_multi(int nx, int ny, int nz, int zc, int gx, int gy, int gz, JCoef<T> co,
+                                                      const T* __restrict__ rho, T* a, T* b, int nsweeps,
+                                                      unsigned long long* __restrict__ slot) {
+    __shared__ double red[4];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int ntiles = gx * gy * gz;
+    for (int s = 0; s < nsweeps; ++s) {
+        const T* src = (s & 1) ? b : a;
+        T* dst = (s & 1) ? a : b;
+        T md = T(0);
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+            const T m = jacobi_tile<T, TY>(nx, ny, nz, zc, co, rho, src, dst, t % gx, (t / gx) % gy, t / (gx * gy));
+            md = md < m ? m : md;
+        }
+        block_max_to((double)md, red, slot + s);
+        grid.sync();
     }
 }
 
@@ -164,9 +208,26 @@ struct Jacobi {
     int zc;
     const T* rho;
 
+    int coop_blocks = 0;  // resident workgroups for the cooperative kernel (0: unavailable)
+
     int sweep(const T* in, T* out, unsigned long long* slot) {
         k_jacobi<T, 4><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, zc, co, rho, in, out, slot);
         DFT_CHECK_LAUNCH(c);
+        return DFT_OK;
+    }
+
+    // nsweeps sweeps starting from buffer a (even sweeps a -> b, odd b -> a)
+    int sweeps(T* a, T* b, int nsweeps, unsigned long long* slot) {
+        if (coop_blocks <= 0 || nsweeps == 1) {
+            for (int s = 0; s < nsweeps; ++s)
+                if (int rc = sweep((s & 1) ? b : a, (s & 1) ? a : b, slot + s)) return rc;
+            return DFT_OK;
+        }
+        int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz, gx = (int)grid.x, gy = (int)grid.y, gz = (int)grid.z;
+        int zcv = zc;
+        void* args[] = {&nx, &ny, &nz, &zcv, &gx, &gy, &gz, &co, &rho, &a, &b, &nsweeps, &slot};
+        DFT_HIP(c, hipLaunchCooperativeKernel((const void*)k_jacobi_multi<T, 4>, dim3(coop_blocks), dim3(256), args, 0,
+                                              c->stream));
         return DFT_OK;
     }
 };
@@ -209,13 +270,22 @@ int poisson_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t max_swe
     const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
     const int gx = (nx + 63) / 64, gy = (ny + 15) / 16;
     const int64_t base = (int64_t)gx * gy;
-    int nzc = (int)std::max<int64_t>(1, std::min<int64_t>((8LL * c->num_cus + base - 1) / base, std::max(1, nz / 8)));
+    // resident workgroups of the cooperative kernel; the z-chunking is chosen
+    // so one sweep's tiles fit them in a single round
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_multi<T, 4>, 256, 0) != hipSuccess)
+        per_cu = 0;
+    const int64_t resident = (int64_t)per_cu * c->num_cus;
+    int64_t want = 8LL * c->num_cus;
+    if (resident > 0) want = std::max<int64_t>(base, resident);
+    int nzc = (int)std::max<int64_t>(1, std::min<int64_t>(want / base, std::max(1, nz / 4)));
     J.zc = (nz + nzc - 1) / nzc;
     nzc = (nz + J.zc - 1) / J.zc;
     J.grid = dim3(gx, gy, nzc);
+    if (per_cu > 0) J.coop_blocks = (int)std::min<int64_t>(base * nzc, resident);
 
     if (batch < 1) batch = 1;
-    if (batch > (int)c->sweep_max_cap) batch = (int)c->sweep_max_cap;
+    if (batch > (int)c->sweep_max_cap / 2) batch = (int)c->sweep_max_cap / 2;
     T* bufs[2] = {v, tmp};
     int curi = 0;  // buffer holding the latest iterate
     int64_t done = 0;
@@ -226,10 +296,8 @@ int poisson_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t max_swe
         DFT_HIP(c, hipMemsetAsync(c->sweep_max, 0, sizeof(unsigned long long) * nb, c->stream));
         if (exact) DFT_HIP(c, hipMemcpyAsync(snap, bufs[curi], bytes, hipMemcpyDeviceToDevice, c->stream));
         const int start = curi;
-        for (int s = 0; s < nb; ++s) {
-            if (int rc = J.sweep(bufs[curi], bufs[curi ^ 1], c->sweep_max + s)) return rc;
-            curi ^= 1;
-        }
+        if (int rc = J.sweeps(bufs[curi], bufs[curi ^ 1], nb, c->sweep_max)) return rc;
+        if (nb & 1) curi ^= 1;
         if (!exact && done + nb < max_sweeps) { done += nb; continue; }
         DFT_HIP(c, hipMemcpyAsync(hm.data(), c->sweep_max, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, c->stream));
         DFT_HIP(c, hipStreamSynchronize(c->stream));
@@ -250,10 +318,9 @@ int poisson_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t max_swe
         if (stop < nb - 1) {
             curi = start;
             DFT_HIP(c, hipMemcpyAsync(bufs[curi], snap, bytes, hipMemcpyDeviceToDevice, c->stream));
-            for (int s = 0; s <= stop; ++s) {
-                if (int rc = J.sweep(bufs[curi], bufs[curi ^ 1], c->sweep_max)) return rc;
-                curi ^= 1;
-            }
+            // replay into the upper slots (the batch's maxima are already read)
+            if (int rc = J.sweeps(bufs[curi], bufs[curi ^ 1], stop + 1, c->sweep_max + c->sweep_max_cap / 2)) return rc;
+            if ((stop + 1) & 1) curi ^= 1;
         }
         done += stop + 1;
         break;
@@ -271,8 +338,9 @@ extern "C" {
 
 int dft_poisson(dft_ctx* c, const dft_grid* g, int prec, const void* rho, void* v, int64_t max_sweeps, double tol,
                 int zero_start, int sweeps_per_launch, int64_t* sweeps_done, double* last_maxdiff) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && rho && v && max_sweeps >= 0, "dft_poisson: bad arguments");
-    DFT_REQUIRE(c, (g->nz + 0) <= 65535LL * 8 && g->ny < (1 << 30), "dft_poisson: grid too large");
+    DFT_REQUIRE(c, g->nx * g->ny * g->nz < (1LL << 31), "dft_poisson: grid exceeds 2^31 points");
     if (prec == DFT_F64)
         return poisson_t<double>(c, g, (const double*)rho, (double*)v, max_sweeps, tol, zero_start, sweeps_per_launch,
                                  sweeps_done, last_maxdiff);
@@ -283,6 +351,7 @@ int dft_poisson(dft_ctx* c, const dft_grid* g, int prec, const void* rho, void* 
 }
 
 int dft_hartree_energy(dft_ctx* c, const dft_grid* g, int prec, const void* rho, const void* vh, double* e) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && rho && vh && e, "dft_hartree_energy: bad arguments");
     double s = 0.0;
     if (int rc = dft_reduce(c, DFT_RED_DOT, prec, rho, vh, grid_points(g), &s)) return rc;

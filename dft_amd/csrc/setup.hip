@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void k_transpose(int r, int c, const double* _
 extern "C" {
 
 int dft_seed_vectors(dft_ctx* c, const dft_grid* g, int prec, int first_index, int count, void* out, int64_t ld) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && out && count >= 0 && count <= 65535 && first_index >= 0, "dft_seed_vectors: bad arguments");
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, ld >= n, "dft_seed_vectors: ld < grid size");
@@ -90,6 +91,7 @@ int dft_seed_vectors(dft_ctx* c, const dft_grid* g, int prec, int first_index, i
 
 int dft_hydrogenic_density(dft_ctx* c, const dft_grid* g, int prec, int natoms, const double* nuclei_host, double electrons,
                            void* out) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && out && natoms > 0 && nuclei_host, "dft_hydrogenic_density: bad arguments");
     const int64_t n = grid_points(g);
     if (int rc = ensure_partials(c, (size_t)4 * natoms + 8)) return rc;
@@ -109,6 +111,7 @@ int dft_hydrogenic_density(dft_ctx* c, const dft_grid* g, int prec, int natoms, 
 }
 
 int dft_transpose(dft_ctx* c, int rows, int cols, const double* src, int64_t lds, double* dst, int64_t ldd) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, src && dst && rows >= 0 && cols >= 0 && lds >= rows && ldd >= cols, "dft_transpose: bad arguments");
     const int64_t tot = (int64_t)rows * cols;
     if (!tot) return DFT_OK;

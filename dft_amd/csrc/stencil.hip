@@ -6,10 +6,12 @@
 // a z-chunk and streams through it plane by plane.  Lane = x, so a wave is
 // one unit-stride 64-point row segment (coalesced); each lane keeps TY
 // consecutive y-rows and the z-1 / z / z+1 planes of NB orbitals in
-// registers.  x-neighbours come from the adjacent lane by a DPP wave shift
-// (no LDS, no extra memory traffic), y-neighbours from the lane's own rows
-// plus one halo row above and below, z-neighbours from the register queue.
-// V_eff is loaded once per plane and reused by the NB orbitals of the group.
+// registers.  x-neighbours come from the adjacent lane by a DPP wave shift,
+// the two values beyond the segment from one extra load per row; y-
+// neighbours from the lane's own rows plus one halo row above and below;
+// z-neighbours from the register queue.  V_eff is loaded once per plane and
+// reused by the NB orbitals of the group.  Offsets are 32-bit from uniform
+// column bases, which keeps the kernel small enough for several waves per SIMD.
 #include "common.hpp"
 
 #include <algorithm>
@@ -52,57 +54,62 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
     const int ybase = blockIdx.y * (4 * TY) + w * TY;
     const int chunk = blockIdx.z % nzc, grp = blockIdx.z / nzc;
     const int k0 = chunk * zc, k1 = min(nz, k0 + zc);
-    const int64_t plane = (int64_t)nx * ny;
+    const int plane = nx * ny;  // the launcher guarantees nx*ny*nz < 2^31: 32-bit offsets
     const bool xin = x < nx;
-    const bool has_xm = x0 > 0, has_xp = x0 + 64 < nx;
-    const T* P[NB];
-    T* O[NB];
-#pragma unroll
-    for (int o = 0; o < NB; ++o) {
-        P[o] = psi + (int64_t)(grp * NB + o) * ld;
-        O[o] = out + (int64_t)(grp * NB + o) * ldo;
-    }
+    // x-halo: lane 0 reads x0-1 and lane 63 reads x0+64; the DPP shifts take
+    // no fill value from the other lanes, which stay masked off
+    int hx = x;
+    bool hin = false;
+    if (lane == 0) { hx = x0 - 1; hin = x0 > 0; }
+    if (lane == 63) { hx = x0 + 64; hin = x0 + 64 < nx; }
     bool yin[TY];
-    int64_t rowoff[TY];
+    int roff[TY], hoff[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
-        yin[r] = (ybase + r) < ny;
-        rowoff[r] = (int64_t)(ybase + r) * nx;
+        const int y = ybase + r;
+        yin[r] = y < ny;
+        roff[r] = y * nx + x;
+        hoff[r] = y * nx + hx;
     }
     const bool ym_in = ybase > 0 && ybase - 1 < ny, yp_in = ybase + TY < ny;
-    const int64_t ymoff = (int64_t)(ybase - 1) * nx, ypoff = (int64_t)(ybase + TY) * nx;
+    const int ymo = (ybase - 1) * nx + x, ypo = (ybase + TY) * nx + x;
+    const T* P = psi + (int64_t)(grp * NB) * ld;
+    T* Out = out + (int64_t)(grp * NB) * ldo;
 
     T prev[NB][TY], cur[NB][TY], nxt[NB][TY];
 #pragma unroll
-    for (int o = 0; o < NB; ++o)
+    for (int o = 0; o < NB; ++o) {
+        const T* p = P + (int64_t)o * ld;
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
             const bool ok = xin && yin[r];
-            prev[o][r] = (ok && k0 > 0) ? P[o][(int64_t)(k0 - 1) * plane + rowoff[r] + x] : T(0);
-            cur[o][r] = ok ? P[o][(int64_t)k0 * plane + rowoff[r] + x] : T(0);
+            prev[o][r] = (ok && k0 > 0) ? p[(k0 - 1) * plane + roff[r]] : T(0);
+            cur[o][r] = ok ? p[k0 * plane + roff[r]] : T(0);
         }
-
+    }
     for (int k = k0; k < k1; ++k) {
-        const int64_t pk = (int64_t)k * plane;
+        const int kp = k * plane;
         const bool has_next = k + 1 < nz;
         T v[TY];
 #pragma unroll
-        for (int r = 0; r < TY; ++r) v[r] = (xin && yin[r]) ? V[pk + rowoff[r] + x] : T(0);
+        for (int r = 0; r < TY; ++r) v[r] = (xin && yin[r]) ? V[kp + roff[r]] : T(0);
 #pragma unroll
         for (int o = 0; o < NB; ++o) {
-            const T* p = P[o] + pk;
-#pragma unroll
-            for (int r = 0; r < TY; ++r) nxt[o][r] = (xin && yin[r] && has_next) ? p[plane + rowoff[r] + x] : T(0);
-            const T ym = (xin && ym_in) ? p[ymoff + x] : T(0);
-            const T yp = (xin && yp_in) ? p[ypoff + x] : T(0);
+            const T* p = P + (int64_t)o * ld;
+            T hv[TY];
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
-                T hm = T(0), hp = T(0);
-                if (lane == 0 && has_xm && yin[r]) hm = p[rowoff[r] + x0 - 1];
-                if (lane == 63 && has_xp && yin[r]) hp = p[rowoff[r] + x0 + 64];
+                nxt[o][r] = (xin && yin[r] && has_next) ? p[kp + plane + roff[r]] : T(0);
+                hv[r] = (hin && yin[r]) ? p[kp + hoff[r]] : T(0);
+            }
+            const T ym = (xin && ym_in) ? p[kp + ymo] : T(0);
+            const T yp = (xin && yp_in) ? p[kp + ypo] : T(0);
+            T* q = Out + (int64_t)o * ldo;
+#pragma unroll
+            for (int r = 0; r < TY; ++r) {
                 const T c = cur[o][r];
-                const T xm = shr1(c, hm);
-                const T xp = shl1(c, hp);
+                const T xm = shr1(c, hv[r]);
+                const T xp = shl1(c, hv[r]);
                 const T ymv = r > 0 ? cur[o][r > 0 ? r - 1 : 0] : ym;
                 const T ypv = r < TY - 1 ? cur[o][r < TY - 1 ? r + 1 : 0] : yp;
                 // KohnShamHamiltonianMolecule.cpp:88-187 order: centre, x-, x+, y-, y+, z-, z+
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
                 lap += nxt[o][r] * co.cz;
                 const T kin = T(-0.5) * lap;
                 const T pot = v[r] * c;
-                if (xin && yin[r]) O[o][pk + rowoff[r] + x] = kin + pot;
+                if (xin && yin[r]) q[kp + roff[r]] = kin + pot;
             }
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
@@ -149,13 +156,16 @@ int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
     return DFT_OK;
 }
 
+// Orbitals per group: V_eff is read once per group (FP64: 4, FP32: 8).
 template <typename T>
 int hpsi_t(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int nb, T* out, int64_t ldo) {
-    constexpr int TY = 4;
-    const int n4 = nb / 4;
-    if (n4 > 0)
-        if (int rc = launch_hpsi<T, TY, 4>(c, g, V, psi, ld, n4, out, ldo)) return rc;
-    for (int o = n4 * 4; o < nb; ++o)
+    constexpr int TY = 2;
+    constexpr int NB = sizeof(T) == 8 ? 4 : 8;
+    DFT_REQUIRE(c, g->nx * g->ny * g->nz < (1LL << 31), "dft_hpsi: grid exceeds 2^31 points");
+    const int ng = nb / NB;
+    if (ng > 0)
+        if (int rc = launch_hpsi<T, TY, NB>(c, g, V, psi, ld, ng, out, ldo)) return rc;
+    for (int o = ng * NB; o < nb; ++o)
         if (int rc = launch_hpsi<T, TY, 1>(c, g, V, psi + (int64_t)o * ld, ld, 1, out + (int64_t)o * ldo, ldo)) return rc;
     return DFT_OK;
 }
@@ -210,6 +220,7 @@ extern "C" {
 
 int dft_hpsi(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* psi, int64_t ld, int nb, void* out,
              int64_t ldo) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g), "dft_hpsi: invalid grid");
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, veff && psi && out && nb >= 0 && ld >= n && ldo >= n, "dft_hpsi: bad arguments");
@@ -222,6 +233,7 @@ int dft_hpsi(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const vo
 
 int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
                        const int* electrons_host, double* t_host) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && psi && t_host && nb >= 0 && (nb == 0 || electrons_host), "dft_kinetic_energy: bad arguments");
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, ld >= n, "dft_kinetic_energy: ld < grid size");
@@ -246,6 +258,7 @@ int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi,
 
 int dft_precond_diag(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr, int nb,
                      const double* eps, void* t, int64_t ldt) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && veff && r && eps && t && nb >= 0 && nb < 65536, "dft_precond_diag: bad arguments");
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, ldr >= n && ldt >= n, "dft_precond_diag: leading dimension < grid size");

@@ -159,6 +159,7 @@ extern "C" {
 
 int dft_xc(dft_ctx* c, const dft_grid* g, int prec, int functional, const void* na, const void* nb, void* va, void* vb,
            double* exc_host) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && na && nb && va, "dft_xc: bad arguments");
     DFT_REQUIRE(c, functional == DFT_XC_PZ81 || functional == DFT_XC_PBE96, "dft_xc: unknown functional");
     if (prec == DFT_F64)
@@ -169,6 +170,7 @@ int dft_xc(dft_ctx* c, const dft_grid* g, int prec, int functional, const void* 
 }
 
 int dft_xc_points(dft_ctx* c, int functional, const double* in, int64_t npts, double* out) {
+    if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, in && out && npts >= 0, "dft_xc_points: bad arguments");
     DFT_REQUIRE(c, functional == DFT_XC_PZ81 || functional == DFT_XC_PBE96, "dft_xc_points: unknown functional");
     if (!npts) return DFT_OK;

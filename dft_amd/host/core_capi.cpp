@@ -51,6 +51,25 @@ int dft_scf_run(const dft_grid* g, int natoms, const double* nuclei, const dft_s
     });
 }
 
+int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_scf_config* cfg,
+                 const double* ra, const double* rb, dft_scf_report* rep, double* eig, dft_scf_fields* fields) {
+    return guarded([&] {
+        if (!g || !cfg || !rep || !ra || !rb || (natoms > 0 && !nuclei)) throw std::invalid_argument("dft_scf_step: null argument");
+        dftc::Ctx ctx(0);
+        dft_scf_config c = *cfg;
+        c.max_iterations = 1;
+        c.refine_iterations = 0;
+        c.stop_rule = DFT_STOP_REFERENCE;  // fixed Davidson tolerance; one step never meets the stop rule
+        dftc::ScfDriver drv(ctx, *g, natoms, nuclei, c);
+        dftc::ScfOutputs out;
+        out.init_rho_a = ra;
+        out.init_rho_b = rb;
+        out.export_fields = fields;
+        drv.run(*rep, out);
+        if (eig) std::copy(out.eig.begin(), out.eig.end(), eig);
+    });
+}
+
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
                      int max_subspace, double tol, int max_iter, double* eig, double* psi, double* resid,
                      int* iterations) {
@@ -61,7 +80,7 @@ int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* v
         dftc::Buf v64(ctx, (size_t)n * 8), v(ctx, (size_t)n * dftc::esize(prec));
         v64.upload(veff_host, (size_t)n * 8);
         dftc::check(ctx, dft_convert(ctx, v64.get(), DFT_F64, v.get(), prec, n), "dft_convert");
-        dftc::BlockDavidson D(ctx, *g, prec, nstates, max_subspace > 0 ? max_subspace : 3 * nstates);
+        dftc::BlockDavidson D(ctx, *g, prec, nstates, max_subspace > 0 ? max_subspace : std::max(3 * nstates, nstates + 16));
         dftc::DiagonalPreconditioner P;
         dftc::DavidsonOptions o;
         o.nconv = nconv > 0 ? nconv : nstates;

@@ -20,12 +20,14 @@ BlockDavidson::BlockDavidson(dft_ctx* c, const dft_grid& g, int prec, int nstate
     HX_ = Buf(c, (size_t)n_ * k_ * es_);
     R_ = Buf(c, (size_t)n_ * k_ * es_);
     T_ = Buf(c, (size_t)n_ * k_ * es_);
+    PV_ = Buf(c, (size_t)n_ * k_ * es_);
     G_ = Buf(c, sizeof(double) * (size_t)mmax_ * mmax_);
     Q_ = Buf(c, sizeof(double) * (size_t)mmax_ * mmax_);
     P_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
     S_ = Buf(c, sizeof(double) * (size_t)k_ * k_);
     theta_ = Buf(c, sizeof(double) * (size_t)mmax_);
     rn_ = Buf(c, sizeof(double) * (size_t)k_);
+    Qprev_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
     G_.zero();
 }
 
@@ -41,6 +43,12 @@ void BlockDavidson::gram(int ma, int mb, const void* A, const void* B, double* o
 void BlockDavidson::load_X(const void* src, int src_prec, int ncols) {
     ncols = std::min(ncols, k_);
     check(c_, dft_convert(c_, src, src_prec, X_.get(), prec_, n_ * (int64_t)ncols), "dft_convert");
+}
+
+void BlockDavidson::seed_columns(int first) {
+    if (first >= k_) return;
+    check(c_, dft_seed_vectors(c_, &g_, prec_, first, k_ - first, static_cast<char*>(X_.get()) + (size_t)first * n_ * es_, n_),
+          "dft_seed_vectors");
 }
 
 int BlockDavidson::orthonormalise(void* src, int cnt, void* dst, void* tmp) {
@@ -82,6 +90,63 @@ int BlockDavidson::orthonormalise(void* src, int cnt, void* dst, void* tmp) {
     return cnt;
 }
 
+int BlockDavidson::restart(int m, int mprev) {
+    // Coefficients (in the current basis V, m columns) of the Ritz block X
+    // (= Q[:, :k]) and of the previous Ritz block (Qprev padded with zeros).
+    const int k = k_;
+    std::vector<double> Q((size_t)mmax_ * k), Qp((size_t)mmax_ * k, 0.0);
+    check(c_, dft_download(c_, Q.data(), Q_.get(), sizeof(double) * (size_t)mmax_ * k), "dft_download");
+    check(c_, dft_download(c_, Qp.data(), Qprev_.get(), sizeof(double) * (size_t)mmax_ * k), "dft_download");
+    auto q = [&](int i, int j) -> double& { return Q[(size_t)j * mmax_ + i]; };
+    auto p = [&](int i, int j) -> double& { return Qp[(size_t)j * mmax_ + i]; };
+    for (int j = 0; j < k; ++j)
+        for (int i = mprev; i < m; ++i) p(i, j) = 0.0;
+    // P = (I - Q Q^T) Qprev, twice, then modified Gram-Schmidt among its columns
+    int kept = 0;
+    std::vector<double> Pc((size_t)m * k);
+    for (int j = 0; j < k; ++j) {
+        std::vector<double> v(m);
+        for (int i = 0; i < m; ++i) v[i] = p(i, j);
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int a = 0; a < k; ++a) {
+                double d = 0;
+                for (int i = 0; i < m; ++i) d += q(i, a) * v[i];
+                for (int i = 0; i < m; ++i) v[i] -= d * q(i, a);
+            }
+            for (int a = 0; a < kept; ++a) {
+                double d = 0;
+                for (int i = 0; i < m; ++i) d += Pc[(size_t)a * m + i] * v[i];
+                for (int i = 0; i < m; ++i) v[i] -= d * Pc[(size_t)a * m + i];
+            }
+        }
+        double nn = 0;
+        for (double x : v) nn += x * x;
+        nn = std::sqrt(nn);
+        if (nn < 1e-8) continue;  // previous block already inside span(X)
+        for (int i = 0; i < m; ++i) Pc[(size_t)kept * m + i] = v[i] / nn;
+        ++kept;
+    }
+    const size_t colb = (size_t)n_ * es_;
+    char* V = static_cast<char*>(V_.get());
+    char* W = static_cast<char*>(W_.get());
+    double* S = P_.as<double>();  // mmax x k scratch for the coefficients
+    if (kept > 0) {
+        check(c_, dft_upload(c_, S, Pc.data(), sizeof(double) * (size_t)m * kept), "dft_upload");
+        // P in PV_, HP in R_ (both free at this point)
+        check(c_, dft_block_update(c_, prec_, n_, m, kept, 1.0, V, n_, S, m, 0.0, PV_.get(), n_), "dft_block_update");
+        check(c_, dft_block_update(c_, prec_, n_, m, kept, 1.0, W, n_, S, m, 0.0, R_.get(), n_), "dft_block_update");
+    }
+    check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
+    check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
+    if (kept > 0) {
+        check(c_, dft_copy(c_, V + (size_t)k * colb, PV_.get(), colb * kept), "dft_copy");
+        check(c_, dft_copy(c_, W + (size_t)k * colb, R_.get(), colb * kept), "dft_copy");
+    }
+    const int mn = k + kept;
+    gram(mn, mn, V, W, G_.as<double>(), mmax_);
+    return mn;
+}
+
 DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOptions& opt, Preconditioner& P) {
     DavidsonStats st;
     gram_mode_ = opt.gram_mode;
@@ -103,6 +168,8 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     theta_h_.assign(k, 0.0);
     rn_h_.assign(k, INFINITY);
     std::vector<double> prev_rn;
+    int mprev = 0;  // basis size when Qprev was taken (0: none yet)
+    const int nact = nconv;
     for (int it = 1; it <= opt.max_iter; ++it) {
         st.iterations = it;
         check(c_, dft_copy(c_, Q, G, sizeof(double) * (size_t)mmax_ * mmax_), "dft_copy");
@@ -127,12 +194,12 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             break;
         }
         if (it == opt.max_iter) break;
-        // corrections for every unconverged column, in contiguous runs
+        // corrections for the unconverged states that must converge, in contiguous runs
         int nt = 0;
-        for (int j = 0; j < k;) {
+        for (int j = 0; j < nact;) {
             if (rn_h_[j] < opt.tol) { ++j; continue; }
             int j1 = j;
-            while (j1 < k && rn_h_[j1] >= opt.tol) ++j1;
+            while (j1 < nact && rn_h_[j1] >= opt.tol) ++j1;
             P.apply(c_, g_, prec_, veff, static_cast<char*>(R_.get()) + j * colb, n_, j1 - j, theta + j,
                     static_cast<char*>(T_.get()) + (size_t)nt * colb, n_);
             nt += j1 - j;
@@ -143,12 +210,20 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             gram(m, nt, V, T_.get(), Pm, mmax_);
             check(c_, dft_block_update(c_, prec_, n_, m, nt, -1.0, V, n_, Pm, mmax_, 1.0, T_.get(), n_), "dft_block_update");
         }
-        if (m + nt > mmax_) {  // restart on the Ritz block
-            check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
-            check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
-            m = k;
-            gram(k, k, V, W, G, mmax_);
+        if (m + nt > mmax_) {  // thick restart on [X, P]
+            m = mprev > 0 ? restart(m, mprev) : 0;
+            if (m == 0) {
+                check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
+                check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
+                m = k;
+                gram(k, k, V, W, G, mmax_);
+            }
             ++st.restarts;
+            mprev = 0;
+        } else {
+            // remember this iteration's Ritz coefficients for the next restart
+            check(c_, dft_copy(c_, Qprev_.get(), Q, sizeof(double) * (size_t)mmax_ * k), "dft_copy");
+            mprev = m;
         }
         const int kept = orthonormalise(T_.get(), nt, V + (size_t)m * colb, R_.get());
         if (kept == 0) break;  // no new direction: stagnation

@@ -8,6 +8,10 @@
 //   T = P(R) for unconverged columns (P = diagonal or learned preconditioner)
 //   T <- T - V (V^T T) twice, then CholQR twice, appended to V;
 //   restart to V = X when m would exceed the subspace limit.
+// Restart keeps momentum (LOBPCG-style thick restart): the new basis is
+// [X, P] with P the previous Ritz block orthogonalised against X, formed in
+// the coefficient space of V (no extra H applications).  Corrections are
+// computed only for the unconverged states that must converge.
 // Vectors are kept at unit 2-norm; with psi = u / sqrt(dV) the reference's
 // residual ||H psi - eps psi|| sqrt(dV) equals ||H u - eps u||.
 #pragma once
@@ -68,6 +72,8 @@ class BlockDavidson {
     const std::vector<double>& residuals() const { return rn_h_; }
     // Replace X with a block given in another precision (refinement hand-over).
     void load_X(const void* src, int src_prec, int ncols);
+    // Overwrite columns first..k-1 of X with the seed vectors of those indices.
+    void seed_columns(int first);
 
    private:
     void hpsi(const void* veff, const void* in, void* out, int ncols);
@@ -75,14 +81,16 @@ class BlockDavidson {
     // Orthonormalise `cnt` columns at src into dst (CholQR twice, with an
     // eigenvalue filter if the block is rank deficient); returns kept count.
     int orthonormalise(void* src, int cnt, void* dst, void* tmp);
+    // Thick restart: V <- [X, P], W <- [HX, HP]; returns the new basis size.
+    int restart(int m, int mprev);
 
     dft_ctx* c_;
     dft_grid g_;
     int prec_, k_, mmax_;
     int64_t n_;
     size_t es_;
-    Buf V_, W_, X_, HX_, R_, T_;
-    Buf G_, Q_, P_, S_, theta_, rn_;
+    Buf V_, W_, X_, HX_, R_, T_, PV_;
+    Buf G_, Q_, P_, S_, theta_, rn_, Qprev_;
     std::vector<double> theta_h_, rn_h_;
     int gram_mode_ = DFT_GRAM_F64;
 };

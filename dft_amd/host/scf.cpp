@@ -61,7 +61,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     if (nbe <= 0) throw std::invalid_argument("El numero de orbitales debe ser mayor que cero.");
     const int nocc_max = std::max(nal, nbe);
     const int k = std::max(cfg_.nstates > 0 ? cfg_.nstates : nocc_max, nocc_max);
-    const int mmax = cfg_.max_subspace > 0 ? cfg_.max_subspace : 3 * k;
+    const int mmax = cfg_.max_subspace > 0 ? cfg_.max_subspace : std::max(3 * k, k + 16);
     const int maxit = cfg_.max_iterations > 0 ? cfg_.max_iterations : 250;
     const int batch = cfg_.poisson_batch > 0 ? cfg_.poisson_batch : 64;
     const double dv = g_.dx * g_.dy * g_.dz;
@@ -102,13 +102,21 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     dop.max_iter = cfg_.davidson_max_iter > 0 ? cfg_.davidson_max_iter : 200;
     bool have_prev = false;
 
+    // Under the |dE| stop rule the orbitals need only be as accurate as the
+    // density is settled: tol = clamp(0.1 dRho_prev, floor, 1e-3), the floor
+    // being the configured tolerance (FP64) or the split-BF16 Rayleigh-Ritz
+    // floor (FP32 storage, ~3e-4).  The reference stop rule keeps the fixed
+    // tolerance of EigenvalueSolverMolecule.cpp:19.
+    double dd_prev = std::numeric_limits<double>::infinity();
     auto davidson = [&](Fields& f, BlockDavidson& D, const void* veff, int nocc) {
         DavidsonOptions o = dop;
         o.nconv = nocc;
         if (f.prec == DFT_F32) {
             o.gram_mode = DFT_GRAM_SPLIT_BF16;
-            o.tol = std::max(o.tol, 5.0e-5);  // FP32 residual floor
+            o.tol = std::max(o.tol, 3.0e-4);
+            o.max_iter = std::min(o.max_iter, 60);
         }
+        if (cfg_.stop_rule == DFT_STOP_DE) o.tol = std::max(o.tol, std::min(1.0e-3, 0.1 * dd_prev));
         DavidsonStats st = D.solve(veff, have_prev, o, *precond);
         rep.happly_columns += st.happly_columns;
         rep.davidson_iterations += st.iterations;
@@ -144,7 +152,13 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     {
         Fields& f = *F;
         check(c_, dft_nuclear_potential(c_, &g_, f.prec, natoms, nuc_.data(), f.vnuc.get()), "dft_nuclear_potential");
-        if (cfg_.init_guess == DFT_INIT_HYDROGENIC) {
+        if (out.init_rho_a && out.init_rho_b) {
+            Buf tmp(c_, (size_t)n * 8);
+            tmp.upload(out.init_rho_a, (size_t)n * 8);
+            check(c_, dft_convert(c_, tmp.get(), DFT_F64, f.ra.get(), f.prec, n), "dft_convert");
+            tmp.upload(out.init_rho_b, (size_t)n * 8);
+            check(c_, dft_convert(c_, tmp.get(), DFT_F64, f.rb.get(), f.prec, n), "dft_convert");
+        } else if (cfg_.init_guess == DFT_INIT_HYDROGENIC) {
             check(c_, dft_hydrogenic_density(c_, &g_, f.prec, natoms, nuc_.data(), nal, f.ra.get()), "dft_hydrogenic_density");
             check(c_, dft_hydrogenic_density(c_, &g_, f.prec, natoms, nuc_.data(), nbe, f.rb.get()), "dft_hydrogenic_density");
         } else {
@@ -253,6 +267,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         check(c_, dft_axpby(c_, p, n, imix, f.ora.get(), 1.0 - imix, f.ra.get()), "dft_axpby");
         check(c_, dft_axpby(c_, p, n, imix, f.orb.get(), 1.0 - imix, f.rb.get()), "dft_axpby");
         prevDD = dd;
+        dd_prev = dd;
         prevE = E[5];
         if (hand_over) {
             // FP64 refinement: carry densities, V_H warm start and orbitals over
@@ -267,6 +282,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             refining = true;
             prevE = std::numeric_limits<double>::infinity();
             prevDD = prevE;
+            dd_prev = 1.0e-3;
         }
     }
     check(c_, dft_sync(c_), "dft_sync");
@@ -275,6 +291,38 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     rep.seconds_xc = t_xc;
     rep.seconds_davidson = t_dav;
     rep.refine_iterations = refine_steps;
+    if (out.export_fields) {
+        Fields& f = *F;
+        Buf tmp(c_, (size_t)n * 8);
+        auto put = [&](const void* src, double* dst) {
+            if (!dst) return;
+            check(c_, dft_convert(c_, src, f.prec, tmp.get(), DFT_F64, n), "dft_convert");
+            tmp.download(dst, (size_t)n * 8);
+        };
+        const dft_scf_fields* e = out.export_fields;
+        put(f.vh.get(), e->vh);
+        put(f.vxa.get(), e->vxc_alpha);
+        put(closed ? f.vxa.get() : f.vxb.get(), e->vxc_beta);
+        put(f.ora.get(), e->rho_alpha);
+        put(f.orb.get(), e->rho_beta);
+        put(f.ovh.get(), e->vh_out);
+    }
+    if (out.want_fields) {
+        Fields& f = *F;
+        Buf tmp(c_, (size_t)n * 8 * std::max(nal, nbe));
+        auto grab = [&](const void* src, int cols, std::vector<double>& dst, double scale) {
+            check(c_, dft_convert(c_, src, f.prec, tmp.get(), DFT_F64, n * (int64_t)cols), "dft_convert");
+            dst.assign((size_t)n * cols, 0.0);
+            tmp.download(dst.data(), (size_t)n * cols * 8);
+            if (scale != 1.0)
+                for (double& x : dst) x *= scale;
+        };
+        const double s = 1.0 / std::sqrt(dv);
+        grab(f.da->X(), nal, out.psi_a, s);
+        if (closed) out.psi_b = out.psi_a; else grab(f.db->X(), nbe, out.psi_b, s);
+        grab(f.va.get(), 1, out.veff_a, 1.0);
+        if (closed) out.veff_b = out.veff_a; else grab(f.vb.get(), 1, out.veff_b, 1.0);
+    }
     if (out.want_rho) {
         Fields& f = *F;
         out.rho.assign(2 * (size_t)n, 0.0);

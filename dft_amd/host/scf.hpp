@@ -23,6 +23,14 @@ struct ScfOutputs {
     std::vector<double> trace;  // 5 per step
     std::vector<double> rho;    // output alpha then beta density (FP64), optional
     bool want_rho = false;
+    // final-step fields for the reference API (FP64, host): occupied orbitals
+    // in the reference normalisation (sum psi^2 dV = 1), V_eff per spin, V_H
+    bool want_fields = false;
+    std::vector<double> psi_a, psi_b, veff_a, veff_b;
+    // given input densities (one-step mode) and step-field export
+    const double* init_rho_a = nullptr;
+    const double* init_rho_b = nullptr;
+    const dft_scf_fields* export_fields = nullptr;
 };
 
 // Factory for the learned preconditioner (cnn.cpp); null when unavailable.

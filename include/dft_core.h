@@ -34,7 +34,7 @@ typedef struct {
     int multiplicity;      /* 1: closed shell (beta copies alpha, MolecularSCFIteration.cpp:77-90,127-133) */
     int precision;         /* DFT_SCF_FP64 or DFT_SCF_MIXED */
     int nstates;           /* Davidson block per spin (>= occupied); 0: occupied count */
-    int max_subspace;      /* 0: 3 * block */
+    int max_subspace;      /* 0: max(3 * block, block + 16) */
     int max_iterations;    /* SCF steps; 0: 250 (DFTConstants.h:15) */
     int poisson_fixed;     /* 0: reference rule (tol 1e-8, cap 10000, zero start); k > 0: k warm sweeps */
     int poisson_batch;     /* sweeps between host checks; 0: 64 */
@@ -75,6 +75,25 @@ const char* dft_core_last_error(void);
  * (2*N: output alpha then beta density). */
 int dft_scf_run(const dft_grid* g, int natoms, const double* nuclei, const dft_scf_config* cfg, dft_scf_report* rep,
                 double* eig, double* trace, int trace_cap, double* rho);
+
+/* Fields of one SCF step (all FP64 host arrays of N values; NULL to skip). */
+typedef struct {
+    double* vh;         /* V_H of the input density */
+    double* vxc_alpha;  /* V_xc of the input densities */
+    double* vxc_beta;
+    double* rho_alpha;  /* output spin densities */
+    double* rho_beta;
+    double* vh_out;     /* V_H of the output density */
+} dft_scf_fields;
+
+/* One SCF iteration (MolecularSCFIteration.cpp:12-252) on given input spin
+ * densities (host, N each), orbitals started from seed vectors; cfg selects
+ * functional, multiplicity, precision, Poisson mode and Davidson block.
+ * rep receives energies, dRho and the KS residual of the step; eig the
+ * occupied eigenvalues (alpha then beta). */
+int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_scf_config* cfg,
+                 const double* rho_alpha_in, const double* rho_beta_in, dft_scf_report* rep, double* eig,
+                 dft_scf_fields* fields);
 
 /* Block Davidson on a given potential (tests and benchmarks).  veff_host: N
  * FP64 values.  prec/gram_mode select the arithmetic.  eig: nstates values;

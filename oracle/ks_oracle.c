@@ -1073,3 +1073,71 @@ done:
     free(prev_a); free(prev_b); free(ea); free(eb); free(occa); free(occb);
     return rc;
 }
+
+int og_scf_step(const og_grid* g, int natoms, const double* nuc, int functional, int multiplicity,
+                const double* ra, const double* rb, double* vh, double* vxa, double* vxb, double* ora,
+                double* orb, double* ovh, double* eig, double* E, double* dd_out, double* ks_out) {
+    /* MolecularSCFIteration.cpp:12-252 on the given densities */
+    int zsum = 0;
+    for (int a = 0; a < natoms; ++a) zsum += (int)nuc[4 * a];
+    const int sd = multiplicity - 1, ne = zsum;
+    if (ne <= 0 || (ne - sd) % 2 || sd > ne) return fail(-1, "bad electron count / multiplicity");
+    const int na = (ne + sd) / 2, nb = (ne - sd) / 2, closed = multiplicity == 1;
+    if (nb <= 0) return fail(-1, "El numero de orbitales debe ser mayor que cero.");
+    const long n = gsize(g);
+    const size_t vb = sizeof(double) * (size_t)n;
+    const double dv = volume_element(g);
+    int rc = 0;
+    double *vnuc = malloc(vb), *rho = malloc(vb), *va = malloc(vb), *vbp = malloc(vb), *orho = malloc(vb), *hp = malloc(vb);
+    double *pa = malloc(vb * (size_t)na), *pb = malloc(vb * (size_t)nb);
+    int* occ = malloc(sizeof(int) * (size_t)(na > nb ? na : nb));
+    for (int i = 0; i < (na > nb ? na : nb); ++i) occ[i] = 1;
+    long sw;
+    double md, enn;
+    if ((rc = og_nuclear_potential(g, natoms, nuc, vnuc))) goto done;
+    if ((rc = og_nuclear_repulsion(natoms, nuc, &enn))) goto done;
+    for (long i = 0; i < n; ++i) rho[i] = ra[i] + rb[i];
+    og_poisson(g, rho, vh, 10000, 1.0e-8, 1, &sw, &md);
+    if ((rc = og_xc_potential(g, functional, ra, rb, 0, vxa))) goto done;
+    if (closed) memcpy(vxb, vxa, vb);
+    else if ((rc = og_xc_potential(g, functional, ra, rb, 1, vxb))) goto done;
+    for (long i = 0; i < n; ++i) {
+        va[i] = vnuc[i] + vh[i] + vxa[i];
+        vbp[i] = vnuc[i] + vh[i] + vxb[i];
+    }
+    if ((rc = og_davidson(g, va, na, occ, NULL, 5000, pa, eig, NULL))) goto done;
+    if (closed) { memcpy(pb, pa, vb * (size_t)nb); memcpy(eig + na, eig, sizeof(double) * (size_t)nb); }
+    else if ((rc = og_davidson(g, vbp, nb, occ, NULL, 5000, pb, eig + na, NULL))) goto done;
+    og_spin_density(n, na, pa, occ, ora);
+    og_spin_density(n, nb, pb, occ, orb);
+    for (long i = 0; i < n; ++i) orho[i] = ora[i] + orb[i];
+    og_poisson(g, orho, ovh, 10000, 1.0e-8, 1, &sw, &md);
+    E[0] = og_kinetic_energy(g, na, pa, occ) + og_kinetic_energy(g, nb, pb, occ);
+    E[1] = og_external_energy(g, orho, vnuc);
+    E[2] = og_hartree_energy(g, orho, ovh);
+    E[3] = og_xc_energy(g, functional, ora, orb);
+    E[4] = enn;
+    E[5] = E[0] + E[1] + E[2] + E[3] + E[4];
+    {
+        const double ddt = density_difference(g, rho, orho);
+        const double dds = dmax(density_difference(g, ra, ora), density_difference(g, rb, orb));
+        *dd_out = dmax(ddt, dds);
+    }
+    *ks_out = 0.0;
+    for (int s = 0; s < 2; ++s) {
+        const int no = s ? nb : na;
+        for (int o = 0; o < no; ++o) {
+            const double* p = (s ? pb : pa) + (size_t)o * (size_t)n;
+            og_hpsi(g, s ? vbp : va, p, hp);
+            double r2 = 0.0;
+            for (long i = 0; i < n; ++i) {
+                const double r = hp[i] - eig[(s ? na : 0) + o] * p[i];
+                r2 += r * r;
+            }
+            *ks_out = dmax(*ks_out, sqrt(r2 * dv));
+        }
+    }
+done:
+    free(vnuc); free(rho); free(va); free(vbp); free(orho); free(hp); free(pa); free(pb); free(occ);
+    return rc;
+}

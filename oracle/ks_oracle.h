@@ -113,6 +113,14 @@ typedef struct {
 int og_scf(const og_grid* g, int natoms, const double* nuclei, const og_scf_config* cfg,
            og_scf_result* res, double* trace, double* eig_out, double* rho_out);
 
+/* One SCF iteration on given input spin densities (MolecularSCFIteration.cpp:12-252)
+ * with the reference Poisson rule and seed-vector Davidson starts.  Outputs
+ * (all N unless noted): vh, vxa, vxb, ora, orb, ovh; eig (n_alpha + n_beta);
+ * E[6]; dd (effective density difference); ks (max KS residual). */
+int og_scf_step(const og_grid* g, int natoms, const double* nuclei, int functional, int multiplicity,
+                const double* ra, const double* rb, double* vh, double* vxa, double* vxb, double* ora,
+                double* orb, double* ovh, double* eig, double* E, double* dd, double* ks);
+
 #ifdef __cplusplus
 }
 #endif

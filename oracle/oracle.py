@@ -207,6 +207,25 @@ class Oracle:
         return out
 
 
+def _scf_step(self, g, nuclei, functional, multiplicity, ra, rb):
+    nuc = _f64(nuclei).reshape(-1, 4)
+    n = g.size
+    out = {k: np.empty(n) for k in ("vh", "vxa", "vxb", "ora", "orb", "ovh")}
+    ne = int(round(nuc[:, 0].sum()))
+    eig = np.empty(ne + 2)
+    E = np.empty(6)
+    dd, ks = C.c_double(), C.c_double()
+    self._chk(self.L.og_scf_step(C.byref(self._g(g)), C.c_int(nuc.shape[0]), _d(nuc), C.c_int(functional),
+                                 C.c_int(multiplicity), _d(_f64(ra, n)), _d(_f64(rb, n)), _d(out["vh"]), _d(out["vxa"]),
+                                 _d(out["vxb"]), _d(out["ora"]), _d(out["orb"]), _d(out["ovh"]), _d(eig), _d(E),
+                                 C.byref(dd), C.byref(ks)))
+    out.update(eigenvalues=eig[:ne], energy=E, dRho=dd.value, ks=ks.value)
+    return out
+
+
+Oracle.scf_step = _scf_step
+
+
 class Reference:
     """The reference's compiled 3D path (only where oracle/_ref was built)."""
 
