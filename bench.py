@@ -124,8 +124,10 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
     dev = dft_amd.Device(0)
     n = grid.size
     rows = []
+    rows_tag = [""]
 
     def timed(fn, reps=reps):
+        print(f"[kernels] timing {rows_tag[0]}", file=sys.stderr, flush=True)
         fn()
         dev.sync()
         e0, e1 = dev.event(), dev.event()
@@ -142,21 +144,25 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
         psi = dev.array(shape=(nb, n), dtype=dt)
         dev.seed_vectors(grid, psi, 0, nb)
         out = dev.array(shape=(nb, n), dtype=dt)
+        rows_tag[0] = f"hpsi_{tag}"
         ms = timed(lambda: dev.hpsi(grid, v, psi, out))
         rows.append(dict(kernel=f"hpsi_{tag}", ms=ms, bytes=n * nb * (2 * es + es / nb), flops=13.0 * n * nb))
         rho = dev.array(shape=(n,), dtype=dt)
         dev.seed_vectors(grid, rho, 0, 1)
         vh = dev.zeros((n,), dtype=dt)
         sweeps = 200
+        rows_tag[0] = f"jacobi_{tag}"
         ms = timed(lambda: dev.poisson(grid, rho, vh, max_sweeps=sweeps, tol=0.0, zero_start=False, batch=sweeps), reps=3)
         rows.append(dict(kernel=f"jacobi_{tag}", ms=ms / sweeps, bytes=3 * es * n, flops=10.0 * n, note=f"per sweep, {sweeps} per call"))
         na = dev.array(shape=(n,), dtype=dt)
         dev.seed_vectors(grid, na, 0, 1)
         va, vb = dev.array(shape=(n,), dtype=dt), dev.array(shape=(n,), dtype=dt)
+        rows_tag[0] = f"xc_{tag}"
         ms = timed(lambda: dev.xc(grid, w["functional"], na, na, va, vb, energy=False))
         rows.append(dict(kernel=f"xc_{tag}", ms=ms, bytes=4 * es * n, flops=None))
         dens = dev.array(shape=(n,), dtype=dt)
         occ = np.ones(nb)
+        rows_tag[0] = f"density_{tag}"
         ms = timed(lambda: dev.density(psi, occ, dens, nb=nb, n=n))
         rows.append(dict(kernel=f"density_{tag}", ms=ms, bytes=n * (nb + 1) * es, flops=2.0 * n * nb))
         m = 3 * nb
@@ -164,10 +170,12 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
         dev.seed_vectors(grid, V, 0, m)
         G = dev.zeros((nb, m))
         mode = dft_amd.GRAM_F64 if es == 8 else dft_amd.GRAM_SPLIT_BF16
+        rows_tag[0] = f"gram_{tag}"
         ms = timed(lambda: dev.gram(mode, V, psi, G, n, m, nb))
         rows.append(dict(kernel=f"gram_{'f64' if es == 8 else 'split_bf16'}", ms=ms, bytes=n * (m + nb) * es,
                          flops=2.0 * n * m * nb, note=f"{m}x{nb}"))
         C = dev.array(np.eye(m, nb).T.copy())
+        rows_tag[0] = f"update_{tag}"
         ms = timed(lambda: dev.block_update(V, C, out, n, m, nb))
         rows.append(dict(kernel=f"update_{tag}", ms=ms, bytes=n * (m + nb) * es, flops=2.0 * n * m * nb, note=f"{m}x{nb}"))
         del V, G, C

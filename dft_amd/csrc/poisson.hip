@@ -270,19 +270,23 @@ int poisson_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t max_swe
     const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
     const int gx = (nx + 63) / 64, gy = (ny + 15) / 16;
     const int64_t base = (int64_t)gx * gy;
-    // resident workgroups of the cooperative kernel; the z-chunking is chosen
-    // so one sweep's tiles fit them in a single round
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_multi<T, 4>, 256, 0) != hipSuccess)
-        per_cu = 0;
-    const int64_t resident = (int64_t)per_cu * c->num_cus;
-    int64_t want = 8LL * c->num_cus;
-    if (resident > 0) want = std::max<int64_t>(base, resident);
-    int nzc = (int)std::max<int64_t>(1, std::min<int64_t>(want / base, std::max(1, nz / 4)));
+    int nzc = (int)std::max<int64_t>(1, std::min<int64_t>((8LL * c->num_cus + base - 1) / base, std::max(1, nz / 8)));
     J.zc = (nz + nzc - 1) / nzc;
     nzc = (nz + J.zc - 1) / J.zc;
     J.grid = dim3(gx, gy, nzc);
-    if (per_cu > 0) J.coop_blocks = (int)std::min<int64_t>(base * nzc, resident);
+    // The cooperative kernel needs every workgroup resident at once.  It is
+    // used only when the grid has at most one workgroup per CU, where that
+    // holds by construction (one 256-thread workgroup always fits an idle
+    // CU); larger grids run one launch per sweep, whose launch overhead is
+    // small next to a sweep of that size.
+    {
+        int per_cu = 0;
+        const int64_t tiles = base * nzc;
+        if (tiles <= c->num_cus &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_multi<T, 4>, 256, 0) == hipSuccess &&
+            per_cu > 0)
+            J.coop_blocks = (int)tiles;
+    }
 
     if (batch < 1) batch = 1;
     if (batch > (int)c->sweep_max_cap / 2) batch = (int)c->sweep_max_cap / 2;
