@@ -235,6 +235,10 @@ def main():
     ap.add_argument("--hpsi-only", type=int, default=0, help="only run N H psi launches (profiling)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="only time the hot kernels at the workload size")
+    ap.add_argument("--profile-scf", type=int, default=0,
+                    help="only run this many SCF steps of each precision (profiling)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="with --hpsi-only: also run 5 device copies of this many MiB (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding is a design note, "
@@ -247,10 +251,32 @@ def main():
     grid = dft_amd.Grid.centered(w["n"], w["h"])
     nb = w["nstates"] if w["nstates"] % 4 == 0 else 4 * ((w["nstates"] + 3) // 4)
     if a.hpsi_only:
+        if a.copy_calib:
+            dev = dft_amd.Device(0)
+            nbytes = a.copy_calib << 20
+            src = dev.zeros((nbytes // 8,))
+            dst = dev.zeros((nbytes // 8,))
+            for _ in range(5):
+                dev.copy(dst, src)
+            dev.sync()
+            print(json.dumps({"copy_bytes": nbytes, "copies": 5}), flush=True)
+            dev.close()
         ms, by = time_hpsi(dft_amd, grid, nb, dft_amd.F32, reps=a.hpsi_only)
         ms64, by64 = time_hpsi(dft_amd, grid, nb, dft_amd.F64, reps=a.hpsi_only)
         print(json.dumps({"hpsi_f32_ms": ms, "hpsi_f32_GBps": by / ms / 1e6, "hpsi_f64_ms": ms64,
                           "hpsi_f64_GBps": by64 / ms64 / 1e6}))
+        return
+    if a.profile_scf:
+        for prec in (dft_amd.SCF_FP64, dft_amd.SCF_MIXED):
+            t = time.perf_counter()
+            r = dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=prec, nstates=w["nstates"],
+                                max_iterations=a.profile_scf, poisson_fixed=w["poisson_fixed"], poisson_batch=64,
+                                stop_rule=dft_amd.STOP_DE, energy_tol=1e-5, init_guess=w["init"], davidson_tol=1e-7,
+                                refine_iterations=0)
+            print(json.dumps({"precision": prec, "steps": r["iterations"], "seconds": time.perf_counter() - t,
+                              "davidson_iterations": r["davidson_iterations"], "happly": r["happly_columns"],
+                              "phase": {k: r[f"seconds_{k}"] for k in ("poisson", "xc", "davidson", "other")}}),
+                  flush=True)
         return
     if a.kernels:
         for r in kernel_table(dft_amd, grid, w, nb):

@@ -194,6 +194,11 @@ class Device:
     def sync(self):
         self._chk(self.L.dft_sync(self.ctx))
 
+    def copy(self, dst, src, nbytes=None):
+        """Device-to-device copy on the context stream."""
+        nb = src.nbytes if nbytes is None else int(nbytes)
+        self._chk(self.L.dft_copy(self.ctx, _vp(dst.ptr), _vp(src.ptr), C.c_size_t(nb)))
+
     # ---- kernels -------------------------------------------------------
     def hpsi(self, g, veff, psi, out, nb=None, ld=None, ldo=None):
         n = g.size
@@ -281,6 +286,72 @@ class Device:
         self._chk(self.L.dft_precond_diag(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
                                           C.c_int64(g.size), C.c_int(nb), _vp(eps.ptr), _vp(t.ptr), C.c_int64(g.size)))
 
+    # ---- learned preconditioner kernels (cnn.hip, harvest.hip) ----------
+    def cnn_act_count(self, dims, nbatch, training, channels=16):
+        n = C.c_int64()
+        self._chk(self.L.dft_cnn_act_count(C.c_int(channels), C.c_int64(dims[0]), C.c_int64(dims[1]),
+                                           C.c_int64(dims[2]), C.c_int(nbatch), C.c_int(training), C.byref(n)))
+        return n.value
+
+    def cnn_scratch_count(self, dims, nbatch, channels=16):
+        n = C.c_int64()
+        self._chk(self.L.dft_cnn_scratch_count(C.c_int(channels), C.c_int64(dims[0]), C.c_int64(dims[1]),
+                                               C.c_int64(dims[2]), C.c_int(nbatch), C.byref(n)))
+        return n.value
+
+    def cnn_to_bf16(self, w32, wbf):
+        self._chk(self.L.dft_cnn_to_bf16(self.ctx, C.c_int64(int(np.prod(w32.shape))), _vp(w32.ptr), _vp(wbf.ptr)))
+
+    def cnn_forward(self, dims, nbatch, wbf, xin, out, act, training, channels=16):
+        """dims = (nx, ny, nz); xin [nbatch][2][N], out [nbatch][N] (FP32 device arrays)."""
+        self._chk(self.L.dft_cnn_forward(self.ctx, C.c_int(channels), C.c_int64(dims[0]), C.c_int64(dims[1]),
+                                         C.c_int64(dims[2]), C.c_int(nbatch), _vp(wbf.ptr), _vp(xin.ptr), _vp(out.ptr),
+                                         _vp(act.ptr), C.c_int(training)))
+
+    def cnn_backward(self, dims, nbatch, wbf, xin, act, out, target, grad, scratch, channels=16):
+        loss = C.c_double()
+        self._chk(self.L.dft_cnn_backward(self.ctx, C.c_int(channels), C.c_int64(dims[0]), C.c_int64(dims[1]),
+                                          C.c_int64(dims[2]), C.c_int(nbatch), _vp(wbf.ptr), _vp(xin.ptr), _vp(act.ptr),
+                                          _vp(out.ptr), _vp(target.ptr), _vp(grad.ptr), _vp(scratch.ptr), C.byref(loss)))
+        return loss.value
+
+    def cnn_adam(self, w32, wbf, grad, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._chk(self.L.dft_cnn_adam(self.ctx, C.c_int64(int(np.prod(w32.shape))), _vp(w32.ptr), _vp(wbf.ptr),
+                                      _vp(grad.ptr), _vp(m.ptr), _vp(v.ptr), C.c_int(step), C.c_float(lr),
+                                      C.c_float(beta1), C.c_float(beta2), C.c_float(eps)))
+
+    def column_maxabs(self, x, n, ncols, out):
+        self._chk(self.L.dft_column_maxabs(self.ctx, C.c_int(x.prec), C.c_int64(n), C.c_int(ncols), _vp(x.ptr),
+                                           C.c_int64(n), _vp(out.ptr)))
+
+    def cnn_prepare(self, g, veff, r, ncols, theta, scale, xin):
+        self._chk(self.L.dft_cnn_prepare(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
+                                         C.c_int64(g.size), C.c_int(ncols), _vp(theta.ptr), _vp(scale.ptr), _vp(xin.ptr)))
+
+    def cnn_finish(self, g, ncols, out, scale, t):
+        self._chk(self.L.dft_cnn_finish(self.ctx, C.byref(g), C.c_int(t.prec), C.c_int(ncols), _vp(out.ptr),
+                                        _vp(scale.ptr), _vp(t.ptr), C.c_int64(g.size)))
+
+    def column_dots(self, x, y, n, ncols, out):
+        self._chk(self.L.dft_column_dots(self.ctx, C.c_int(x.prec), C.c_int64(n), C.c_int(ncols), _vp(x.ptr),
+                                         C.c_int64(n), _vp(y.ptr), C.c_int64(n), _vp(out.ptr)))
+
+    def cols_axpby(self, a, x, b, y, n, ncols):
+        self._chk(self.L.dft_cols_axpby(self.ctx, C.c_int(x.prec), C.c_int64(n), C.c_int(ncols), _vp(a.ptr),
+                                        _vp(x.ptr), C.c_int64(n), _vp(b.ptr), _vp(y.ptr), C.c_int64(n)))
+
+    def patch_energy(self, g, patch, r, ncols, out):
+        self._chk(self.L.dft_patch_energy(self.ctx, C.byref(g), C.c_int(r.prec), C.c_int(patch), _vp(r.ptr),
+                                          C.c_int64(g.size), C.c_int(ncols), _vp(out.ptr)))
+
+    def extract_patches(self, g, patch, desc, ncols, veff, r, cmat, theta, scale, cscale, xin, tgt):
+        d = np.ascontiguousarray(desc, dtype=np.int32).reshape(-1, 4)
+        self._chk(self.L.dft_extract_patches(self.ctx, C.byref(g), C.c_int(r.prec), C.c_int(patch), C.c_int(d.shape[0]),
+                                             d.ctypes.data_as(C.POINTER(C.c_int)), C.c_int(ncols), _vp(veff.ptr),
+                                             _vp(r.ptr), C.c_int64(g.size), _vp(cmat.ptr), C.c_int64(g.size),
+                                             _vp(theta.ptr), _vp(scale.ptr), _vp(cscale.ptr), _vp(xin.ptr),
+                                             _vp(tgt.ptr)))
+
     # ---- timing on the context stream ---------------------------------
     def event(self):
         e = C.c_void_p()
@@ -339,7 +410,7 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
     nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
                     poisson_fixed, poisson_batch, stop_rule, energy_tol, init_guess, davidson_tol,
-                    davidson_max_iter, refine_iterations, precond, cnn, 1 if verbose else 0)
+                    davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
     eig = np.zeros(ne + 2)
@@ -366,21 +437,162 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
 
 
 def davidson_run(grid, veff, nstates, nconv=None, prec=F64, gram_mode=GRAM_F64, max_subspace=0, tol=1e-7,
-                 max_iter=0, want_psi=False):
+                 max_iter=0, want_psi=False, cnn=None):
+    """Block Davidson on a fixed potential; cnn: a CnnModel for the learned correction."""
     L = core_lib()
     v = np.ascontiguousarray(veff, dtype=np.float64)
     nconv = nconv or nstates
     eig = np.zeros(nstates)
     res = np.zeros(nstates)
     its = C.c_int()
+    stats = np.zeros(4, dtype=np.int64)
     psi = np.zeros((nconv, grid.size)) if want_psi else None
     rc = L.dft_davidson_run(C.byref(grid), C.c_int(prec), C.c_int(gram_mode), v.ctypes.data_as(_dp), C.c_int(nstates),
                             C.c_int(nconv), C.c_int(max_subspace), C.c_double(tol), C.c_int(max_iter),
-                            eig.ctypes.data_as(_dp), None if psi is None else psi.ctypes.data_as(_dp),
-                            res.ctypes.data_as(_dp), C.byref(its))
+                            _model_ptr(cnn), eig.ctypes.data_as(_dp), None if psi is None else psi.ctypes.data_as(_dp),
+                            res.ctypes.data_as(_dp), C.byref(its), stats.ctypes.data_as(C.POINTER(C.c_int64)))
     if rc != 0:
         _core_err(rc)
-    return dict(eigenvalues=eig, residuals=res, iterations=its.value, psi=psi)
+    return dict(eigenvalues=eig, residuals=res, iterations=its.value, psi=psi, happly_columns=int(stats[0]),
+                restarts=int(stats[1]), cnn_applied=int(stats[2]), cnn_fallbacks=int(stats[3]))
+
+
+# ---------------------------------------------------------------- learned preconditioner
+CNN_CHANNELS = 16
+
+
+def cnn_param_count(channels=CNN_CHANNELS):
+    n = C.c_int64()
+    if hip_lib().dft_cnn_param_count(C.c_int(channels), C.byref(n)) != 0:
+        raise ValueError("bad channel count")
+    return n.value
+
+
+class _CnnModelC(C.Structure):
+    _fields_ = [("channels", C.c_int), ("count", C.c_int64), ("params", C.POINTER(C.c_float))]
+
+
+class CnnModel:
+    """Trained network parameters (FP32 master copy, kernel layout); save/load as .npz."""
+
+    def __init__(self, params, channels=CNN_CHANNELS):
+        self.params = np.ascontiguousarray(params, dtype=np.float32)
+        if self.params.size != cnn_param_count(channels):
+            raise ValueError("parameter count does not match the network")
+        self.channels = channels
+        self._c = _CnnModelC(channels, self.params.size, self.params.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def save(self, path):
+        np.savez(path, params=self.params, channels=np.int64(self.channels))
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            return cls(z["params"], int(z["channels"]))
+
+
+def _model_ptr(m):
+    if m is None:
+        return None
+    if not isinstance(m, CnnModel):
+        raise TypeError("cnn must be a CnnModel")
+    return C.cast(C.pointer(m._c), C.c_void_p)
+
+
+class HarvestConfig(C.Structure):
+    """dft_cnn_harvest_config (include/dft_core.h)."""
+
+    _fields_ = [("patch", C.c_int), ("max_columns", C.c_int), ("patches_per_column", C.c_int), ("every", C.c_int),
+                ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("seed", C.c_uint64), ("target_scale", C.c_int)]
+
+
+class CnnTrainer:
+    """Device patch set + network + Adam state (dft_cnn_trainer)."""
+
+    def __init__(self, patch=32, capacity=4096, seed=1, channels=CNN_CHANNELS):
+        self.L = core_lib()
+        self.L.dft_cnn_trainer_create.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_void_p)]
+        self.L.dft_cnn_trainer_destroy.argtypes = [C.c_void_p]
+        h = C.c_void_p()
+        rc = self.L.dft_cnn_trainer_create(channels, patch, capacity, seed, C.byref(h))
+        if rc != 0:
+            _core_err(rc)
+        self.h = h
+        self.patch, self.channels = patch, channels
+        self.count_params = cnn_param_count(channels)
+
+    def close(self):
+        if self.h:
+            self.L.dft_cnn_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def harvest(self, grid, nuclei, functional=PZ81, nstates=0, max_iterations=30, poisson_fixed=0, energy_tol=1e-5,
+                init_guess=INIT_HYDROGENIC, davidson_tol=1e-6, max_columns=8, patches_per_column=4, every=3,
+                cg_tol=1e-6, cg_max_iter=400, seed=7, multiplicity=1, target_scale=0):
+        nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
+        cfg = ScfConfig(functional, 0, multiplicity, SCF_FP64, nstates, 0, max_iterations, poisson_fixed, 64, STOP_DE,
+                        energy_tol, init_guess, davidson_tol, 0, 0, PRECOND_DIAG, None, 0)
+        hc = HarvestConfig(self.patch, max_columns, patches_per_column, every, cg_tol, cg_max_iter, seed, target_scale)
+        rep = ScfReport()
+        added = C.c_int64()
+        rc = self.L.dft_cnn_trainer_harvest(self.h, C.byref(grid), C.c_int(nuc.shape[0]), nuc.ctypes.data_as(_dp),
+                                            C.byref(cfg), C.byref(hc), C.byref(rep), C.byref(added))
+        if rc != 0:
+            _core_err(rc)
+        return added.value
+
+    def count(self):
+        n = C.c_int64()
+        self.L.dft_cnn_trainer_count(self.h, C.byref(n))
+        return n.value
+
+    def train(self, steps, batch=8, lr=1e-3, train_count=0):
+        loss = np.zeros(max(steps, 1))
+        rc = self.L.dft_cnn_trainer_train(self.h, C.c_int(steps), C.c_int(batch), C.c_float(lr), C.c_int64(train_count),
+                                          loss.ctypes.data_as(_dp))
+        if rc != 0:
+            _core_err(rc)
+        return loss[:steps]
+
+    def eval(self, first, count):
+        loss = C.c_double()
+        rc = self.L.dft_cnn_trainer_eval(self.h, C.c_int64(first), C.c_int64(count), C.byref(loss))
+        if rc != 0:
+            _core_err(rc)
+        return loss.value
+
+    def params(self):
+        p = np.zeros(self.count_params, dtype=np.float32)
+        rc = self.L.dft_cnn_trainer_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), C.c_int64(p.size))
+        if rc != 0:
+            _core_err(rc)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        rc = self.L.dft_cnn_trainer_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), C.c_int64(p.size))
+        if rc != 0:
+            _core_err(rc)
+
+    def model(self):
+        return CnnModel(self.params(), self.channels)
+
+    def patches(self, first, count):
+        P3 = self.patch ** 3
+        xin = np.zeros((count, 2, P3), dtype=np.float32)
+        tgt = np.zeros((count, P3), dtype=np.float32)
+        rc = self.L.dft_cnn_trainer_get_patches(self.h, C.c_int64(first), C.c_int64(count),
+                                                xin.ctypes.data_as(C.POINTER(C.c_float)),
+                                                tgt.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            _core_err(rc)
+        return xin, tgt
 
 
 class ScfFields(C.Structure):

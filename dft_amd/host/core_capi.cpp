@@ -7,15 +7,19 @@
 #include <string>
 #include <vector>
 
+#include <functional>
+
+#include "capi_util.hpp"
 #include "davidson.hpp"
 #include "dft_core.h"
 #include "scf.hpp"
 
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
-template <class F>
-int guarded(F&& f) {
+namespace dftc {
+int guarded_call(const std::function<void()>& f) {
     try {
         f();
         return DFT_OK;
@@ -26,6 +30,13 @@ int guarded(F&& f) {
         g_err = e.what();
         return DFT_ENUMERIC;
     }
+}
+}  // namespace dftc
+
+namespace {
+template <class F>
+int guarded(F&& f) {
+    return dftc::guarded_call(std::function<void()>(std::forward<F>(f)));
 }
 }  // namespace
 
@@ -71,8 +82,8 @@ int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_
 }
 
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
-                     int max_subspace, double tol, int max_iter, double* eig, double* psi, double* resid,
-                     int* iterations) {
+                     int max_subspace, double tol, int max_iter, const void* cnn, double* eig, double* psi,
+                     double* resid, int* iterations, int64_t* stats) {
     return guarded([&] {
         if (!g || !veff_host || !eig || nstates <= 0) throw std::invalid_argument("dft_davidson_run: bad arguments");
         dftc::Ctx ctx(0);
@@ -81,16 +92,24 @@ int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* v
         v64.upload(veff_host, (size_t)n * 8);
         dftc::check(ctx, dft_convert(ctx, v64.get(), DFT_F64, v.get(), prec, n), "dft_convert");
         dftc::BlockDavidson D(ctx, *g, prec, nstates, max_subspace > 0 ? max_subspace : std::max(3 * nstates, nstates + 16));
-        dftc::DiagonalPreconditioner P;
+        std::unique_ptr<dftc::Preconditioner> P;
+        if (cnn) P = dftc::make_cnn_preconditioner(ctx, cnn, *g);
+        else P.reset(new dftc::DiagonalPreconditioner());
         dftc::DavidsonOptions o;
         o.nconv = nconv > 0 ? nconv : nstates;
         o.tol = tol > 0 ? tol : 1.0e-7;
         o.max_iter = max_iter > 0 ? max_iter : 200;
         o.gram_mode = gram_mode;
-        dftc::DavidsonStats st = D.solve(v.get(), false, o, P);
+        dftc::DavidsonStats st = D.solve(v.get(), false, o, *P);
         std::copy(D.eigenvalues().begin(), D.eigenvalues().end(), eig);
         if (resid) std::copy(D.residuals().begin(), D.residuals().end(), resid);
         if (iterations) *iterations = st.iterations;
+        if (stats) {
+            stats[0] = st.happly_columns;
+            stats[1] = st.restarts;
+            stats[2] = P->applied();
+            stats[3] = P->fallbacks();
+        }
         if (psi) {
             // reference normalisation: sum psi^2 dV = 1
             const int nc = std::min(o.nconv, nstates);

@@ -168,6 +168,7 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     theta_h_.assign(k, 0.0);
     rn_h_.assign(k, INFINITY);
     std::vector<double> prev_rn;
+    P.reset();
     int mprev = 0;  // basis size when Qprev was taken (0: none yet)
     const int nact = nconv;
     for (int it = 1; it <= opt.max_iter; ++it) {
@@ -200,7 +201,7 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             if (rn_h_[j] < opt.tol) { ++j; continue; }
             int j1 = j;
             while (j1 < nact && rn_h_[j1] >= opt.tol) ++j1;
-            P.apply(c_, g_, prec_, veff, static_cast<char*>(R_.get()) + j * colb, n_, j1 - j, theta + j,
+            P.apply(c_, g_, prec_, veff, X_.get(), n_, static_cast<char*>(R_.get()) + j * colb, n_, j, j1 - j, theta + j,
                     static_cast<char*>(T_.get()) + (size_t)nt * colb, n_);
             nt += j1 - j;
             j = j1;

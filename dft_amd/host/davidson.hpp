@@ -23,19 +23,28 @@
 
 namespace dftc {
 
-// A correction operator T = P(R) on `ncols` residual columns.
+// A correction operator T = P(R) on the residual columns first..first+ncols-1
+// of the block.  R, T and theta_dev point at column `first`; X is the whole
+// Ritz block (columns 0..first+ncols-1 are current).
 struct Preconditioner {
     virtual ~Preconditioner() = default;
-    // Writes T columns; returns how many of the columns used the fallback.
-    virtual int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* R, int64_t ldr, int ncols,
-                      const double* theta_dev, void* T, int64_t ldt) = 0;
-    // Told after each iteration whether residual norms went down (per column index in the block).
+    // Writes the T columns; returns how many of them used the fallback.
+    virtual int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* X, int64_t ldx,
+                      const void* R, int64_t ldr, int first, int ncols, const double* theta_dev, void* T,
+                      int64_t ldt) = 0;
+    // Residual norms of every block column before and after an iteration.
     virtual void feedback(const std::vector<double>& /*before*/, const std::vector<double>& /*after*/) {}
+    // Called when a solve starts (new operator).
+    virtual void reset() {}
+    virtual int64_t applied() const { return 0; }
+    virtual int64_t fallbacks() const { return 0; }
 };
 
+// davidsonCorrection (EigenvalueSolverMolecule.cpp:483-555)
 struct DiagonalPreconditioner : Preconditioner {
-    int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* R, int64_t ldr, int ncols,
-              const double* theta_dev, void* T, int64_t ldt) override {
+    int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* /*X*/, int64_t /*ldx*/,
+              const void* R, int64_t ldr, int /*first*/, int ncols, const double* theta_dev, void* T,
+              int64_t ldt) override {
         check(c, dft_precond_diag(c, &g, prec, veff, R, ldr, ncols, theta_dev, T, ldt), "dft_precond_diag");
         return 0;
     }

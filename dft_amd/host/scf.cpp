@@ -76,13 +76,16 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             enn += nuc_[4 * a] * nuc_[4 * b] / std::sqrt(d2);
         }
 
-    std::unique_ptr<Preconditioner> precond;
-    if (cfg_.precond == DFT_PRECOND_CNN) {
-        precond = make_cnn_preconditioner(c_, cfg_.cnn, g_);
-        if (!precond) throw std::invalid_argument("learned preconditioner requested without a model");
-    } else {
-        precond.reset(new DiagonalPreconditioner());
+    std::unique_ptr<Preconditioner> own_precond;
+    if (!ext_precond_) {
+        if (cfg_.precond == DFT_PRECOND_CNN) {
+            own_precond = make_cnn_preconditioner(c_, cfg_.cnn, g_);
+            if (!own_precond) throw std::invalid_argument("learned preconditioner requested without a model");
+        } else {
+            own_precond.reset(new DiagonalPreconditioner());
+        }
     }
+    Preconditioner* precond = ext_precond_ ? ext_precond_ : own_precond.get();
 
     const bool mixed = cfg_.precision == DFT_SCF_MIXED;
     std::unique_ptr<Fields> F(new Fields(c_, g_, mixed ? DFT_F32 : DFT_F64, k, mmax, closed));
@@ -290,6 +293,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     rep.seconds_poisson = t_pois;
     rep.seconds_xc = t_xc;
     rep.seconds_davidson = t_dav;
+    rep.cnn_applied = precond->applied();
+    rep.cnn_fallbacks = precond->fallbacks();
     rep.refine_iterations = refine_steps;
     if (out.export_fields) {
         Fields& f = *F;

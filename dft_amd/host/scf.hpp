@@ -40,6 +40,8 @@ class ScfDriver {
    public:
     ScfDriver(dft_ctx* c, const dft_grid& g, int natoms, const double* nuclei, const dft_scf_config& cfg);
     void run(dft_scf_report& rep, ScfOutputs& out);
+    // Use p (not owned) for every Davidson correction instead of the one cfg selects.
+    void set_preconditioner(Preconditioner* p) { ext_precond_ = p; }
 
    private:
     struct Fields;
@@ -47,6 +49,7 @@ class ScfDriver {
     dft_grid g_;
     std::vector<double> nuc_;
     dft_scf_config cfg_;
+    Preconditioner* ext_precond_ = nullptr;
 };
 
 }  // namespace dftc

@@ -45,7 +45,7 @@ typedef struct {
     int davidson_max_iter; /* 0: 200 */
     int refine_iterations; /* mixed mode: max FP64 refinement steps after the FP32 phase stops */
     int precond;           /* DFT_PRECOND_DIAG or DFT_PRECOND_CNN */
-    const void* cnn;       /* trained network (dft_cnn_model) for DFT_PRECOND_CNN */
+    const void* cnn;       /* trained network (const dft_cnn_model*) for DFT_PRECOND_CNN */
     int verbose;           /* 1: one line per SCF step on stderr */
 } dft_scf_config;
 
@@ -98,10 +98,60 @@ int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_
 /* Block Davidson on a given potential (tests and benchmarks).  veff_host: N
  * FP64 values.  prec/gram_mode select the arithmetic.  eig: nstates values;
  * psi (optional): nconv orbitals, N each, normalised sum psi^2 dV = 1 like
- * the reference's MolecularOrbital::psi; resid (optional): nstates residuals. */
+ * the reference's MolecularOrbital::psi; resid (optional): nstates residuals.
+ * cnn: a trained network (dft_cnn_model) for the learned correction, NULL
+ * for the diagonal one.  stats (optional, 4 values): H-applied columns,
+ * restarts, learned corrections applied, diagonal fallbacks. */
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
-                     int max_subspace, double tol, int max_iter, double* eig, double* psi, double* resid,
-                     int* iterations);
+                     int max_subspace, double tol, int max_iter, const void* cnn, double* eig, double* psi,
+                     double* resid, int* iterations, int64_t* stats);
+
+/* ---- learned preconditioner: model, training-data harvest, training ----
+ * No reference counterpart (the reference preconditions with the diagonal
+ * correction only, EigenvalueSolverMolecule.cpp:483-555).  The network maps
+ * {r / max|r|, (V_eff - theta) h2} to c / max|c| (h2 = 1 / sum(1/h^2); only
+ * the direction of a correction column matters to Davidson), where c solves
+ * the projected correction equation
+ * (I - QQ^T)(H - theta)(I - QQ^T) c = -r, Q = the Ritz vectors up to and
+ * including the state's own. */
+typedef struct {
+    int channels;          /* 16 */
+    int64_t count;         /* dft_cnn_param_count(channels) */
+    const float* params;   /* FP32 master parameters (host) */
+} dft_cnn_model;
+
+typedef struct {
+    int patch;             /* patch edge (32) */
+    int max_columns;       /* residual columns harvested per Davidson correction step */
+    int patches_per_column;
+    int every;             /* harvest at every k-th correction step of the solve */
+    double cg_tol;         /* relative residual of the projected CG */
+    int cg_max_iter;
+    uint64_t seed;         /* patch jitter */
+    int target_scale;      /* 0: target c / max|c|;  1: c / (max|r| h2), the input's scale */
+} dft_cnn_harvest_config;
+
+typedef struct dft_cnn_trainer dft_cnn_trainer;
+
+/* A trainer owns a context, a device patch set of `capacity` patches and the
+ * network (parameters initialised from `seed`, He-uniform). */
+int dft_cnn_trainer_create(int channels, int patch, int64_t capacity, uint64_t seed, dft_cnn_trainer** out);
+void dft_cnn_trainer_destroy(dft_cnn_trainer* t);
+/* Run an FP64 SCF (cfg) and harvest training patches from its Davidson
+ * corrections; *added = patches added (stops adding when full). */
+int dft_cnn_trainer_harvest(dft_cnn_trainer* t, const dft_grid* g, int natoms, const double* nuclei,
+                            const dft_scf_config* cfg, const dft_cnn_harvest_config* hc, dft_scf_report* rep,
+                            int64_t* added);
+int dft_cnn_trainer_count(dft_cnn_trainer* t, int64_t* patches);
+/* Adam steps on random batches of the first `train_count` patches
+ * (0: all); loss (optional) receives one value per step. */
+int dft_cnn_trainer_train(dft_cnn_trainer* t, int steps, int batch, float lr, int64_t train_count, double* loss);
+/* Mean loss over patches [first, first + count) in batches. */
+int dft_cnn_trainer_eval(dft_cnn_trainer* t, int64_t first, int64_t count, double* loss);
+int dft_cnn_trainer_get_params(dft_cnn_trainer* t, float* params, int64_t count);
+int dft_cnn_trainer_set_params(dft_cnn_trainer* t, const float* params, int64_t count);
+/* Copy patches [first, first + count) to the host: in (count x 2 x P^3), tgt (count x P^3). */
+int dft_cnn_trainer_get_patches(dft_cnn_trainer* t, int64_t first, int64_t count, float* in, float* tgt);
 
 #ifdef __cplusplus
 }

@@ -178,28 +178,62 @@ int dft_seed_vectors(dft_ctx* ctx, const dft_grid* g, int prec, int first_index,
 int dft_hydrogenic_density(dft_ctx* ctx, const dft_grid* g, int prec, int natoms, const double* nuclei_host,
                            double electrons, void* out);
 
-/* ---- learned preconditioner: 3D CNN (BF16 weights, FP32 accumulation) ---- */
-/* Network: 4 layers of 3x3x3 convolutions, channels 2 -> C -> C -> C -> 1,
- * SiLU after layers 1-3, zero padding, input {residual, V_eff} on a D^3
- * patch (or a whole grid for inference).  Parameters are one FP32 master
- * array (dft_cnn_param_count values) plus its BF16 copy used by the
- * kernels. */
+/* ---- learned preconditioner: 3D CNN (BF16 weights, FP32 accumulation) ----
+ * No reference counterpart: the network augments the reference's
+ * davidsonCorrection (EigenvalueSolverMolecule.cpp:483-555) at its call site
+ * (:1042-1047), with the diagonal correction as the fallback.
+ * Network: 4 layers of 3x3x3 convolutions, channels 2 -> C -> C -> C -> 1,
+ * SiLU after layers 1-3, zero padding.  Parameters: one array, for each
+ * layer W[cout][cin][27] (tap = kz*9 + ky*3 + kx; the output at v reads the
+ * input at v + (kx-1, ky-1, kz-1)) followed by b[cout]; FP32 master copy
+ * plus a BF16 copy that the kernels use.  Activations are FP32,
+ * [item][channel][z][y][x].  Only C = 16 is built. */
 int dft_cnn_param_count(int channels, int64_t* count);
-/* Forward over nbatch patches of nx*ny*nz; stores activations in act
- * (dft_cnn_act_count floats) for backward when act != NULL. */
-int dft_cnn_act_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int64_t* count);
+/* Floats of the activation buffer: training = 1 keeps the three hidden
+ * pre-activations for backward, 0 uses two ping-pong layers. */
+int dft_cnn_act_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int training, int64_t* count);
+/* Forward over nbatch items of nx*ny*nz: in [nbatch][2][N], out [nbatch][N]. */
 int dft_cnn_forward(dft_ctx* ctx, int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch,
-                    const uint16_t* w_bf16, const float* in, float* out, float* act);
-/* Mean-squared-error loss vs target and its gradient wrt the parameters
- * (FP32, accumulated into grad). loss_host may be NULL. */
+                    const uint16_t* w_bf16, const float* in, float* out, float* act, int training);
+/* Mean-squared-error loss L = mean((out - target)^2) and its gradient with
+ * respect to the BF16 parameters, written (FP32) to grad.  act must come
+ * from dft_cnn_forward with training = 1.  loss_host may be NULL. */
 int dft_cnn_backward(dft_ctx* ctx, int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch,
                      const uint16_t* w_bf16, const float* in, const float* act, const float* out,
                      const float* target, float* grad, float* scratch, double* loss_host);
 int dft_cnn_scratch_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int64_t* count);
-/* Adam step on the FP32 master weights; refreshes the BF16 copy. */
+/* Adam step (bias-corrected) on the FP32 master weights; refreshes the BF16 copy. */
 int dft_cnn_adam(dft_ctx* ctx, int64_t count, float* w, uint16_t* w_bf16, const float* grad, float* m, float* v,
                  int step, float lr, float beta1, float beta2, float eps);
 int dft_cnn_to_bf16(dft_ctx* ctx, int64_t count, const float* w, uint16_t* w_bf16);
+/* Preconditioner maps.  s_b = scale_dev[b] (max |r_b|), h2 = 1 / sum(1/h^2):
+ * prepare: in[b] = {r_b / s_b, (V_eff - theta_b) h2};  finish: T_b = out_b s_b h2
+ * (a positive per-column scale: Davidson uses only each column's direction). */
+int dft_column_maxabs(dft_ctx* ctx, int prec, int64_t n, int ncols, const void* x, int64_t ldx, double* out_dev);
+int dft_cnn_prepare(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr, int ncols,
+                    const double* theta_dev, const double* scale_dev, float* in);
+int dft_cnn_finish(dft_ctx* ctx, const dft_grid* g, int prec, int ncols, const float* out, const double* scale_dev,
+                   void* t, int64_t ldt);
+
+/* ---- training-data harvest (projected CG on (H - theta) c = -r) ---- */
+/* out_j = x_j . y_j (FP64, device) */
+int dft_column_dots(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, int64_t ldx, const void* y, int64_t ldy,
+                    double* out_dev);
+/* y_j = a_j x_j + b_j y_j with a, b device arrays of nb doubles */
+int dft_cols_axpby(dft_ctx* ctx, int prec, int64_t n, int nb, const double* a_dev, const void* x, int64_t ldx,
+                   const double* b_dev, void* y, int64_t ldy);
+/* sum r^2 of each column over each aligned patch^3 block:
+ * out_dev[col * nblocks + block], block = bx + nbx*(by + nby*bz), nb* = n* / patch */
+int dft_patch_energy(dft_ctx* ctx, const dft_grid* g, int prec, int patch, const void* r, int64_t ldr, int ncols,
+                     double* out_dev);
+/* Cut npatch training patches; desc (host) = {column, x0, y0, z0} per patch,
+ * each inside the grid and column < ncols.  in [npatch][2][patch^3] =
+ * {r/s, (V_eff - theta) h2}, tgt [npatch][patch^3] = c / cs, with s and cs
+ * the columns' max |r| and max |c| (scale_dev, cscale_dev). */
+int dft_extract_patches(dft_ctx* ctx, const dft_grid* g, int prec, int patch, int npatch, const int* desc, int ncols,
+                        const void* veff, const void* r, int64_t ldr, const void* c, int64_t ldc,
+                        const double* theta_dev, const double* scale_dev, const double* cscale_dev, float* in,
+                        float* tgt);
 
 #ifdef __cplusplus
 }
