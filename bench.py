@@ -26,6 +26,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SCF time to |dE|<1e-5 Ha: mixed+learned vs plain FP64 on 1 GPU; H·psi GB/s"
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profiled_traffic(key):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated) for the kernel/shape `key`, from the summaries in
+    profiles/ ('traffic_bytes_per_launch <key>: <bytes>'); None if absent."""
+    if not os.path.isdir(PROFILES):
+        return None
+    for fn in sorted(os.listdir(PROFILES)):
+        if not fn.endswith((".md", ".txt")):
+            continue
+        with open(os.path.join(PROFILES, fn)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch ") and line.split()[1].rstrip(":") == key:
+                    return float(line.split(":")[1])
+    return None
 
 
 def benzene():
@@ -73,10 +90,11 @@ WORKLOADS = {
                functional=1, nstates=4, poisson_fixed=0, init=1),
     "c2": dict(desc="Ne atom, 96^3, h=0.2, PBE96, Davidson block 8, Jacobi 200 sweeps/step", atoms=[[10, 0.0, 0.0, 0.0]],
                n=96, h=0.2, functional=2, nstates=8, poisson_fixed=200, init=0),
-    "c3": dict(desc="benzene, 160^3, h=0.2, LSDA/PZ81, Davidson block 29", atoms=benzene(), n=160, h=0.2, functional=1,
-               nstates=29, poisson_fixed=400, init=1),
-    "c5": dict(desc="40-atom random cluster (seed 5, radius 7 bohr, 150-170 electrons), 256^3, h=0.18, PBE96, block 96",
-               atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, init=1),
+    "c3": dict(desc="benzene, 160^3, h=0.2, LSDA/PZ81, Davidson block 29, Chebyshev-accelerated Jacobi Poisson",
+               atoms=benzene(), n=160, h=0.2, functional=1, nstates=29, poisson_fixed=400, poisson="cheb", init=1),
+    "c5": dict(desc="40-atom random cluster (seed 5, radius 7 bohr, 150-170 electrons), 256^3, h=0.18, PBE96, block 96, "
+                    "Chebyshev-accelerated Jacobi Poisson",
+               atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, poisson="cheb", init=1),
 }
 
 
@@ -86,11 +104,21 @@ def workload_atoms(w):
     return np.array(random_cluster(5, 40, 7.0, ne_range=(150, 170)), dtype=float)
 
 
-def run_scf(dft_amd, grid, atoms, w, precision, max_iterations=200, verbose=False):
+def poisson_args(dft_amd, w):
+    """Jacobi with a fixed number of warm sweeps per call (C2's definition), or
+    Chebyshev-accelerated Jacobi sweeps to max|dV| < 1e-8 (C3, C5: the
+    definition leaves the Poisson schedule open)."""
+    if w.get("poisson", "jacobi") == "cheb":
+        return dict(poisson_method=dft_amd.POISSON_CHEBYSHEV, poisson_tol=1e-8, poisson_fixed=0)
+    return dict(poisson_method=dft_amd.POISSON_JACOBI, poisson_fixed=w["poisson_fixed"])
+
+
+def run_scf(dft_amd, grid, atoms, w, precision, max_iterations=200, verbose=False, precond=0, cnn=None):
     return dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=precision, nstates=w["nstates"],
-                           max_iterations=max_iterations, poisson_fixed=w["poisson_fixed"], poisson_batch=64,
+                           max_iterations=max_iterations, poisson_batch=64, **poisson_args(dft_amd, w),
                            stop_rule=dft_amd.STOP_DE, energy_tol=1e-5, init_guess=w["init"], davidson_tol=1e-7,
-                           refine_iterations=30 if precision == dft_amd.SCF_MIXED else 0, verbose=verbose)
+                           refine_iterations=30 if precision == dft_amd.SCF_MIXED else 0, verbose=verbose,
+                           precond=precond, cnn=cnn)
 
 
 def time_hpsi(dft_amd, grid, nb, prec, reps=20):
@@ -188,41 +216,63 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
     return rows
 
 
-def cpu_baseline(w, atoms, grid_n, h, rep, nocc, budget_s=20.0):
-    """Single-thread FP64 port (oracle/) of the reference loops, timed on this
-    host on a bounded sample and extrapolated to the GPU run's operation counts."""
-    from oracle.oracle import Grid, Oracle
+def cpu_baseline(w, atoms, grid_n, h, rep, nocc):
+    """The reference's CPU path on this host's cores (one thread: the reference
+    is single-threaded), timed on bounded samples and extrapolated to the
+    reference algorithm at the workload size for the FP64 GPU run's SCF step
+    count: per step two Poisson calls with the reference rule (boundary pass
+    plus Jacobi from zero to 1e-8, capped at 10 000 sweeps: the cap is reached
+    beyond ~68^3), two XC potentials, and the GPU run's H applications at the
+    single-vector Davidson's cost per H application (its MGS and dot products
+    included).  Uses the reference compiled from its sources (oracle/_ref,
+    kind "reference") where present, else the FP64 restatement (kind "port",
+    which computes the boundary charge once per call and so understates the
+    reference's boundary pass)."""
+    from oracle.oracle import Grid, Oracle, Reference
     O = Oracle()
+    kind = "reference" if Reference.available() else "port"
+    R = Reference() if kind == "reference" else None
     g = Grid.centered(grid_n, h)
     rng = np.random.default_rng(0)
-    v = rng.standard_normal(g.size)
-    p = rng.standard_normal(g.size)
+    v, p = rng.standard_normal(g.size), rng.standard_normal(g.size)
+    hp = R.hpsi if R else O.hpsi
+    hp(g, v, p)  # first touch
     t = time.perf_counter()
-    O.hpsi(g, v, p)
+    hp(g, v, p)
     t_hpsi = time.perf_counter() - t
-    sweeps = max(1, min(50, int(budget_s / 4 / max(t_hpsi * 2.5, 1e-9))))
+    # one Jacobi sweep costs about one H apply per point; the boundary pass is
+    # timed alone as a call on a zero density (its first sweep already meets
+    # the tolerance) at two sizes and fitted as a power of n
+    tb = {}
+    for m in (32, 40):
+        gm = Grid.centered(m, h * grid_n / m)
+        zero = np.zeros(gm.size)
+        pcall = (lambda: R.poisson(gm, zero)) if R else (lambda: O.poisson(gm, zero))
+        pcall()
+        t = time.perf_counter()
+        pcall()
+        tb[m] = max(time.perf_counter() - t, 1e-6)
+    pw = float(np.log(tb[40] / tb[32]) / np.log(40 / 32))
+    t_bound = tb[40] * (grid_n / 40) ** pw
+    t_poisson = t_bound + 10000 * t_hpsi
+    gx = Grid.centered(min(grid_n, 48), h * grid_n / min(grid_n, 48))
+    na = np.abs(rng.standard_normal(gx.size)) * 0.1
     t = time.perf_counter()
-    O.poisson(g, np.abs(p) * 1e-3, v0=np.zeros(g.size), max_sweeps=sweeps, tol=0.0)
-    t_sweep = (time.perf_counter() - t) / sweeps
-    na = np.abs(p) * 0.1
-    t = time.perf_counter()
-    O.xc_potential(g, w["functional"], na, na, 0)
-    t_xc = time.perf_counter() - t
-    # Davidson cost per H application incl. the reference's MGS/dot work,
-    # measured on a 32^3 grid and scaled by the point count
-    gs = Grid.centered(32, h * grid_n / 32)
+    (R.xc_potential if R else O.xc_potential)(gx, w["functional"], na, na, 0)
+    t_xc = (time.perf_counter() - t) * g.size / gx.size
+    gs = Grid.centered(24, h * grid_n / 24)
     vs = O.nuclear_potential(gs, [[float(atoms[0][0]), 0.013, 0.021, 0.007]])
-    t = time.perf_counter()
     _, _, hap = O.davidson(gs, vs, 2)
+    t = time.perf_counter()
+    (R.davidson if R else O.davidson)(gs, vs, 2)
     t_dav = (time.perf_counter() - t) / max(int(hap.sum()), 1) * (g.size / gs.size)
     steps = rep["iterations"]
-    est = (rep["happly_columns"] * t_dav + rep["poisson_sweeps"] * t_sweep + steps * 2 * t_xc
-           + steps * nocc * t_hpsi)
-    sample = (f"{grid_n}^3 grid, 1 thread: H psi x1 ({t_hpsi * 1e3:.1f} ms), {sweeps} Jacobi sweeps "
-              f"({t_sweep * 1e3:.1f} ms/sweep), 1 XC potential ({t_xc:.2f} s), single-vector Davidson on 32^3 "
-              f"({t_dav * 1e3:.1f} ms per H apply scaled to {grid_n}^3); extrapolated to the GPU run's counts "
-              f"({rep['happly_columns']} H applies, {rep['poisson_sweeps']} sweeps, {steps} SCF steps)")
-    return dict(value=est, unit="s", cores=1, kind="port", sample=sample)
+    est = steps * (2 * t_poisson + 2 * t_xc) + rep["happly_columns"] * t_dav
+    sample = (f"1 thread, {kind}: H psi at {grid_n}^3 ({t_hpsi * 1e3:.0f} ms, = one Jacobi sweep); boundary pass at "
+              f"32^3 and 40^3 -> ~n^{pw:.1f}, {t_bound:.0f} s at {grid_n}^3, plus 10000 sweeps per call; "
+              f"XC at 48^3 scaled ({t_xc:.2f} s); single-vector Davidson at 24^3 ({t_dav * 1e3:.0f} ms per H apply "
+              f"scaled); extrapolated to {steps} SCF steps x (2 Poisson + 2 XC) + {rep['happly_columns']} H applies")
+    return dict(value=float(est), unit="s", cores=1, kind=kind, sample=sample)
 
 
 def main():
@@ -230,11 +280,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default=os.environ.get("DFT_BENCH_WORKLOAD", "c2"), choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=os.environ.get("DFT_BENCH_WORKLOAD", "c3"), choices=sorted(WORKLOADS))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--hpsi-only", type=int, default=0, help="only run N H psi launches (profiling)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="only time the hot kernels at the workload size")
+    ap.add_argument("--precond", default="kinetic", choices=["diag", "kinetic", "cnn"],
+                    help="Davidson correction of the timed (mixed) run; the FP64 run uses the diagonal one")
+    ap.add_argument("--cnn-model", default=None, help="trained network (.npz) for --precond cnn")
     ap.add_argument("--profile-scf", type=int, default=0,
                     help="only run this many SCF steps of each precision (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -270,7 +323,7 @@ def main():
         for prec in (dft_amd.SCF_FP64, dft_amd.SCF_MIXED):
             t = time.perf_counter()
             r = dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=prec, nstates=w["nstates"],
-                                max_iterations=a.profile_scf, poisson_fixed=w["poisson_fixed"], poisson_batch=64,
+                                max_iterations=a.profile_scf, poisson_batch=64, **poisson_args(dft_amd, w),
                                 stop_rule=dft_amd.STOP_DE, energy_tol=1e-5, init_guess=w["init"], davidson_tol=1e-7,
                                 refine_iterations=0)
             print(json.dumps({"precision": prec, "steps": r["iterations"], "seconds": time.perf_counter() - t,
@@ -284,22 +337,33 @@ def main():
         return
     ne = int(round(atoms[:, 0].sum()))
     nocc = ne // 2
+    pc = {"diag": dft_amd.PRECOND_DIAG, "kinetic": dft_amd.PRECOND_KINETIC, "cnn": dft_amd.PRECOND_CNN}[a.precond]
+    model = dft_amd.CnnModel.load(a.cnn_model) if a.precond == "cnn" else None
     print(f"[bench] workload {a.workload}: {w['desc']}; {len(atoms)} atoms, {ne} electrons", file=sys.stderr, flush=True)
     for i in range(a.warmup):
         dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=dft_amd.SCF_MIXED, nstates=w["nstates"],
-                        max_iterations=2, poisson_fixed=w["poisson_fixed"] or 50, stop_rule=dft_amd.STOP_DE,
+                        max_iterations=2, poisson_fixed=50, stop_rule=dft_amd.STOP_DE,
                         energy_tol=1e-5, init_guess=w["init"], refine_iterations=1)
         print(f"[bench] warmup {i + 1} done", file=sys.stderr, flush=True)
     # plain FP64 GPU path (the accuracy reference and the speedup denominator)
     t = time.perf_counter()
     r64 = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, verbose=a.verbose)
     t64 = time.perf_counter() - t
-    print(f"[bench] fp64 SCF: E={r64['total']:.10f} steps={r64['iterations']} {t64:.2f} s", file=sys.stderr, flush=True)
+    print(f"[bench] fp64 SCF (diagonal correction): E={r64['total']:.10f} steps={r64['iterations']} {t64:.2f} s",
+          file=sys.stderr, flush=True)
+    t64p, r64p = None, None
+    if pc != dft_amd.PRECOND_DIAG:
+        # the same correction in FP64: the precision-only speed-up
+        t = time.perf_counter()
+        r64p = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, verbose=a.verbose, precond=pc, cnn=model)
+        t64p = time.perf_counter() - t
+        print(f"[bench] fp64 SCF ({a.precond}): E={r64p['total']:.10f} steps={r64p['iterations']} {t64p:.2f} s",
+              file=sys.stderr, flush=True)
     # timed region: K mixed-precision solves
     runs = []
     t0 = time.perf_counter()
     for k in range(a.steps):
-        runs.append(run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_MIXED, verbose=a.verbose))
+        runs.append(run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_MIXED, verbose=a.verbose, precond=pc, cnn=model))
         print(f"[bench] mixed SCF {k + 1}: E={runs[-1]['total']:.10f} steps={runs[-1]['iterations']} "
               f"(refine {runs[-1]['refine_iterations']})", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
@@ -314,9 +378,16 @@ def main():
         "ms_per_step": per * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+bf16x3 (f64 subspace, f64 refinement)", "data": "synthetic",
         "config": {"workload": f"{a.workload}: {w['desc']}", "grid": w["n"], "h": w["h"], "atoms": len(atoms),
-                   "electrons": ne, "davidson_block": w["nstates"], "poisson_sweeps_per_call": w["poisson_fixed"],
-                   "stop": "|dE| < 1e-5 Ha", "preconditioner": "diagonal (learned model not loaded)"},
+                   "electrons": ne, "davidson_block": w["nstates"],
+                   "poisson": ("chebyshev to max|dV|<1e-8" if w.get("poisson") == "cheb"
+                               else f"{w['poisson_fixed']} warm Jacobi sweeps per call"),
+                   "stop": "|dE| < 1e-5 Ha",
+                   "preconditioner": {"diag": "diagonal (the reference's)", "kinetic": "kinetic smoothing",
+                                      "cnn": f"learned ({a.cnn_model}) with diagonal fallback"}[a.precond],
+                   "fp64_preconditioner": "diagonal"},
         "fp64_seconds": t64, "speedup_vs_fp64": t64 / per if per > 0 else None,
+        "fp64_same_precond_seconds": t64p, "speedup_vs_fp64_same_precond": (t64p / per) if t64p else None,
+        "energy_fp64_same_precond": r64p["total"] if r64p else None,
         "energy_fp64": r64["total"], "energy_mixed": rm["total"], "energy_error_vs_fp64": abs(rm["total"] - r64["total"]),
         "eig_error_vs_fp64": float(np.max(np.abs(np.array(rm["eigenvalues"]) - np.array(r64["eigenvalues"])))),
         "scf_steps_fp64": r64["iterations"], "scf_steps_mixed": rm["iterations"], "refine_steps": rm["refine_iterations"],
@@ -324,7 +395,8 @@ def main():
         "hpsi_f32_GBps": ach, "hpsi_f64_GBps": by64 / ms64 / 1e6,
         "hpsi_f32_gpt_orb_per_s": grid.size * nb / (ms * 1e-3), "hpsi_f64_gpt_orb_per_s": grid.size * nb / (ms64 * 1e-3),
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": f"H psi FP32, {w['n']}^3 x {nb} orbitals",
+                     "traffic": profiled_traffic(f"hpsi_f32_{w['n']}x{w['n']}x{w['n']}_nb{nb}"),
+                     "kernel": f"H psi FP32, {w['n']}^3 x {nb} orbitals",
                      "bytes_per_launch": by, "avg_launch_ms": ms},
     }
     if not a.skip_cpu:

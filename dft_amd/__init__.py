@@ -222,6 +222,13 @@ class Device:
                                      C.c_int(batch), C.byref(sw), C.byref(md)))
         return sw.value, md.value
 
+    def poisson_chebyshev(self, g, rho, v, max_sweeps=100000, tol=1e-8, zero_start=True, batch=64):
+        sw, md = C.c_int64(), C.c_double()
+        self._chk(self.L.dft_poisson_chebyshev(self.ctx, C.byref(g), C.c_int(rho.prec), _vp(rho.ptr), _vp(v.ptr),
+                                               C.c_int64(max_sweeps), C.c_double(tol), C.c_int(1 if zero_start else 0),
+                                               C.c_int(batch), C.byref(sw), C.byref(md)))
+        return sw.value, md.value
+
     def hartree_energy(self, g, rho, vh):
         e = C.c_double()
         self._chk(self.L.dft_hartree_energy(self.ctx, C.byref(g), C.c_int(rho.prec), _vp(rho.ptr), _vp(vh.ptr), C.byref(e)))
@@ -371,7 +378,8 @@ class Device:
 SCF_FP64, SCF_MIXED = 0, 1
 STOP_REFERENCE, STOP_DE = 0, 1
 INIT_BARE, INIT_HYDROGENIC = 0, 1
-PRECOND_DIAG, PRECOND_CNN = 0, 1
+PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC = 0, 1, 2
+POISSON_JACOBI, POISSON_CHEBYSHEV = 0, 1
 
 
 class ScfConfig(C.Structure):
@@ -379,7 +387,8 @@ class ScfConfig(C.Structure):
 
     _fields_ = [("functional", C.c_int), ("charge", C.c_int), ("multiplicity", C.c_int), ("precision", C.c_int),
                 ("nstates", C.c_int), ("max_subspace", C.c_int), ("max_iterations", C.c_int),
-                ("poisson_fixed", C.c_int), ("poisson_batch", C.c_int), ("stop_rule", C.c_int),
+                ("poisson_fixed", C.c_int), ("poisson_batch", C.c_int), ("poisson_method", C.c_int),
+                ("poisson_tol", C.c_double), ("stop_rule", C.c_int),
                 ("energy_tol", C.c_double), ("init_guess", C.c_int), ("davidson_tol", C.c_double),
                 ("davidson_max_iter", C.c_int), ("refine_iterations", C.c_int), ("precond", C.c_int),
                 ("cnn", C.c_void_p), ("verbose", C.c_int)]
@@ -402,14 +411,15 @@ def _core_err(rc):
 
 
 def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=SCF_FP64, nstates=0, max_subspace=0,
-            max_iterations=250, poisson_fixed=0, poisson_batch=64, stop_rule=STOP_REFERENCE, energy_tol=1e-5,
+            max_iterations=250, poisson_fixed=0, poisson_batch=64, poisson_method=POISSON_JACOBI, poisson_tol=0.0,
+            stop_rule=STOP_REFERENCE, energy_tol=1e-5,
             init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
             precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False):
     """Run the device-resident SCF; returns a dict of results."""
     L = core_lib()
     nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
-                    poisson_fixed, poisson_batch, stop_rule, energy_tol, init_guess, davidson_tol,
+                    poisson_fixed, poisson_batch, poisson_method, poisson_tol, stop_rule, energy_tol, init_guess, davidson_tol,
                     davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
@@ -437,7 +447,7 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
 
 
 def davidson_run(grid, veff, nstates, nconv=None, prec=F64, gram_mode=GRAM_F64, max_subspace=0, tol=1e-7,
-                 max_iter=0, want_psi=False, cnn=None):
+                 max_iter=0, want_psi=False, cnn=None, precond=None):
     """Block Davidson on a fixed potential; cnn: a CnnModel for the learned correction."""
     L = core_lib()
     v = np.ascontiguousarray(veff, dtype=np.float64)
@@ -449,6 +459,7 @@ def davidson_run(grid, veff, nstates, nconv=None, prec=F64, gram_mode=GRAM_F64, 
     psi = np.zeros((nconv, grid.size)) if want_psi else None
     rc = L.dft_davidson_run(C.byref(grid), C.c_int(prec), C.c_int(gram_mode), v.ctypes.data_as(_dp), C.c_int(nstates),
                             C.c_int(nconv), C.c_int(max_subspace), C.c_double(tol), C.c_int(max_iter),
+                            C.c_int(precond if precond is not None else (PRECOND_CNN if cnn is not None else PRECOND_DIAG)),
                             _model_ptr(cnn), eig.ctypes.data_as(_dp), None if psi is None else psi.ctypes.data_as(_dp),
                             res.ctypes.data_as(_dp), C.byref(its), stats.ctypes.data_as(C.POINTER(C.c_int64)))
     if rc != 0:
@@ -534,9 +545,11 @@ class CnnTrainer:
 
     def harvest(self, grid, nuclei, functional=PZ81, nstates=0, max_iterations=30, poisson_fixed=0, energy_tol=1e-5,
                 init_guess=INIT_HYDROGENIC, davidson_tol=1e-6, max_columns=8, patches_per_column=4, every=3,
+                poisson_method=POISSON_JACOBI,
                 cg_tol=1e-6, cg_max_iter=400, seed=7, multiplicity=1, target_scale=0):
         nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
-        cfg = ScfConfig(functional, 0, multiplicity, SCF_FP64, nstates, 0, max_iterations, poisson_fixed, 64, STOP_DE,
+        cfg = ScfConfig(functional, 0, multiplicity, SCF_FP64, nstates, 0, max_iterations, poisson_fixed, 64,
+                        poisson_method, 0.0, STOP_DE,
                         energy_tol, init_guess, davidson_tol, 0, 0, PRECOND_DIAG, None, 0)
         hc = HarvestConfig(self.patch, max_columns, patches_per_column, every, cg_tol, cg_max_iter, seed, target_scale)
         rep = ScfReport()
@@ -607,7 +620,8 @@ def scf_step(grid, nuclei, rho_alpha, rho_beta, functional=PZ81, multiplicity=1,
     n = grid.size
     ra = np.ascontiguousarray(rho_alpha, dtype=np.float64)
     rb = np.ascontiguousarray(rho_beta, dtype=np.float64)
-    cfg = ScfConfig(functional, 0, multiplicity, precision, nstates, 0, 1, poisson_fixed, 64, STOP_DE, 0.0,
+    cfg = ScfConfig(functional, 0, multiplicity, precision, nstates, 0, 1, poisson_fixed, 64, POISSON_JACOBI, 0.0,
+                    STOP_DE, 0.0,
                     INIT_BARE, davidson_tol, 0, 0, PRECOND_DIAG, None, 0)
     rep = ScfReport()
     out = {k: np.zeros(n) for k in ("vh", "vxc_alpha", "vxc_beta", "rho_alpha", "rho_beta", "vh_out")}

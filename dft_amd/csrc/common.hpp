@@ -103,6 +103,16 @@ __device__ inline T block_sum(T v, T* lds32) {
     return s;
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear workgroup id b runs on XCD b % 8); this maps b to a tile index
+// so that each XCD owns one contiguous range of tiles, whose neighbouring
+// tiles then share that XCD's L2 (halo lines are fetched from HBM once).
+__device__ inline int64_t xcd_tile(int64_t b, int64_t total) {
+    const int64_t per = total / 8, rem = total % 8;
+    const int64_t xcd = b % 8, slot = b / 8;
+    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
 // Ordered bit pattern of a non-negative double for atomicMax.
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 

@@ -87,9 +87,56 @@ __global__ __launch_bounds__(64) void k_colnorm_final(const double* __restrict__
     if (threadIdx.x == 0) out[j] = s;
 }
 
+// first: t = c r;  else t = t + c (r - kt)   (per column, FP64 arithmetic)
+template <typename T>
+__global__ __launch_bounds__(256) void k_smooth_step(int64_t n, double cf, const T* __restrict__ r, int64_t ldr,
+                                                     const T* __restrict__ kt, int64_t ldk, T* __restrict__ t,
+                                                     int64_t ldt, int first) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (p >= n) return;
+    const double rv = (double)r[(int64_t)j * ldr + p];
+    T* tp = t + (int64_t)j * ldt + p;
+    *tp = first ? (T)(cf * rv) : (T)((double)*tp + cf * (rv - (double)kt[(int64_t)j * ldk + p]));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(int64_t n, double v, T* __restrict__ x) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) x[p] = (T)v;
+}
+
 }  // namespace
 
 extern "C" {
+
+int dft_smooth_step(dft_ctx* c, int prec, int64_t n, int nb, double cf, const void* r, int64_t ldr, const void* kt,
+                    int64_t ldk, void* t, int64_t ldt, int first) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, r && t && (first || kt) && n > 0 && nb >= 0 && nb <= 65535 && ldr >= n && ldt >= n && (first || ldk >= n),
+                "dft_smooth_step: bad arguments");
+    if (!nb) return DFT_OK;
+    dim3 g((unsigned)((n + 255) / 256), nb);
+    if (prec == DFT_F64)
+        k_smooth_step<double><<<g, 256, 0, c->stream>>>(n, cf, (const double*)r, ldr, (const double*)kt, ldk, (double*)t, ldt, first);
+    else if (prec == DFT_F32)
+        k_smooth_step<float><<<g, 256, 0, c->stream>>>(n, cf, (const float*)r, ldr, (const float*)kt, ldk, (float*)t, ldt, first);
+    else
+        return set_err(c, DFT_EINVAL, "dft_smooth_step: bad precision");
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+int dft_fill(dft_ctx* c, int prec, int64_t n, double value, void* x) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, x && n > 0, "dft_fill: bad arguments");
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (prec == DFT_F64) k_fill<double><<<g, 256, 0, c->stream>>>(n, value, (double*)x);
+    else if (prec == DFT_F32) k_fill<float><<<g, 256, 0, c->stream>>>(n, value, (float*)x);
+    else return set_err(c, DFT_EINVAL, "dft_fill: bad precision");
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
 
 int dft_density(dft_ctx* c, int prec, const void* psi, int64_t ld, int nb, const double* occ_host, int64_t n, void* out,
                 int accumulate) {

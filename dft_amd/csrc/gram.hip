@@ -219,6 +219,19 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm
         }
 }
 
+// First stage of the K-split reduction: segment s of `step` partials -> part2[s]
+// (enough workgroups to keep many loads in flight; fixed summation order).
+__global__ __launch_bounds__(256) void k_gram_reduce1(const double* __restrict__ part, int nks, int64_t tot, int step,
+                                                      double* __restrict__ part2) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int seg = blockIdx.y;
+    if (e >= tot) return;
+    const int k0 = seg * step, k1 = min(nks, k0 + step);
+    double s = 0.0;
+    for (int k = k0; k < k1; ++k) s += part[(int64_t)k * tot + e];
+    part2[(int64_t)seg * tot + e] = s;
+}
+
 __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ part, int nks, int ma, int mb,
                                                      double alpha, double beta, double* __restrict__ out, int64_t ldo) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -251,7 +264,9 @@ int dft_gram(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, cons
     nks = (n + kchunk - 1) / kchunk;
     nks = (nks + 7) / 8 * 8;
     DFT_REQUIRE(c, nks * ntiles < (1LL << 31), "dft_gram: too many workgroups");
-    if (int rc = ensure_partials(c, (size_t)nks * ma * mb + 8)) return rc;
+    const int64_t tot = (int64_t)ma * mb;
+    const int nseg = (int)std::min<int64_t>(nks, 32), step = (int)((nks + nseg - 1) / nseg);
+    if (int rc = ensure_partials(c, (size_t)(nks + nseg) * tot + 8)) return rc;
     const unsigned nblk = (unsigned)(nks * ntiles);
     if (mode == DFT_GRAM_F64)
         k_gram<DFT_GRAM_F64><<<nblk, 256, 0, c->stream>>>(n, ma, mb, ntm, ntiles, (int)nks, (const double*)a, lda, (const double*)b, ldb, kchunk, c->partials);
@@ -260,8 +275,10 @@ int dft_gram(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, cons
     else
         k_gram<DFT_GRAM_SPLIT_BF16><<<nblk, 256, 0, c->stream>>>(n, ma, mb, ntm, ntiles, (int)nks, (const float*)a, lda, (const float*)b, ldb, kchunk, c->partials);
     DFT_CHECK_LAUNCH(c);
-    const int64_t tot = (int64_t)ma * mb;
-    k_gram_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(c->partials, (int)nks, ma, mb, alpha, beta, out, ldo);
+    double* part2 = c->partials + nks * tot;
+    k_gram_reduce1<<<dim3((unsigned)((tot + 255) / 256), nseg), 256, 0, c->stream>>>(c->partials, (int)nks, tot, step, part2);
+    DFT_CHECK_LAUNCH(c);
+    k_gram_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(part2, nseg, ma, mb, alpha, beta, out, ldo);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }

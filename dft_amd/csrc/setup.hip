@@ -62,6 +62,19 @@ __global__ __launch_bounds__(256) void k_hydrogenic(int nx, int ny, int nz, doub
 }
 
 // dst(j, i) = src(i, j) for an r x c block (column-major, FP64)
+// Counter-based uniform values in [-1, 1): splitmix64 of (seed, column, point).
+template <typename T>
+__global__ __launch_bounds__(256) void k_random(int64_t n, uint64_t seed, T* __restrict__ out, int64_t ld) {
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (p >= n) return;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL * ((uint64_t)j * 0x100000000ULL + (uint64_t)p + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    out[(int64_t)j * ld + p] = (T)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+}
+
 __global__ __launch_bounds__(256) void k_transpose(int r, int c, const double* __restrict__ src, int64_t lds,
                                                    double* __restrict__ dst, int64_t ldd) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -116,6 +129,18 @@ int dft_transpose(dft_ctx* c, int rows, int cols, const double* src, int64_t lds
     const int64_t tot = (int64_t)rows * cols;
     if (!tot) return DFT_OK;
     k_transpose<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(rows, cols, src, lds, dst, ldd);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+int dft_random_vectors(dft_ctx* c, int prec, int64_t n, int count, uint64_t seed, void* out, int64_t ld) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, out && n > 0 && count >= 0 && count <= 65535 && ld >= n, "dft_random_vectors: bad arguments");
+    if (!count) return DFT_OK;
+    const dim3 grid((unsigned)((n + 255) / 256), count);
+    if (prec == DFT_F64) k_random<double><<<grid, 256, 0, c->stream>>>(n, seed, (double*)out, ld);
+    else if (prec == DFT_F32) k_random<float><<<grid, 256, 0, c->stream>>>(n, seed, (float*)out, ld);
+    else return set_err(c, DFT_EINVAL, "dft_random_vectors: bad precision");
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }

@@ -50,9 +50,13 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
                                               const T* __restrict__ V, const T* __restrict__ psi, int64_t ld,
                                               T* __restrict__ out, int64_t ldo) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 64, x = x0 + lane;
-    const int ybase = blockIdx.y * (4 * TY) + w * TY;
-    const int chunk = blockIdx.z % nzc, grp = blockIdx.z / nzc;
+    const int64_t tile = xcd_tile(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z),
+                                  (int64_t)gridDim.x * gridDim.y * gridDim.z);
+    const int bx = (int)(tile % gridDim.x), by = (int)((tile / gridDim.x) % gridDim.y);
+    const int bz = (int)(tile / ((int64_t)gridDim.x * gridDim.y));
+    const int x0 = bx * 64, x = x0 + lane;
+    const int ybase = by * (4 * TY) + w * TY;
+    const int chunk = bz % nzc, grp = bz / nzc;
     const int k0 = chunk * zc, k1 = min(nz, k0 + zc);
     const int plane = nx * ny;  // the launcher guarantees nx*ny*nz < 2^31: 32-bit offsets
     const bool xin = x < nx;

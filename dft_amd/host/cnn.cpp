@@ -6,8 +6,9 @@
 // H only (davidsonCorrection, EigenvalueSolverMolecule.cpp:483-555, called at
 // :1042-1047; its non-finite / tiny-denominator fallbacks are :1049-1104).
 // Here that diagonal correction stays the fallback: a column whose residual
-// norm did not go down after a learned correction gets the diagonal one at
-// the next iteration, and after two such failures for the rest of the solve.
+// norm did not go down (by at least 5%) after a learned correction gets the
+// diagonal one at the next iteration, and after two such failures for the
+// rest of the solve.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -134,7 +135,8 @@ class CnnPreconditioner final : public Preconditioner {
         const size_t n = std::min({before.size(), after.size(), mode_.size()});
         for (size_t j = 0; j < n; ++j) {
             if (mode_[j] == kLearned) {
-                const bool reduced = after[j] < before[j];
+                // "reduced" asks for real progress: a 5% drop of the column's residual norm
+                const bool reduced = after[j] < kProgress * before[j];
                 if (!reduced) ++fails_[j];
                 mode_[j] = reduced ? kNone : kFallbackNext;
             } else if (mode_[j] == kDiagonal) {
@@ -152,6 +154,7 @@ class CnnPreconditioner final : public Preconditioner {
 
    private:
     static constexpr double kBudget = 6.0e9;  // bytes of activations per forward pass
+    static constexpr double kProgress = 0.95;
     enum : int { kNone = 0, kLearned = 1, kDiagonal = 2, kFallbackNext = 3 };
     dft_ctx* c_;
     dft_grid g_;

@@ -82,8 +82,8 @@ int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_
 }
 
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
-                     int max_subspace, double tol, int max_iter, const void* cnn, double* eig, double* psi,
-                     double* resid, int* iterations, int64_t* stats) {
+                     int max_subspace, double tol, int max_iter, int precond, const void* cnn, double* eig,
+                     double* psi, double* resid, int* iterations, int64_t* stats) {
     return guarded([&] {
         if (!g || !veff_host || !eig || nstates <= 0) throw std::invalid_argument("dft_davidson_run: bad arguments");
         dftc::Ctx ctx(0);
@@ -92,9 +92,7 @@ int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* v
         v64.upload(veff_host, (size_t)n * 8);
         dftc::check(ctx, dft_convert(ctx, v64.get(), DFT_F64, v.get(), prec, n), "dft_convert");
         dftc::BlockDavidson D(ctx, *g, prec, nstates, max_subspace > 0 ? max_subspace : std::max(3 * nstates, nstates + 16));
-        std::unique_ptr<dftc::Preconditioner> P;
-        if (cnn) P = dftc::make_cnn_preconditioner(ctx, cnn, *g);
-        else P.reset(new dftc::DiagonalPreconditioner());
+        std::unique_ptr<dftc::Preconditioner> P = dftc::make_preconditioner(ctx, precond, cnn, *g);
         dftc::DavidsonOptions o;
         o.nconv = nconv > 0 ? nconv : nstates;
         o.tol = tol > 0 ? tol : 1.0e-7;

@@ -147,6 +147,31 @@ int BlockDavidson::restart(int m, int mprev) {
     return mn;
 }
 
+int KineticPreconditioner::apply(dft_ctx* c, const dft_grid& g, int prec, const void* /*veff*/, const void* /*X*/,
+                                 int64_t /*ldx*/, const void* R, int64_t ldr, int /*first*/, int ncols,
+                                 const double* /*theta_dev*/, void* T, int64_t ldt) {
+    if (ncols <= 0) return 0;
+    const int64_t n = g.nx * g.ny * g.nz;
+    const size_t es = esize(prec);
+    if (sig_prec_ != prec) {
+        sig_ = Buf(c, n * es);
+        check(c, dft_fill(c, prec, n, sigma_, sig_.get()), "dft_fill");
+        sig_prec_ = prec;
+        kt_cols_ = 0;
+    }
+    if (kt_cols_ < ncols) {
+        kt_ = Buf(c, (size_t)n * ncols * es);
+        kt_cols_ = ncols;
+    }
+    const double D = 1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy) + 1.0 / (g.dz * g.dz) + sigma_;
+    check(c, dft_smooth_step(c, prec, n, ncols, 1.0 / D, R, ldr, nullptr, n, T, ldt, 1), "dft_smooth_step");
+    for (int s = 0; s < sweeps_; ++s) {
+        check(c, dft_hpsi(c, &g, prec, sig_.get(), T, ldt, ncols, kt_.get(), n), "dft_hpsi");
+        check(c, dft_smooth_step(c, prec, n, ncols, omega_ / D, R, ldr, kt_.get(), n, T, ldt, 0), "dft_smooth_step");
+    }
+    return 0;
+}
+
 DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOptions& opt, Preconditioner& P) {
     DavidsonStats st;
     gram_mode_ = opt.gram_mode;
@@ -160,6 +185,21 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     double* theta = theta_.as<double>();
     if (!warm) check(c_, dft_seed_vectors(c_, &g_, prec_, 0, k, X_.get(), n_), "dft_seed_vectors");
     int m = orthonormalise(X_.get(), k, V, T_.get());
+    if (m < k) {
+        // The starting block is numerically rank deficient (wide seed Gaussians
+        // on large boxes, or FP32 storage): complete it with pseudo-random
+        // directions orthogonalised against the kept ones.
+        const int need = k - m;
+        const size_t colb0 = (size_t)n_ * es_;
+        check(c_, dft_random_vectors(c_, prec_, n_, need, 0x5eedULL + (uint64_t)m, T_.get(), n_), "dft_random_vectors");
+        if (m > 0)
+            for (int pass = 0; pass < 2; ++pass) {
+                gram(m, need, V, T_.get(), P_.as<double>(), mmax_);
+                check(c_, dft_block_update(c_, prec_, n_, m, need, -1.0, V, n_, P_.as<double>(), mmax_, 1.0, T_.get(), n_),
+                      "dft_block_update");
+            }
+        m += orthonormalise(T_.get(), need, V + (size_t)m * colb0, R_.get());
+    }
     if (m < k) throw std::runtime_error("No se pudo construir el vector inicial.");
     hpsi(veff, V, W, k);
     st.happly_columns += k;

@@ -50,6 +50,28 @@ struct DiagonalPreconditioner : Preconditioner {
     }
 };
 
+// Kinetic smoothing correction: t ~ (-1/2 Lap + sigma)^{-1} r by `sweeps`
+// damped-Jacobi steps started from the diagonal solution r / D,
+// D = sum(1/h^2) + sigma.  A polynomial low-pass filter on the residual: the
+// high-frequency components (kinetic energy far above the occupied
+// spectrum) are damped relative to the smooth ones, which the diagonal
+// correction (a near-constant scaling on these grids) cannot do.  No
+// reference counterpart; the diagonal correction stays the reference's choice.
+class KineticPreconditioner : public Preconditioner {
+   public:
+    KineticPreconditioner(int sweeps = 4, double sigma = 1.0, double omega = 2.0 / 3.0)
+        : sweeps_(sweeps), sigma_(sigma), omega_(omega) {}
+    int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* X, int64_t ldx, const void* R,
+              int64_t ldr, int first, int ncols, const double* theta_dev, void* T, int64_t ldt) override;
+
+   private:
+    int sweeps_;
+    double sigma_, omega_;
+    Buf kt_, sig_;
+    int64_t kt_cols_ = 0;
+    int sig_prec_ = -1;
+};
+
 struct DavidsonOptions {
     int nconv = 1;         // leading states that must reach tol
     double tol = 1.0e-7;   // EigenvalueSolverMolecule.cpp:19

@@ -19,6 +19,8 @@ double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::dura
 // DFTConstants.h:14-19 and SelfConsistentFieldMolecule.cpp:27-31
 constexpr double kMixing0 = 0.25, kMinMix = 0.10, kMaxMix = 0.50, kMixUp = 1.10, kMixDown = 0.75, kOsc = 1.05;
 constexpr double kEnergyTol = 1.0e-11, kDensityTol = 1.0e-9, kKsTol = 1.0e-7;
+// mixed mode: where the FP32 phase hands over to FP64
+constexpr double kF32EnergyFloor = 3.0e-4, kF32DensityFloor = 1.0e-5;
 
 }  // namespace
 
@@ -36,6 +38,22 @@ struct ScfDriver::Fields {
         if (!closed) db.reset(new BlockDavidson(c, g, p, k, mmax));
     }
 };
+
+std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g) {
+    switch (kind) {
+        case DFT_PRECOND_DIAG:
+            return std::unique_ptr<Preconditioner>(new DiagonalPreconditioner());
+        case DFT_PRECOND_KINETIC:
+            return std::unique_ptr<Preconditioner>(new KineticPreconditioner());
+        case DFT_PRECOND_CNN: {
+            auto p = make_cnn_preconditioner(c, model, g);
+            if (!p) throw std::invalid_argument("learned preconditioner requested without a model");
+            return p;
+        }
+        default:
+            throw std::invalid_argument("unknown preconditioner");
+    }
+}
 
 ScfDriver::ScfDriver(dft_ctx* c, const dft_grid& g, int natoms, const double* nuclei, const dft_scf_config& cfg)
     : c_(c), g_(g), nuc_(nuclei, nuclei + 4 * (size_t)natoms), cfg_(cfg) {}
@@ -77,14 +95,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         }
 
     std::unique_ptr<Preconditioner> own_precond;
-    if (!ext_precond_) {
-        if (cfg_.precond == DFT_PRECOND_CNN) {
-            own_precond = make_cnn_preconditioner(c_, cfg_.cnn, g_);
-            if (!own_precond) throw std::invalid_argument("learned preconditioner requested without a model");
-        } else {
-            own_precond.reset(new DiagonalPreconditioner());
-        }
-    }
+    if (!ext_precond_) own_precond = make_preconditioner(c_, cfg_.precond, cfg_.cnn, g_);
     Preconditioner* precond = ext_precond_ ? ext_precond_ : own_precond.get();
 
     const bool mixed = cfg_.precision == DFT_SCF_MIXED;
@@ -128,10 +139,21 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         return st;
     };
 
+    int cheb_cold = 1;  // the first Chebyshev solve starts from zero
     auto poisson = [&](Fields& f, const void* rho, void* v) {
         int64_t sw = 0;
         double md = 0;
-        if (cfg_.poisson_fixed > 0) {
+        if (cfg_.poisson_method == DFT_POISSON_CHEBYSHEV) {
+            // warm start from the last V_H; converged to tol every call
+            const double tol = std::max(cfg_.poisson_tol > 0 ? cfg_.poisson_tol : 1.0e-8, f.prec == DFT_F64 ? 0.0 : 1.0e-5);
+            // O(n log(1/tol)) sweeps suffice; the cap only guards against a non-converging setup
+            const int64_t cap = 200 * std::max({g_.nx, g_.ny, g_.nz});
+            check(c_, dft_copy(c_, v, f.vhl.get(), (size_t)n * esize(f.prec)), "dft_copy");
+            check(c_, dft_poisson_chebyshev(c_, &g_, f.prec, rho, v, cap, tol, cheb_cold, batch, &sw, &md),
+                  "dft_poisson_chebyshev");
+            cheb_cold = 0;
+            check(c_, dft_copy(c_, f.vhl.get(), v, (size_t)n * esize(f.prec)), "dft_copy");
+        } else if (cfg_.poisson_fixed > 0) {
             check(c_, dft_copy(c_, v, f.vhl.get(), (size_t)n * esize(f.prec)), "dft_copy");
             check(c_, dft_poisson(c_, &g_, f.prec, rho, v, cfg_.poisson_fixed, 0.0, 0, batch, &sw, &md), "dft_poisson");
             check(c_, dft_copy(c_, f.vhl.get(), v, (size_t)n * esize(f.prec)), "dft_copy");
@@ -257,6 +279,11 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
                          closed ? 0 : sb.iterations);
         bool conv = cfg_.stop_rule == DFT_STOP_DE ? de < etol : (de < kEnergyTol && dd < kDensityTol && ks < kKsTol);
         if (refining) ++refine_steps;
+        // The FP32 phase stops at its own noise floor (FP32 fields carry ~1e-4 Ha
+        // of energy noise and ~1e-6 of relative density noise) and hands over to
+        // the FP64 refinement, which then applies the configured rule.
+        if (mixed && !refining && f.prec == DFT_F32)
+            conv = cfg_.stop_rule == DFT_STOP_DE ? de < std::max(etol, kF32EnergyFloor) : dd < kF32DensityFloor;
         const bool hand_over = conv && mixed && !refining && cfg_.refine_iterations > 0;
         if (conv && !hand_over) {
             rep.converged = 1;

@@ -26,7 +26,11 @@ extern "C" {
 enum { DFT_SCF_FP64 = 0, DFT_SCF_MIXED = 1 };
 enum { DFT_STOP_REFERENCE = 0, DFT_STOP_DE = 1 };
 enum { DFT_INIT_BARE = 0, DFT_INIT_HYDROGENIC = 1 };
-enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1 };
+/* DIAG: the reference's diagonal correction; CNN: the learned correction with
+ * the diagonal one as fallback; KINETIC: 4 damped-Jacobi steps on
+ * (-1/2 Lap + 1) t = r (a smoothing correction, no reference counterpart). */
+enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1, DFT_PRECOND_KINETIC = 2 };
+enum { DFT_POISSON_JACOBI = 0, DFT_POISSON_CHEBYSHEV = 1 };
 
 typedef struct {
     int functional;        /* DFT_XC_PZ81 / DFT_XC_PBE96 */
@@ -38,6 +42,8 @@ typedef struct {
     int max_iterations;    /* SCF steps; 0: 250 (DFTConstants.h:15) */
     int poisson_fixed;     /* 0: reference rule (tol 1e-8, cap 10000, zero start); k > 0: k warm sweeps */
     int poisson_batch;     /* sweeps between host checks; 0: 64 */
+    int poisson_method;    /* DFT_POISSON_JACOBI (the reference's sweeps) or DFT_POISSON_CHEBYSHEV */
+    double poisson_tol;    /* Chebyshev stop: max|dV| per sweep (0: 1e-8 FP64; FP32 uses >= 1e-5) */
     int stop_rule;         /* DFT_STOP_REFERENCE (dE, dRho, KS triple) or DFT_STOP_DE */
     double energy_tol;     /* |dE| threshold for DFT_STOP_DE */
     int init_guess;        /* DFT_INIT_BARE (reference) or DFT_INIT_HYDROGENIC */
@@ -99,12 +105,12 @@ int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_
  * FP64 values.  prec/gram_mode select the arithmetic.  eig: nstates values;
  * psi (optional): nconv orbitals, N each, normalised sum psi^2 dV = 1 like
  * the reference's MolecularOrbital::psi; resid (optional): nstates residuals.
- * cnn: a trained network (dft_cnn_model) for the learned correction, NULL
- * for the diagonal one.  stats (optional, 4 values): H-applied columns,
+ * precond: DFT_PRECOND_DIAG / CNN / KINETIC; cnn: the trained network
+ * (dft_cnn_model) for DFT_PRECOND_CNN.  stats (optional, 4 values): H-applied columns,
  * restarts, learned corrections applied, diagonal fallbacks. */
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
-                     int max_subspace, double tol, int max_iter, const void* cnn, double* eig, double* psi,
-                     double* resid, int* iterations, int64_t* stats);
+                     int max_subspace, double tol, int max_iter, int precond, const void* cnn, double* eig,
+                     double* psi, double* resid, int* iterations, int64_t* stats);
 
 /* ---- learned preconditioner: model, training-data harvest, training ----
  * No reference counterpart (the reference preconditions with the diagonal

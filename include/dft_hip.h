@@ -91,6 +91,13 @@ int dft_kinetic_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* ps
  * or not finite.  eps_dev: nb FP64 values on the device. */
 int dft_precond_diag(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr,
                      int nb, const double* eps_dev, void* t, int64_t ldt);
+/* One damped-Jacobi step of the kinetic smoothing correction (no reference
+ * counterpart; see dft_core.h DFT_PRECOND_KINETIC): first = 1: t = cf r;
+ * otherwise t = t + cf (r - kt), per column. */
+int dft_smooth_step(dft_ctx* ctx, int prec, int64_t n, int nb, double cf, const void* r, int64_t ldr, const void* kt,
+                    int64_t ldk, void* t, int64_t ldt, int first);
+/* x[0..n) = value */
+int dft_fill(dft_ctx* ctx, int prec, int64_t n, double value, void* x);
 
 /* ---- (b) Poisson ---------------------------------------------------------
  * Replaces calculateHartreePotential (HartreePotentialMolecule.cpp:99-324).
@@ -104,6 +111,13 @@ int dft_precond_diag(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff
  * rho/v. */
 int dft_poisson(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, void* v, int64_t max_sweeps, double tol,
                 int zero_start, int sweeps_per_launch, int64_t* sweeps_done, double* last_maxdiff);
+/* The same discrete problem (boundary rule of HartreePotentialMolecule.cpp:37-95,
+ * Jacobi splitting of :295-322) solved by Chebyshev semi-iteration on the
+ * Jacobi sweeps: O(n) sweeps to a given accuracy instead of the reference's
+ * O(n^2).  Not the reference's iterate sequence; its fixed point.  Stops
+ * after the batch (sweeps between host checks) in which max|dV| < tol. */
+int dft_poisson_chebyshev(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, void* v, int64_t max_sweeps,
+                          double tol, int zero_start, int batch, int64_t* sweeps_done, double* last_maxdiff);
 /* Hartree energy 1/2 sum rho V dV (HartreePotentialMolecule.cpp:326-379). */
 int dft_hartree_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, const void* vh, double* e_host);
 
@@ -173,6 +187,10 @@ int dft_transpose(dft_ctx* ctx, int rows, int cols, const double* src, int64_t l
 /* Davidson seed vectors buildInitialVector (EigenvalueSolverMolecule.cpp:177-283)
  * for indices first_index .. first_index+count-1 into out columns. */
 int dft_seed_vectors(dft_ctx* ctx, const dft_grid* g, int prec, int first_index, int count, void* out, int64_t ld);
+/* count columns of deterministic pseudo-random values in [-1, 1) (splitmix64
+ * of seed, column and point): completes a seed block that is numerically
+ * rank-deficient (no reference counterpart). */
+int dft_random_vectors(dft_ctx* ctx, int prec, int64_t n, int count, uint64_t seed, void* out, int64_t ld);
 /* Initial density guess: sum over atoms of Z * hydrogenic 1s density
  * (Z^3/pi) exp(-2 Z r), normalised on the grid to `electrons`. */
 int dft_hydrogenic_density(dft_ctx* ctx, const dft_grid* g, int prec, int natoms, const double* nuclei_host,

@@ -101,6 +101,26 @@ def test_poisson_fixed_warm_f64_and_f32(dev, oracle):
     assert rel(v32.download(), ref) <= 1e-5
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_poisson_chebyshev_reaches_jacobi_fixed_point(dev, oracle, prec):
+    """Chebyshev semi-iteration converges to the fixed point of the reference's
+    Jacobi iteration (same boundary rule and splitting) in far fewer sweeps."""
+    from oracle.oracle import Grid
+    og = Grid(26, 22, 19, -3.9, 3.6, -3.1, 3.2, -2.8, 2.6)
+    xs, ys, zs = og.coords()
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    rho = (np.exp(-1.5 * ((X - 0.2) ** 2 + Y ** 2 + (Z + 0.1) ** 2)) * 0.7).ravel()
+    ref, jsw, jmd = oracle.poisson(og, rho, v0=np.zeros(og.size), max_sweeps=400000, tol=1e-15)
+    assert jmd < 1e-15
+    g = dgrid(og)
+    dt = np.float64 if prec == "f64" else np.float32
+    d_rho, d_v = dev.array(rho, dtype=dt), dev.zeros((og.size,), dtype=dt)
+    sw, md = dev.poisson_chebyshev(g, d_rho, d_v, tol=1e-14 if prec == "f64" else 1e-6, batch=16)
+    got = d_v.download()
+    assert rel(got, ref) <= (1e-11 if prec == "f64" else 2e-6)
+    assert sw < jsw / 5, (sw, jsw)
+
+
 def ulp_sensitivity(oracle, fid, pts):
     """Largest relative change of each output under a 1-ulp change of any input
     (computed with the CPU oracle): the conditioning floor of a pointwise test."""
