@@ -60,19 +60,20 @@ struct GramT<DFT_GRAM_SPLIT_BF16> {
 };
 
 // Workgroup = 4 waves in 2 x 2, each wave 3 x 3 MFMA 16x16 tiles: a 96 x 96
-// output tile over one K-chunk.  Each K-slab (KS rows of the 96 A columns and
-// 96 B columns) is read with 16-byte loads (8 lanes cover a 128-byte column
-// segment) and stored in LDS in operand order; for SPLIT_BF16 the FP32 values
-// are split into hi/lo bf16 on the way in.
+// output tile over one K-chunk.  Only the tile's valid columns (ca of A, cb
+// of B) are read: each K-slab (KS rows) is loaded with 16-byte pieces into
+// registers while the MFMAs of the previous slab run, then stored to LDS in
+// operand order (for SPLIT_BF16 split into hi/lo bf16 on the way in).
+// Waves and MFMA sub-tiles wholly outside the valid columns skip their work.
 template <int MODE>
 struct Slab;
 template <>
 struct Slab<DFT_GRAM_F64> {
-    static constexpr int KS = 16, PAD = 17;  // doubles per column in LDS
+    static constexpr int KS = 32, PAD = 33;  // doubles per column in LDS
 };
 template <>
 struct Slab<DFT_GRAM_F32> {
-    static constexpr int KS = 16, PAD = 17;  // floats per column
+    static constexpr int KS = 32, PAD = 33;  // floats per column
 };
 template <>
 struct Slab<DFT_GRAM_SPLIT_BF16> {
@@ -104,10 +105,12 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm
     const int tile = slot % ntiles;
     const int ks = (slot / ntiles) * 8 + xcd;
     const int i0 = (tile % ntm) * GT, j0 = (tile / ntm) * GT;
+    const int ca = min(GT, ma - i0), cb = min(GT, mb - j0);  // valid columns of this tile
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wi = (w >> 1) * 48, wj = (w & 1) * 48;  // wave's 48 x 48 block inside the tile
     const int r16 = lane & 15, kq = lane >> 4;
     const int64_t kbeg = (int64_t)ks * kchunk, kend = min(n, kbeg + kchunk);
+    const bool wave_live = wi < ca && wj < cb;
 
     Acc acc[3][3];
 #pragma unroll
@@ -115,50 +118,78 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[a][c] = Acc{0, 0, 0, 0};
 
-    // slab loader: 16-byte pieces; EPV elements per piece, PPC pieces per column
+    // invalid columns stay zero for the whole loop
+    for (int q = threadIdx.x; q < (SPLIT ? 4 : 2) * GT * PAD; q += 256) {
+        const int plane = q / (GT * PAD), col = (q % (GT * PAD)) / PAD;
+        const bool isA = (plane & 1) == 0;
+        if (isA ? col >= ca : col >= cb) lds[q] = Ld(0);
+    }
+
+    // slab pieces: 16 bytes = EPV elements; PPC pieces per column; NP pieces per thread
     constexpr int EPV = 16 / sizeof(In);
     constexpr int PPC = KS / EPV;
-    constexpr int COLS_PER_PASS = 256 / PPC;
-    const int lcol = threadIdx.x / PPC, lpiece = threadIdx.x % PPC;
+    constexpr int NP = (2 * GT * PPC + 255) / 256;
+    const int npieces = (ca + cb) * PPC;
+    In reg[NP][EPV];
 
-    for (int64_t k0 = kbeg; k0 < kend; k0 += KS) {
-        // ---- global -> LDS
-        for (int cbase = 0; cbase < 2 * GT; cbase += COLS_PER_PASS) {
-            const int cc = cbase + lcol;  // 0..2*GT-1: A columns then B columns
-            const bool isA = cc < GT;
-            const int col = isA ? i0 + cc : j0 + (cc - GT);
-            const bool cval = isA ? col < ma : col < mb;
-            const In* src = isA ? A + (int64_t)(cval ? col : 0) * lda : B + (int64_t)(cval ? col : 0) * ldb;
-            const int64_t kk = k0 + lpiece * EPV;
-            In v[EPV];
-            if (cval && kk + EPV <= kend && ((((uintptr_t)(src + kk)) & 15) == 0)) {
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int q = threadIdx.x + 256 * p;
+            if (q >= npieces) break;
+            const int cc = q / PPC, lp = q % PPC;
+            const bool isA = cc < ca;
+            const In* src = isA ? A + (int64_t)(i0 + cc) * lda : B + (int64_t)(j0 + cc - ca) * ldb;
+            const int64_t kk = k0 + lp * EPV;
+            if (kk + EPV <= kend && ((((uintptr_t)(src + kk)) & 15) == 0)) {
                 if constexpr (sizeof(In) == 8) {
-                    const double2 q = *reinterpret_cast<const double2*>(src + kk);
-                    v[0] = q.x; v[1] = q.y;
+                    const double2 v = *reinterpret_cast<const double2*>(src + kk);
+                    reg[p][0] = v.x;
+                    reg[p][1] = v.y;
                 } else {
-                    const float4 q = *reinterpret_cast<const float4*>(src + kk);
-                    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+                    const float4 v = *reinterpret_cast<const float4*>(src + kk);
+                    reg[p][0] = v.x;
+                    reg[p][1] = v.y;
+                    reg[p][2] = v.z;
+                    reg[p][3] = v.w;
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < EPV; ++e) v[e] = (cval && kk + e < kend) ? src[kk + e] : In(0);
-            }
-            const int lc = isA ? cc : cc - GT;
-            Ld* dst = (isA ? ldsA : ldsB) + lc * PAD + lpiece * EPV;
-            if constexpr (SPLIT) {
-                Ld* dstl = (isA ? ldsAl : ldsBl) + lc * PAD + lpiece * EPV;
-#pragma unroll
-                for (int e = 0; e < EPV; ++e) {
-                    const __bf16 h = (__bf16)v[e];
-                    dst[e] = h;
-                    dstl[e] = (__bf16)(v[e] - (float)h);
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < EPV; ++e) dst[e] = v[e];
+                for (int e = 0; e < EPV; ++e) reg[p][e] = kk + e < kend ? src[kk + e] : In(0);
             }
         }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int q = threadIdx.x + 256 * p;
+            if (q >= npieces) break;
+            const int cc = q / PPC, lp = q % PPC;
+            const bool isA = cc < ca;
+            const int lc = isA ? cc : cc - ca;
+            Ld* dst = (isA ? ldsA : ldsB) + lc * PAD + lp * EPV;
+            if constexpr (SPLIT) {
+                Ld* dstl = (isA ? ldsAl : ldsBl) + lc * PAD + lp * EPV;
+#pragma unroll
+                for (int e = 0; e < EPV; ++e) {
+                    const __bf16 h = (__bf16)reg[p][e];
+                    dst[e] = h;
+                    dstl[e] = (__bf16)(reg[p][e] - (float)h);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < EPV; ++e) dst[e] = reg[p][e];
+            }
+        }
+    };
+
+    if (kbeg < kend) load(kbeg);
+    for (int64_t k0 = kbeg; k0 < kend; k0 += KS) {
+        __syncthreads();  // the previous slab's MFMA reads are done
+        store();
         __syncthreads();
+        if (k0 + KS < kend) load(k0 + KS);  // in flight during this slab's MFMAs
+        if (!wave_live) continue;
         // ---- MFMA over the slab
         if constexpr (SPLIT) {
             bf8 ah[3], al[3], bh[3], bl[3];
@@ -175,13 +206,16 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm
                 bl[c] = *reinterpret_cast<const bf8*>(ldsBl + col * PAD + 8 * kq);
             }
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a) {
+                if (wi + 16 * a >= ca) continue;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
+                    if (wj + 16 * c >= cb) continue;
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
                 }
+            }
         } else {
 #pragma unroll
             for (int k4 = 0; k4 < KS; k4 += 4) {
@@ -191,17 +225,19 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm
 #pragma unroll
                 for (int c = 0; c < 3; ++c) xb[c] = ldsB[(wj + 16 * c + r16) * PAD + k4 + kq];
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
+                for (int a = 0; a < 3; ++a) {
+                    if (wi + 16 * a >= ca) continue;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
+                        if (wj + 16 * c >= cb) continue;
                         if constexpr (MODE == DFT_GRAM_F64)
                             acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a], xb[c], acc[a][c], 0, 0, 0);
                         else
                             acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[a], xb[c], acc[a][c], 0, 0, 0);
                     }
+                }
             }
         }
-        __syncthreads();
     }
     // partial tile -> part[ks][j*ma + i] (K-chunks past the end write zeros)
     double* P = part + (int64_t)ks * ma * mb;

@@ -29,6 +29,65 @@ __global__ __launch_bounds__(256) void k_copy_sq(int m, const double* __restrict
     const int i = (int)(e % m), j = (int)(e / m);
     t[(int64_t)j * m + i] = a[(int64_t)j * lda + i];
 }
+// ---- tall-skinny block update on the VALU --------------------------------
+// Y = alpha * A C + beta * Y with A: N x m (N grid points), C: m x k small.
+// Each thread owns two grid points and KMAX output columns in registers; the
+// m x k coefficients are read with wave-uniform addresses (scalar loads), so
+// every loaded A value feeds KMAX FMAs with no LDS traffic.  Columns beyond
+// KMAX run in further passes over A.  Used for FP32 blocks with k <= 64,
+// where rocBLAS' SGEMM tiles waste most of their width.
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void k_update_valu(int64_t n, int m, int kc, T alpha, const T* __restrict__ A,
+                                                     int64_t lda, const T* __restrict__ Cm, int ldc, T beta,
+                                                     T* __restrict__ Y, int64_t ldy) {
+    const int64_t p0 = (int64_t)blockIdx.x * 512 + threadIdx.x, p1 = p0 + 256;
+    const bool ok0 = p0 < n, ok1 = p1 < n;
+    T acc0[KMAX], acc1[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        acc0[j] = T(0);
+        acc1[j] = T(0);
+    }
+    for (int i = 0; i < m; ++i) {
+        const T* a = A + (int64_t)i * lda;
+        const T v0 = ok0 ? a[p0] : T(0);
+        const T v1 = ok1 ? a[p1] : T(0);
+        const T* crow = Cm + i;  // C[i][j] at crow[j * ldc]
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            const T cj = j < kc ? crow[(int64_t)j * ldc] : T(0);
+            acc0[j] = fma(v0, cj, acc0[j]);
+            acc1[j] = fma(v1, cj, acc1[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        if (j >= kc) break;
+        T* y = Y + (int64_t)j * ldy;
+        if (ok0) y[p0] = beta == T(0) ? alpha * acc0[j] : alpha * acc0[j] + beta * y[p0];
+        if (ok1) y[p1] = beta == T(0) ? alpha * acc1[j] : alpha * acc1[j] + beta * y[p1];
+    }
+}
+
+template <typename T>
+int update_valu(dft_ctx* c, int64_t n, int m, int k, T alpha, const T* A, int64_t lda, const T* Cm, int ldc, T beta, T* Y,
+                int64_t ldy) {
+    const unsigned blocks = (unsigned)((n + 511) / 512);
+    for (int j0 = 0; j0 < k; j0 += 64) {
+        const int kc = std::min(64, k - j0);
+        const T* C0 = Cm + (int64_t)j0 * ldc;
+        T* Y0 = Y + (int64_t)j0 * ldy;
+        if (kc <= 16)
+            k_update_valu<T, 16><<<blocks, 256, 0, c->stream>>>(n, m, kc, alpha, A, lda, C0, ldc, beta, Y0, ldy);
+        else if (kc <= 32)
+            k_update_valu<T, 32><<<blocks, 256, 0, c->stream>>>(n, m, kc, alpha, A, lda, C0, ldc, beta, Y0, ldy);
+        else
+            k_update_valu<T, 64><<<blocks, 256, 0, c->stream>>>(n, m, kc, alpha, A, lda, C0, ldc, beta, Y0, ldy);
+        DFT_CHECK_LAUNCH(c);
+    }
+    return DFT_OK;
+}
+
 }  // namespace
 
 namespace dft {
@@ -57,6 +116,7 @@ static int blas(dft_ctx* c, rocblas_handle* h) {
 
 extern "C" {
 
+
 // Block update Y = alpha A C + beta Y (a plain tall-skinny GEMM, N x m times
 // m x k): rocBLAS' 64-bit-index GEMM on the context stream; FP32 blocks use
 // an FP32 copy of the small coefficient matrix.
@@ -77,6 +137,8 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
         k_to_f32<<<(unsigned)(((int64_t)m * k + 255) / 256), 256, 0, c->stream>>>(m, k, cm, ldc, c32);
         DFT_CHECK_LAUNCH(c);
         const float al = (float)alpha, be = (float)beta;
+        if (k <= 64 && m <= 512)
+            return update_valu<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
         st = rocblas_sgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &al, (const float*)a, lda, c32,
                               m, &be, (float*)y, ldy);
     } else {

@@ -115,6 +115,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     dop.tol = cfg_.davidson_tol > 0 ? cfg_.davidson_tol : 1.0e-7;
     dop.max_iter = cfg_.davidson_max_iter > 0 ? cfg_.davidson_max_iter : 200;
     bool have_prev = false;
+    const double etol = cfg_.energy_tol > 0 ? cfg_.energy_tol : 1.0e-5;
 
     // Under the |dE| stop rule the orbitals need only be as accurate as the
     // density is settled: tol = clamp(0.1 dRho_prev, floor, 1e-3), the floor
@@ -130,7 +131,12 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             o.tol = std::max(o.tol, 3.0e-4);
             o.max_iter = std::min(o.max_iter, 60);
         }
-        if (cfg_.stop_rule == DFT_STOP_DE) o.tol = std::max(o.tol, std::min(1.0e-3, 0.1 * dd_prev));
+        // |dE| < etol only means convergence if the orbitals are converged
+        // well beyond it: energy errors go with the square of the residual,
+        // so the tolerance never exceeds 0.3 sqrt(etol) (else a loose solve
+        // that leaves the orbitals unchanged gives dE = 0 by itself)
+        if (cfg_.stop_rule == DFT_STOP_DE)
+            o.tol = std::max(o.tol, std::min({1.0e-3, 0.1 * dd_prev, 0.3 * std::sqrt(etol)}));
         DavidsonStats st = D.solve(veff, have_prev, o, *precond);
         rep.happly_columns += st.happly_columns;
         rep.davidson_iterations += st.iterations;
@@ -201,7 +207,6 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     double prevE = std::numeric_limits<double>::infinity(), prevDD = prevE, mixing = kMixing0;
     bool refining = false;
     int refine_steps = 0;
-    const double etol = cfg_.energy_tol > 0 ? cfg_.energy_tol : 1.0e-5;
     for (int it = 1; it <= maxit + (mixed ? std::max(cfg_.refine_iterations, 0) : 0); ++it) {
         if (!refining && it > maxit) break;
         Fields& f = *F;

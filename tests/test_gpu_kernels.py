@@ -242,7 +242,7 @@ def test_gram_modes(dev, n, ma, mb):
         assert rel(got, ref32) <= tol
 
 
-@pytest.mark.parametrize("n,m,k", [(1000, 8, 8), (50001, 37, 70), (200000, 192, 96)])
+@pytest.mark.parametrize("n,m,k", [(1000, 8, 8), (50001, 37, 70), (200000, 192, 96), (70001, 29, 13)])
 def test_block_update(dev, n, m, k):
     rng = np.random.default_rng(m)
     A = rng.standard_normal((m, n))
@@ -255,6 +255,10 @@ def test_block_update(dev, n, m, k):
     dy32 = dev.array(Y0, dtype=np.float32)
     dev.block_update(dev.array(A, dtype=np.float32), dev.array(Cm), dy32, n, m, k, alpha=0.5, beta=2.0)
     assert rel(dy32.download(), ref) <= 1e-5
+    # beta = 0 never reads Y (NaN there must not leak)
+    dyn = dev.array(np.full((k, n), np.nan), dtype=np.float32)
+    dev.block_update(dev.array(A, dtype=np.float32), dev.array(Cm), dyn, n, m, k, alpha=0.5, beta=0.0)
+    assert rel(dyn.download(), 0.5 * (Cm @ A)) <= 1e-5
 
 
 def test_syev_and_cholinv(dev):
