@@ -67,6 +67,14 @@ def build(verbose=False, jobs=8):
               "-Wl,-rpath,$ORIGIN", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
               "-Wl,-rpath," + os.path.join(ROCM, "lib")])
         out.append(core_so)
+        # test programs written against the reference-shaped C++ API
+        tdir = os.path.join(ROOT, "tests", "cpp")
+        for src in sorted(glob.glob(os.path.join(tdir, "*.cpp"))):
+            os.makedirs(os.path.join(tdir, "bin"), exist_ok=True)
+            exe = os.path.join(tdir, "bin", os.path.splitext(os.path.basename(src))[0])
+            _run(["g++", "-std=c++17", "-O2", src, "-I", os.path.join(ROOT, "include"), "-L", LIB, "-ldft_core",
+                  "-ldft_hip", "-Wl,-rpath,$ORIGIN/../../../dft_amd/lib", "-o", exe])
+            out.append(exe)
     if verbose:
         print("built", *out)
     return out
