@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--precond", default="kinetic", choices=["diag", "kinetic", "cnn"],
                     help="Davidson correction of the timed (mixed) run; the FP64 run uses the diagonal one")
     ap.add_argument("--cnn-model", default=None, help="trained network (.npz) for --precond cnn")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="only the timed mixed SCF (profiling): no FP64 runs, no CPU baseline")
     ap.add_argument("--profile-scf", type=int, default=0,
                     help="only run this many SCF steps of each precision (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -345,6 +347,13 @@ def main():
                         max_iterations=2, poisson_fixed=50, stop_rule=dft_amd.STOP_DE,
                         energy_tol=1e-5, init_guess=w["init"], refine_iterations=1)
         print(f"[bench] warmup {i + 1} done", file=sys.stderr, flush=True)
+    if a.mixed_only:
+        t = time.perf_counter()
+        r = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_MIXED, verbose=a.verbose, precond=pc, cnn=model)
+        print(json.dumps({"mixed_seconds": time.perf_counter() - t, "energy": r["total"], "steps": r["iterations"],
+                          "refine_steps": r["refine_iterations"], "davidson_iterations": r["davidson_iterations"],
+                          "phase": {k: r[f"seconds_{k}"] for k in ("poisson", "xc", "davidson", "other")}}))
+        return
     # plain FP64 GPU path (the accuracy reference and the speedup denominator)
     t = time.perf_counter()
     r64 = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, verbose=a.verbose)

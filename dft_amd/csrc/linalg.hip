@@ -29,6 +29,115 @@ __global__ __launch_bounds__(256) void k_copy_sq(int m, const double* __restrict
     const int i = (int)(e % m), j = (int)(e / m);
     t[(int64_t)j * m + i] = a[(int64_t)j * lda + i];
 }
+// ---- tall-skinny block update on the matrix cores -------------------------
+// Y^T (k x N) = alpha C^T (k x m) A^T (m x N) + beta Y^T, as 16 x 16 MFMA
+// tiles: operand A = C^T from LDS (lane: row j = l % 16, K slice i = l / 16),
+// operand B = A^T straight from HBM (lane: column = grid point l % 16, K
+// slice i = l / 16, so a quarter-wave reads 16 consecutive points of one
+// column), result rows = output columns j, result column = grid point.  A
+// wave owns 16 grid points and all k <= 96 output columns (6 accumulator
+// tiles) and strides over the points; C is staged once per workgroup.
+template <typename T>
+struct UpdMfma;
+template <>
+struct UpdMfma<double> {
+    typedef double Acc __attribute__((ext_vector_type(4)));
+    static __device__ inline Acc mfma(double a, double b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int q, int r) { return q + 4 * r; }
+};
+template <>
+struct UpdMfma<float> {
+    typedef float Acc __attribute__((ext_vector_type(4)));
+    static __device__ inline Acc mfma(float a, float b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int q, int r) { return 4 * q + r; }
+};
+
+constexpr int kUpdLdsBytes = 48 * 1024;
+
+template <typename T, int NJT>
+__global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m, int k, T alpha, const T* __restrict__ A,
+                                                     int64_t lda, const T* __restrict__ Cm, int ldc, T beta,
+                                                     T* __restrict__ Y, int64_t ldy) {
+    using U = UpdMfma<T>;
+    using Acc = typename U::Acc;
+    extern __shared__ __attribute__((aligned(16))) unsigned char upd_lds[];
+    T* ct = reinterpret_cast<T*>(upd_lds);  // C^T padded to NJT*16 rows: ct[i * (NJT*16) + j]
+    constexpr int KP = NJT * 16;
+    const int mp = (m + 3) / 4 * 4;
+    for (int q = threadIdx.x; q < mp * KP; q += 256) {
+        const int i = q / KP, j = q % KP;
+        ct[q] = (i < m && j < k) ? Cm[(int64_t)j * ldc + i] : T(0);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, qd = lane >> 4;
+    const int64_t ntile = (n + 15) / 16;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < ntile; t += (int64_t)gridDim.x * 4) {
+        const int64_t p = t * 16 + r16;
+        const bool pin = p < n;
+        Acc acc[NJT];
+#pragma unroll
+        for (int jt = 0; jt < NJT; ++jt) acc[jt] = Acc{0, 0, 0, 0};
+        for (int i4 = 0; i4 < mp; i4 += 32) {
+            T b[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int i = i4 + 4 * s + qd;
+                b[s] = (pin && i < m) ? A[(int64_t)i * lda + p] : T(0);
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int ib = i4 + 4 * s;
+                if (ib >= mp) break;
+#pragma unroll
+                for (int jt = 0; jt < NJT; ++jt) {
+                    const T a = ct[(ib + qd) * KP + jt * 16 + r16];
+                    acc[jt] = U::mfma(a, b[s], acc[jt]);
+                }
+            }
+        }
+        if (!pin) continue;
+#pragma unroll
+        for (int jt = 0; jt < NJT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = jt * 16 + U::row(qd, r);
+                if (j >= k) continue;
+                T* y = Y + (int64_t)j * ldy + p;
+                *y = beta == T(0) ? alpha * acc[jt][r] : alpha * acc[jt][r] + beta * *y;
+            }
+    }
+}
+
+// true when the MFMA path applies (k <= 96 and C^T fits the LDS budget)
+template <typename T>
+bool update_mfma_fits(int m, int k) {
+    const int njt = (k + 15) / 16;
+    return k >= 1 && k <= 96 && (int64_t)((m + 3) / 4 * 4) * njt * 16 * (int64_t)sizeof(T) <= kUpdLdsBytes;
+}
+
+template <typename T>
+int update_mfma(dft_ctx* c, int64_t n, int m, int k, T alpha, const T* A, int64_t lda, const T* Cm, int ldc, T beta, T* Y,
+                int64_t ldy) {
+    const int njt = (k + 15) / 16;
+    const size_t lds = (size_t)((m + 3) / 4 * 4) * njt * 16 * sizeof(T);
+    const int64_t ntile = (n + 15) / 16;
+    const unsigned blocks = (unsigned)std::min<int64_t>((ntile + 3) / 4, 8LL * c->num_cus);
+#define DFT_UPD(NJ)                                                                                                   \
+    k_update_mfma<T, NJ><<<blocks, 256, lds, c->stream>>>(n, m, k, alpha, A, lda, Cm, ldc, beta, Y, ldy)
+    switch (njt) {
+        case 1: DFT_UPD(1); break;
+        case 2: DFT_UPD(2); break;
+        case 3: DFT_UPD(3); break;
+        case 4: DFT_UPD(4); break;
+        case 5: DFT_UPD(5); break;
+        default: DFT_UPD(6); break;
+    }
+#undef DFT_UPD
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
 // ---- tall-skinny block update on the VALU --------------------------------
 // Y = alpha * A C + beta * Y with A: N x m (N grid points), C: m x k small.
 // Each thread owns two grid points and KMAX output columns in registers; the
@@ -128,7 +237,9 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
     rocblas_handle h;
     if (int rc = blas(c, &h)) return rc;
     rocblas_status st;
-    if (prec == DFT_F64) {
+    if (prec == DFT_F64 && update_mfma_fits<double>(m, k)) {
+        return update_mfma<double>(c, n, m, k, alpha, (const double*)a, lda, cm, (int)ldc, beta, (double*)y, ldy);
+    } else if (prec == DFT_F64) {
         st = rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &alpha, (const double*)a, lda,
                               cm, ldc, &beta, (double*)y, ldy);
     } else if (prec == DFT_F32) {
@@ -137,6 +248,8 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
         k_to_f32<<<(unsigned)(((int64_t)m * k + 255) / 256), 256, 0, c->stream>>>(m, k, cm, ldc, c32);
         DFT_CHECK_LAUNCH(c);
         const float al = (float)alpha, be = (float)beta;
+        if (update_mfma_fits<float>(m, k))
+            return update_mfma<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
         if (k <= 64 && m <= 512)
             return update_valu<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
         st = rocblas_sgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &al, (const float*)a, lda, c32,
