@@ -289,6 +289,13 @@ class Device:
         self._chk(self.L.dft_seed_vectors(self.ctx, C.byref(g), C.c_int(out.prec), C.c_int(first), C.c_int(count),
                                           _vp(out.ptr), C.c_int64(g.size)))
 
+    def precond_kinetic(self, g, r, t, nb, sigma=1.0, omega=2.0 / 3.0, sweeps=4):
+        tmp = DeviceArray(self, (nb, g.size), r.dtype)
+        self._chk(self.L.dft_precond_kinetic(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma), C.c_double(omega),
+                                             C.c_int(sweeps), _vp(r.ptr), C.c_int64(g.size), C.c_int(nb), _vp(t.ptr),
+                                             C.c_int64(g.size), _vp(tmp.ptr)))
+        tmp.free()
+
     def precond_diag(self, g, veff, r, eps, t, nb):
         self._chk(self.L.dft_precond_diag(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
                                           C.c_int64(g.size), C.c_int(nb), _vp(eps.ptr), _vp(t.ptr), C.c_int64(g.size)))

@@ -45,10 +45,14 @@ struct Coef {
     T cx, cy, cz, cc;  // 1/dx^2, 1/dy^2, 1/dz^2, -2(1/dx^2+1/dy^2+1/dz^2)
 };
 
-template <typename T, int TY, int NB>
+// SMOOTH: instead of H psi, one damped-Jacobi step of the kinetic smoothing
+// correction on psi = t: out = t + cf (r - (-1/2 Lap t + sigma t)), with r in
+// `R` (same layout as psi) and no V_eff read.
+template <typename T, int TY, int NB, bool SMOOTH = false>
 __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, int nzc, Coef<T> co,
                                               const T* __restrict__ V, const T* __restrict__ psi, int64_t ld,
-                                              T* __restrict__ out, int64_t ldo) {
+                                              T* __restrict__ out, int64_t ldo, const T* __restrict__ R = nullptr,
+                                              int64_t ldr = 0, T sigma = T(0), T cf = T(0)) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t tile = xcd_tile(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z),
                                   (int64_t)gridDim.x * gridDim.y * gridDim.z);
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
         const bool has_next = k + 1 < nz;
         T v[TY];
 #pragma unroll
-        for (int r = 0; r < TY; ++r) v[r] = (xin && yin[r]) ? V[kp + roff[r]] : T(0);
+        for (int r = 0; r < TY; ++r) v[r] = (!SMOOTH && xin && yin[r]) ? V[kp + roff[r]] : sigma;
 #pragma unroll
         for (int o = 0; o < NB; ++o) {
             const T* p = P + (int64_t)o * ld;
@@ -126,7 +130,14 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
                 lap += nxt[o][r] * co.cz;
                 const T kin = T(-0.5) * lap;
                 const T pot = v[r] * c;
-                if (xin && yin[r]) q[kp + roff[r]] = kin + pot;
+                if (xin && yin[r]) {
+                    if constexpr (SMOOTH) {
+                        const T rv = R[(int64_t)(grp * NB + o) * ldr + kp + roff[r]];
+                        q[kp + roff[r]] = c + cf * (rv - (kin + pot));
+                    } else {
+                        q[kp + roff[r]] = kin + pot;
+                    }
+                }
             }
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
@@ -137,9 +148,9 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
     }
 }
 
-template <typename T, int TY, int NB>
+template <typename T, int TY, int NB, bool SMOOTH = false>
 int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int ngroups, T* out,
-                int64_t ldo) {
+                int64_t ldo, const T* R = nullptr, int64_t ldr = 0, T sigma = T(0), T cf = T(0)) {
     const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
     Coef<T> co;
     const double ix2 = 1.0 / (g->dx * g->dx), iy2 = 1.0 / (g->dy * g->dy), iz2 = 1.0 / (g->dz * g->dz);
@@ -155,7 +166,7 @@ int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
     nzc = (nz + zc - 1) / zc;
     DFT_REQUIRE(c, (int64_t)nzc * ngroups <= 65535 && gy <= 65535, "dft_hpsi: launch grid too large");
     dim3 grid(gx, gy, nzc * ngroups);
-    k_hpsi<T, TY, NB><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo);
+    k_hpsi<T, TY, NB, SMOOTH><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo, R, ldr, sigma, cf);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }
@@ -182,6 +193,39 @@ int hpsi_t(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, 
     if constexpr (sizeof(T) == 4)
         if (nb >= 64) return hpsi_groups<T, 4>(c, g, V, psi, ld, nb, out, ldo);
     return hpsi_groups<T, 2>(c, g, V, psi, ld, nb, out, ldo);
+}
+
+// Kinetic smoothing correction: t = r / D, then `sweeps` damped-Jacobi steps
+// on (-1/2 Lap + sigma) t = r, ping-ponging between t and tmp (an even
+// number of steps ends in t; an odd one is copied back).
+template <typename T>
+int kinetic_t(dft_ctx* c, const dft_grid* g, double sigma, double omega, int sweeps, const T* r, int64_t ldr, int nb,
+              T* t, int64_t ldt, T* tmp) {
+    const int64_t n = grid_points(g);
+    const double D = 1.0 / (g->dx * g->dx) + 1.0 / (g->dy * g->dy) + 1.0 / (g->dz * g->dz) + sigma;
+    if (int rc = dft_smooth_step(c, sizeof(T) == 8 ? DFT_F64 : DFT_F32, n, nb, 1.0 / D, r, ldr, nullptr, n, t, ldt, 1))
+        return rc;
+    T* src = t;
+    T* dst = tmp;
+    int64_t lds = ldt, ldd = n;
+    for (int s = 0; s < sweeps; ++s) {
+        constexpr int TY = 4, NB = 2;
+        const int ng = nb / NB;
+        if (ng > 0)
+            if (int rc = launch_hpsi<T, TY, NB, true>(c, g, nullptr, src, lds, ng, dst, ldd, r, ldr, (T)sigma, (T)(omega / D)))
+                return rc;
+        for (int o = ng * NB; o < nb; ++o)
+            if (int rc = launch_hpsi<T, TY, 1, true>(c, g, nullptr, src + (int64_t)o * lds, lds, 1, dst + (int64_t)o * ldd, ldd,
+                                                     r + (int64_t)o * ldr, ldr, (T)sigma, (T)(omega / D)))
+                return rc;
+        std::swap(src, dst);
+        std::swap(lds, ldd);
+    }
+    if (src != t)
+        for (int o = 0; o < nb; ++o)
+            DFT_HIP(c, hipMemcpyAsync(t + (int64_t)o * ldt, src + (int64_t)o * lds, sizeof(T) * n, hipMemcpyDeviceToDevice,
+                                      c->stream));
+    return DFT_OK;
 }
 
 // Kinetic energy with the reference's face rule (TotalEnergyMolecule.cpp:80-186):
@@ -268,6 +312,21 @@ int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi,
     if (int rc = finish_sum(c, blocks, &s)) return rc;
     *t_host = s * (g->dx * g->dy * g->dz);
     return DFT_OK;
+}
+
+int dft_precond_kinetic(dft_ctx* c, const dft_grid* g, int prec, double sigma, double omega, int sweeps, const void* r,
+                        int64_t ldr, int nb, void* t, int64_t ldt, void* tmp) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, valid_grid(g) && r && t && tmp && nb >= 0 && nb <= 65535 && sweeps >= 0 && sigma > 0.0 &&
+                       ldr >= grid_points(g) && ldt >= grid_points(g),
+                "dft_precond_kinetic: bad arguments");
+    DFT_REQUIRE(c, g->nx * g->ny * g->nz < (1LL << 31), "dft_precond_kinetic: grid exceeds 2^31 points");
+    if (!nb) return DFT_OK;
+    if (prec == DFT_F64)
+        return kinetic_t<double>(c, g, sigma, omega, sweeps, (const double*)r, ldr, nb, (double*)t, ldt, (double*)tmp);
+    if (prec == DFT_F32)
+        return kinetic_t<float>(c, g, sigma, omega, sweeps, (const float*)r, ldr, nb, (float*)t, ldt, (float*)tmp);
+    return set_err(c, DFT_EINVAL, "dft_precond_kinetic: bad precision");
 }
 
 int dft_precond_diag(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr, int nb,

@@ -152,23 +152,13 @@ int KineticPreconditioner::apply(dft_ctx* c, const dft_grid& g, int prec, const 
                                  const double* /*theta_dev*/, void* T, int64_t ldt) {
     if (ncols <= 0) return 0;
     const int64_t n = g.nx * g.ny * g.nz;
-    const size_t es = esize(prec);
-    if (sig_prec_ != prec) {
-        sig_ = Buf(c, n * es);
-        check(c, dft_fill(c, prec, n, sigma_, sig_.get()), "dft_fill");
-        sig_prec_ = prec;
-        kt_cols_ = 0;
-    }
-    if (kt_cols_ < ncols) {
-        kt_ = Buf(c, (size_t)n * ncols * es);
+    if (kt_cols_ < ncols || sig_prec_ != prec) {
+        kt_ = Buf(c, (size_t)n * ncols * esize(prec));
         kt_cols_ = ncols;
+        sig_prec_ = prec;
     }
-    const double D = 1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy) + 1.0 / (g.dz * g.dz) + sigma_;
-    check(c, dft_smooth_step(c, prec, n, ncols, 1.0 / D, R, ldr, nullptr, n, T, ldt, 1), "dft_smooth_step");
-    for (int s = 0; s < sweeps_; ++s) {
-        check(c, dft_hpsi(c, &g, prec, sig_.get(), T, ldt, ncols, kt_.get(), n), "dft_hpsi");
-        check(c, dft_smooth_step(c, prec, n, ncols, omega_ / D, R, ldr, kt_.get(), n, T, ldt, 0), "dft_smooth_step");
-    }
+    check(c, dft_precond_kinetic(c, &g, prec, sigma_, omega_, sweeps_, R, ldr, ncols, T, ldt, kt_.get()),
+          "dft_precond_kinetic");
     return 0;
 }
 

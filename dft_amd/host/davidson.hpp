@@ -67,7 +67,7 @@ class KineticPreconditioner : public Preconditioner {
    private:
     int sweeps_;
     double sigma_, omega_;
-    Buf kt_, sig_;
+    Buf kt_;  // scratch columns for the ping-pong steps
     int64_t kt_cols_ = 0;
     int sig_prec_ = -1;
 };

@@ -91,6 +91,12 @@ int dft_kinetic_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* ps
  * or not finite.  eps_dev: nb FP64 values on the device. */
 int dft_precond_diag(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr,
                      int nb, const double* eps_dev, void* t, int64_t ldt);
+/* Kinetic smoothing correction (no reference counterpart; DFT_PRECOND_KINETIC
+ * in dft_core.h): t = r / D, D = sum(1/h^2) + sigma, then `sweeps` damped
+ * Jacobi steps t <- t + (omega / D)(r - (-1/2 Lap + sigma) t), each a fused
+ * stencil pass.  tmp: nb columns of N (leading dimension N) of scratch. */
+int dft_precond_kinetic(dft_ctx* ctx, const dft_grid* g, int prec, double sigma, double omega, int sweeps,
+                        const void* r, int64_t ldr, int nb, void* t, int64_t ldt, void* tmp);
 /* One damped-Jacobi step of the kinetic smoothing correction (no reference
  * counterpart; see dft_core.h DFT_PRECOND_KINETIC): first = 1: t = cf r;
  * otherwise t = t + cf (r - kt), per column. */

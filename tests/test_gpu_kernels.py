@@ -121,6 +121,29 @@ def test_poisson_chebyshev_reaches_jacobi_fixed_point(dev, oracle, prec):
     assert sw < jsw / 5, (sw, jsw)
 
 
+@pytest.mark.parametrize("sweeps,nb", [(4, 5), (3, 2)])
+def test_precond_kinetic_steps(dev, oracle, sweeps, nb):
+    """The fused kinetic smoothing steps equal the same damped-Jacobi
+    recurrence written with the oracle's H psi (zero ghosts) on V = sigma."""
+    from oracle.oracle import Grid
+    og = Grid(23, 19, 17, -2.9, 3.1, -2.2, 2.6, -2.0, 2.1)
+    g = dgrid(og)
+    rng = np.random.default_rng(sweeps)
+    R = rng.standard_normal((nb, og.size))
+    sigma, omega = 1.0, 2.0 / 3.0
+    D = 1 / og.dx ** 2 + 1 / og.dy ** 2 + 1 / og.dz ** 2 + sigma
+    vs = np.full(og.size, sigma)
+    ref = R / D
+    for _ in range(sweeps):
+        ref = np.stack([ref[o] + omega / D * (R[o] - oracle.hpsi(og, vs, ref[o])) for o in range(nb)])
+    t = dev.array(shape=(nb, og.size))
+    dev.precond_kinetic(g, dev.array(R), t, nb, sigma, omega, sweeps)
+    assert rel(t.download(), ref) <= 1e-13
+    t32 = dev.array(shape=(nb, og.size), dtype=np.float32)
+    dev.precond_kinetic(g, dev.array(R, dtype=np.float32), t32, nb, sigma, omega, sweeps)
+    assert rel(t32.download(), ref) <= 1e-5
+
+
 def ulp_sensitivity(oracle, fid, pts):
     """Largest relative change of each output under a 1-ulp change of any input
     (computed with the CPU oracle): the conditioning floor of a pointwise test."""
