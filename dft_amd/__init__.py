@@ -277,6 +277,12 @@ class Device:
                                           C.c_double(alpha), _vp(a.ptr), C.c_int64(lda or n), _vp(cmat.ptr),
                                           C.c_int64(ldc or m), C.c_double(beta), _vp(y.ptr), C.c_int64(ldy or n)))
 
+    def ritz_residual(self, v, w, q, theta, r, rn2, n, m, k, ldq=None):
+        """r[:, j] = w q[:, j] - theta[j] v q[:, j] and rn2[j] = ||r[:, j]||^2, j < k."""
+        self._chk(self.L.dft_ritz_residual(self.ctx, C.c_int(v.prec), C.c_int64(n), C.c_int(m), C.c_int(k), _vp(v.ptr),
+                                           C.c_int64(n), _vp(w.ptr), C.c_int64(n), _vp(q.ptr), C.c_int64(ldq or m),
+                                           _vp(theta.ptr), _vp(r.ptr), C.c_int64(n), _vp(rn2.ptr)))
+
     def syev(self, a, m):
         w = self.array(shape=(m,))
         self._chk(self.L.dft_syev(self.ctx, C.c_int(m), _vp(a.ptr), C.c_int64(m), _vp(w.ptr)))

@@ -26,6 +26,10 @@ struct dft_ctx {
     void* scratch = nullptr;
     size_t scratch_cap = 0;
     int num_cus = 256;
+    // set when the runtime refused a cooperative launch (the device's
+    // cooperative queue can be held by another process); Jacobi then runs
+    // one launch per sweep
+    bool coop_refused = false;
     // rocBLAS handle for the dense FP64 subspace solvers (created on first use)
     void* blas = nullptr;
     int* dev_info = nullptr;

@@ -30,13 +30,25 @@ __global__ __launch_bounds__(256) void k_copy_sq(int m, const double* __restrict
     t[(int64_t)j * m + i] = a[(int64_t)j * lda + i];
 }
 // ---- tall-skinny block update on the matrix cores -------------------------
-// Y^T (k x N) = alpha C^T (k x m) A^T (m x N) + beta Y^T, as 16 x 16 MFMA
-// tiles: operand A = C^T from LDS (lane: row j = l % 16, K slice i = l / 16),
-// operand B = A^T straight from HBM (lane: column = grid point l % 16, K
-// slice i = l / 16, so a quarter-wave reads 16 consecutive points of one
-// column), result rows = output columns j, result column = grid point.  A
-// wave owns 16 grid points and all k <= 96 output columns (6 accumulator
-// tiles) and strides over the points; C is staged once per workgroup.
+// Y^T (k x N) = alpha C^T (k x m) [A | A2]^T (m x N) + beta Y^T, as 16 x 16
+// MFMA tiles: operand A = C^T from LDS (lane: row j = l % 16, K slice
+// i = l / 16), operand B = the grid columns straight from HBM, result rows =
+// output columns j, result column = grid point.  Inner rows i < m1 come from
+// A, rows m1 <= i < m from A2 (m1 = m: A alone), so two column blocks in
+// separate buffers enter one pass.  A wave owns 16 VEC grid points and all
+// k <= 96 output columns (6 accumulator tiles) and strides over the points;
+// C is staged once per workgroup.
+//
+// VEC = 1: lane l' = l % 16 owns point p0 + l'.  VEC = 16 / sizeof(T): lane
+// l' owns points p0 + VEC l' ... + VEC - 1, and MFMA c of a K step takes the
+// points p0 + VEC l' + c as its columns; every lane then loads and stores its
+// points as one 16-byte vector (a quarter-wave moves 256 contiguous bytes of
+// a column) and one LDS read of C^T feeds VEC MFMAs.  The vector form needs
+// n, lda, lda2, ldy multiples of VEC and 16-byte aligned A, A2, Y.
+//
+// NORM: part[j * gridDim.x + blockIdx.x] receives the workgroup's FP64 sum of
+// squares of the stored column j of Y (the residual norms of the Davidson
+// iteration come out of the same pass).
 template <typename T>
 struct UpdMfma;
 template <>
@@ -52,14 +64,21 @@ struct UpdMfma<float> {
     static __device__ inline int row(int q, int r) { return 4 * q + r; }
 };
 
+template <typename T, int VEC>
+struct alignas(sizeof(T) * VEC) Pack {
+    T v[VEC];
+};
+
 constexpr int kUpdLdsBytes = 48 * 1024;
 
-template <typename T, int NJT>
-__global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m, int k, T alpha, const T* __restrict__ A,
-                                                     int64_t lda, const T* __restrict__ Cm, int ldc, T beta,
-                                                     T* __restrict__ Y, int64_t ldy) {
+template <typename T, int NJT, int VEC, bool NORM>
+__global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m1, int m, int k, T alpha, const T* __restrict__ A,
+                                                     int64_t lda, const T* __restrict__ A2, int64_t lda2,
+                                                     const T* __restrict__ Cm, int ldc, T beta, T* __restrict__ Y,
+                                                     int64_t ldy, double* __restrict__ part) {
     using U = UpdMfma<T>;
     using Acc = typename U::Acc;
+    using P = Pack<T, VEC>;
     extern __shared__ __attribute__((aligned(16))) unsigned char upd_lds[];
     T* ct = reinterpret_cast<T*>(upd_lds);  // C^T padded to NJT*16 rows: ct[i * (NJT*16) + j]
     constexpr int KP = NJT * 16;
@@ -71,19 +90,33 @@ __global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m, int k, T 
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r16 = lane & 15, qd = lane >> 4;
-    const int64_t ntile = (n + 15) / 16;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < ntile; t += (int64_t)gridDim.x * 4) {
-        const int64_t p = t * 16 + r16;
-        const bool pin = p < n;
-        Acc acc[NJT];
+    constexpr int TP = 16 * VEC;
+    const int64_t ntile = (n + TP - 1) / TP;
+    double s2[NJT][4];
 #pragma unroll
-        for (int jt = 0; jt < NJT; ++jt) acc[jt] = Acc{0, 0, 0, 0};
+    for (int jt = 0; jt < NJT; ++jt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s2[jt][r] = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < ntile; t += (int64_t)gridDim.x * 4) {
+        const int64_t p = t * TP + VEC * r16;
+        const bool pin = p < n;  // n % VEC == 0: all VEC points or none
+        Acc acc[VEC][NJT];
+#pragma unroll
+        for (int c = 0; c < VEC; ++c)
+#pragma unroll
+            for (int jt = 0; jt < NJT; ++jt) acc[c][jt] = Acc{0, 0, 0, 0};
         for (int i4 = 0; i4 < mp; i4 += 32) {
-            T b[8];
+            P b[8];
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int i = i4 + 4 * s + qd;
-                b[s] = (pin && i < m) ? A[(int64_t)i * lda + p] : T(0);
+                if (pin && i < m) {
+                    const T* src = i < m1 ? A + (int64_t)i * lda : A2 + (int64_t)(i - m1) * lda2;
+                    b[s] = *reinterpret_cast<const P*>(src + p);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) b[s].v[c] = T(0);
+                }
             }
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
@@ -92,7 +125,8 @@ __global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m, int k, T 
 #pragma unroll
                 for (int jt = 0; jt < NJT; ++jt) {
                     const T a = ct[(ib + qd) * KP + jt * 16 + r16];
-                    acc[jt] = U::mfma(a, b[s], acc[jt]);
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) acc[c][jt] = U::mfma(a, b[s].v[c], acc[c][jt]);
                 }
             }
         }
@@ -103,10 +137,47 @@ __global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m, int k, T 
             for (int r = 0; r < 4; ++r) {
                 const int j = jt * 16 + U::row(qd, r);
                 if (j >= k) continue;
-                T* y = Y + (int64_t)j * ldy + p;
-                *y = beta == T(0) ? alpha * acc[jt][r] : alpha * acc[jt][r] + beta * *y;
+                P* y = reinterpret_cast<P*>(Y + (int64_t)j * ldy + p);
+                P o;
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) o.v[c] = alpha * acc[c][jt][r];
+                if (beta != T(0)) {
+                    const P yo = *y;
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) o.v[c] += beta * yo.v[c];
+                }
+                *y = o;
+                if constexpr (NORM) {
+#pragma unroll
+                    for (int c = 0; c < VEC; ++c) s2[jt][r] += (double)o.v[c] * (double)o.v[c];
+                }
             }
     }
+    if constexpr (NORM) {
+        __shared__ double red[4][KP];
+#pragma unroll
+        for (int jt = 0; jt < NJT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double v = s2[jt][r];
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 16);
+                if (r16 == 0) red[w][jt * 16 + U::row(qd, r)] = v;
+            }
+        __syncthreads();
+        for (int j = threadIdx.x; j < k; j += 256)
+            part[(int64_t)j * gridDim.x + blockIdx.x] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+    }
+}
+
+// per-column sums of the NORM partials: out[j] = sum_b part[j * nblk + b]
+__global__ __launch_bounds__(64) void k_sum_parts(const double* __restrict__ part, int nblk, double* __restrict__ out) {
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 64) s += part[(int64_t)j * nblk + b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (threadIdx.x == 0) out[j] = s;
 }
 
 // true when the MFMA path applies (k <= 96 and C^T fits the LDS budget)
@@ -116,15 +187,36 @@ bool update_mfma_fits(int m, int k) {
     return k >= 1 && k <= 96 && (int64_t)((m + 3) / 4 * 4) * njt * 16 * (int64_t)sizeof(T) <= kUpdLdsBytes;
 }
 
+// Y = alpha [A | A2] C + beta Y (m1 inner rows from A); with norm2 != null,
+// norm2[j] = ||Y[:, j]||^2 (device FP64, k values) from the same pass.
 template <typename T>
-int update_mfma(dft_ctx* c, int64_t n, int m, int k, T alpha, const T* A, int64_t lda, const T* Cm, int ldc, T beta, T* Y,
-                int64_t ldy) {
+int update_mfma(dft_ctx* c, int64_t n, int m1, int m, int k, T alpha, const T* A, int64_t lda, const T* A2,
+                int64_t lda2, const T* Cm, int ldc, T beta, T* Y, int64_t ldy, double* norm2 = nullptr) {
     const int njt = (k + 15) / 16;
     const size_t lds = (size_t)((m + 3) / 4 * 4) * njt * 16 * sizeof(T);
-    const int64_t ntile = (n + 15) / 16;
+    constexpr int VEC = 16 / sizeof(T);
+    if (m1 >= m) A2 = A, lda2 = lda;
+    const bool vec = n % VEC == 0 && lda % VEC == 0 && lda2 % VEC == 0 && ldy % VEC == 0 &&
+                     reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(A2) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(Y) % 16 == 0;
+    const int64_t ntile = vec ? (n + 16 * VEC - 1) / (16 * VEC) : (n + 15) / 16;
     const unsigned blocks = (unsigned)std::min<int64_t>((ntile + 3) / 4, 8LL * c->num_cus);
+    double* part = nullptr;
+    if (norm2) {
+        if (int rc = ensure_partials(c, (size_t)blocks * k + 8)) return rc;
+        part = c->partials;
+    }
+#define DFT_UPD_K(NJ, V, NRM)                                                                                         \
+    k_update_mfma<T, NJ, V, NRM><<<blocks, 256, lds, c->stream>>>(n, m1, m, k, alpha, A, lda, A2, lda2, Cm, ldc, beta, \
+                                                                  Y, ldy, part)
 #define DFT_UPD(NJ)                                                                                                   \
-    k_update_mfma<T, NJ><<<blocks, 256, lds, c->stream>>>(n, m, k, alpha, A, lda, Cm, ldc, beta, Y, ldy)
+    if (vec) {                                                                                                        \
+        if (norm2) DFT_UPD_K(NJ, VEC, true);                                                                          \
+        else DFT_UPD_K(NJ, VEC, false);                                                                               \
+    } else {                                                                                                          \
+        if (norm2) DFT_UPD_K(NJ, 1, true);                                                                            \
+        else DFT_UPD_K(NJ, 1, false);                                                                                 \
+    }
     switch (njt) {
         case 1: DFT_UPD(1); break;
         case 2: DFT_UPD(2); break;
@@ -134,8 +226,32 @@ int update_mfma(dft_ctx* c, int64_t n, int m, int k, T alpha, const T* A, int64_
         default: DFT_UPD(6); break;
     }
 #undef DFT_UPD
+#undef DFT_UPD_K
     DFT_CHECK_LAUNCH(c);
+    if (norm2) {
+        k_sum_parts<<<k, 64, 0, c->stream>>>(part, (int)blocks, norm2);
+        DFT_CHECK_LAUNCH(c);
+    }
     return DFT_OK;
+}
+
+// coefficients of the fused Davidson residual, 2m x k column-major:
+// rows i < m: Q[i][j]; rows m + i: -theta[j] Q[i][j]
+template <typename T>
+__global__ __launch_bounds__(256) void k_ritz_coef(int m, int k, const double* __restrict__ q, int64_t ldq,
+                                                   const double* __restrict__ theta, T* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)2 * m * k) return;
+    const int i = (int)(e % (2 * m)), j = (int)(e / (2 * m));
+    out[e] = i < m ? (T)q[(int64_t)j * ldq + i] : (T)(-theta[j] * q[(int64_t)j * ldq + (i - m)]);
+}
+// -theta[j] Q[i][j], m x k column-major (the two-pass form's second operand)
+__global__ __launch_bounds__(256) void k_neg_q_theta(int m, int k, const double* __restrict__ q, int64_t ldq,
+                                                     const double* __restrict__ theta, double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * k) return;
+    const int i = (int)(e % m), j = (int)(e / m);
+    out[e] = -theta[j] * q[(int64_t)j * ldq + i];
 }
 
 // ---- tall-skinny block update on the VALU --------------------------------
@@ -238,7 +354,7 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
     if (int rc = blas(c, &h)) return rc;
     rocblas_status st;
     if (prec == DFT_F64 && update_mfma_fits<double>(m, k)) {
-        return update_mfma<double>(c, n, m, k, alpha, (const double*)a, lda, cm, (int)ldc, beta, (double*)y, ldy);
+        return update_mfma<double>(c, n, m, m, k, alpha, (const double*)a, lda, nullptr, 0, cm, (int)ldc, beta, (double*)y, ldy);
     } else if (prec == DFT_F64) {
         st = rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &alpha, (const double*)a, lda,
                               cm, ldc, &beta, (double*)y, ldy);
@@ -249,7 +365,7 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
         DFT_CHECK_LAUNCH(c);
         const float al = (float)alpha, be = (float)beta;
         if (update_mfma_fits<float>(m, k))
-            return update_mfma<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
+            return update_mfma<float>(c, n, m, m, k, al, (const float*)a, lda, nullptr, 0, c32, m, be, (float*)y, ldy);
         if (k <= 64 && m <= 512)
             return update_valu<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
         st = rocblas_sgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &al, (const float*)a, lda, c32,
@@ -301,6 +417,46 @@ int dft_chol_inv(dft_ctx* c, int m, double* a, int64_t lda) {
     k_lower_t<<<gb, 256, 0, c->stream>>>(m, t, a, lda);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
+}
+
+// Davidson residual block straight from the basis V (N x m) and W = H V:
+// R[:, j] = W Q[:, j] - theta[j] V Q[:, j] and rn2[j] = ||R[:, j]||^2 for
+// j < k.  One MFMA pass over [W | V] with the 2m x k coefficients
+// [Q ; -Q diag(theta)] when they fit the LDS budget (norms from the same
+// pass); otherwise R = W Q, then R += V (-Q theta), then the norms.
+int dft_ritz_residual(dft_ctx* c, int prec, int64_t n, int m, int k, const void* v, int64_t ldv, const void* w,
+                      int64_t ldw, const double* q, int64_t ldq, const double* theta, void* r, int64_t ldr, double* rn2) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, v && w && q && theta && r && rn2 && n > 0 && m > 0 && k >= 0 && ldv >= n && ldw >= n && ldr >= n &&
+                       ldq >= m && (prec == DFT_F64 || prec == DFT_F32),
+                "dft_ritz_residual: bad arguments");
+    if (k == 0) return DFT_OK;
+    const unsigned gc = (unsigned)(((int64_t)2 * m * k + 255) / 256);
+    if (prec == DFT_F64 && update_mfma_fits<double>(2 * m, k)) {
+        if (int rc = ensure_scratch(c, sizeof(double) * (size_t)2 * m * k)) return rc;
+        double* cf = (double*)c->scratch;
+        k_ritz_coef<double><<<gc, 256, 0, c->stream>>>(m, k, q, ldq, theta, cf);
+        DFT_CHECK_LAUNCH(c);
+        return update_mfma<double>(c, n, m, 2 * m, k, 1.0, (const double*)w, ldw, (const double*)v, ldv, cf, 2 * m, 0.0,
+                                   (double*)r, ldr, rn2);
+    }
+    if (prec == DFT_F32 && update_mfma_fits<float>(2 * m, k)) {
+        if (int rc = ensure_scratch(c, sizeof(float) * (size_t)2 * m * k)) return rc;
+        float* cf = (float*)c->scratch;
+        k_ritz_coef<float><<<gc, 256, 0, c->stream>>>(m, k, q, ldq, theta, cf);
+        DFT_CHECK_LAUNCH(c);
+        return update_mfma<float>(c, n, m, 2 * m, k, 1.0f, (const float*)w, ldw, (const float*)v, ldv, cf, 2 * m, 0.0f,
+                                  (float*)r, ldr, rn2);
+    }
+    // two passes; -Q theta sits past the FP32 coefficient copy dft_block_update keeps at the scratch start
+    const size_t off = ((sizeof(float) * (size_t)m * k) + 255) / 256 * 256;
+    if (int rc = ensure_scratch(c, off + sizeof(double) * (size_t)m * k)) return rc;
+    double* nq = (double*)((char*)c->scratch + off);
+    k_neg_q_theta<<<(unsigned)(((int64_t)m * k + 255) / 256), 256, 0, c->stream>>>(m, k, q, ldq, theta, nq);
+    DFT_CHECK_LAUNCH(c);
+    if (int rc = dft_block_update(c, prec, n, m, k, 1.0, w, ldw, q, ldq, 0.0, r, ldr)) return rc;
+    if (int rc = dft_block_update(c, prec, n, m, k, 1.0, v, ldv, nq, m, 1.0, r, ldr)) return rc;
+    return dft_column_norms2(c, prec, n, k, r, ldr, rn2);
 }
 
 }  // extern "C"

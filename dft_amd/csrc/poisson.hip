@@ -229,16 +229,20 @@ struct Jacobi {
 
     // nsweeps sweeps starting from buffer a (even sweeps a -> b, odd b -> a)
     int sweeps(T* a, T* b, int nsweeps, unsigned long long* slot) {
-        if (coop_blocks <= 0 || nsweeps == 1) {
-            for (int s = 0; s < nsweeps; ++s)
-                if (int rc = sweep((s & 1) ? b : a, (s & 1) ? a : b, slot + s)) return rc;
-            return DFT_OK;
+        if (coop_blocks > 0 && nsweeps > 1 && !c->coop_refused) {
+            int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz, gx = (int)grid.x, gy = (int)grid.y, gz = (int)grid.z;
+            int zcv = zc;
+            void* args[] = {&nx, &ny, &nz, &zcv, &gx, &gy, &gz, &co, &rho, &a, &b, &nsweeps, &slot};
+            const hipError_t e = hipLaunchCooperativeKernel((const void*)k_jacobi_multi<T, 4>, dim3(coop_blocks),
+                                                            dim3(256), args, 0, c->stream);
+            if (e == hipSuccess) return DFT_OK;
+            // refused before anything was queued (e.g. the cooperative queue
+            // is held by another process): clear the error, use plain launches
+            (void)hipGetLastError();
+            c->coop_refused = true;
         }
-        int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz, gx = (int)grid.x, gy = (int)grid.y, gz = (int)grid.z;
-        int zcv = zc;
-        void* args[] = {&nx, &ny, &nz, &zcv, &gx, &gy, &gz, &co, &rho, &a, &b, &nsweeps, &slot};
-        DFT_HIP(c, hipLaunchCooperativeKernel((const void*)k_jacobi_multi<T, 4>, dim3(coop_blocks), dim3(256), args, 0,
-                                              c->stream));
+        for (int s = 0; s < nsweeps; ++s)
+            if (int rc = sweep((s & 1) ? b : a, (s & 1) ? a : b, slot + s)) return rc;
         return DFT_OK;
     }
 };

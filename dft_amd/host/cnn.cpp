@@ -189,6 +189,7 @@ struct PatchSet {
 // up to max_columns residuals, whose high-residual patches become training pairs.
 class HarvestPreconditioner final : public Preconditioner {
    public:
+    bool needs_ritz_vectors() const override { return true; }  // projects the CG targets against X
     HarvestPreconditioner(dft_ctx* c, const dft_grid& g, PatchSet& ps, const dft_cnn_harvest_config& hc)
         : c_(c), g_(g), n_(grid_n(g)), ps_(ps), hc_(hc), rng_(hc.seed) {
         if (hc.patch != ps.P) throw std::invalid_argument("harvest: patch size differs from the trainer's");

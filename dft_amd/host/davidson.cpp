@@ -201,14 +201,26 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     P.reset();
     int mprev = 0;  // basis size when Qprev was taken (0: none yet)
     const int nact = nconv;
+    // X_ holds V Q_k of the latest Rayleigh-Ritz step only when x_fresh
+    bool x_fresh = false;
+    int m_rr = m;
+    auto ritz_block = [&](bool with_hx) {
+        check(c_, dft_block_update(c_, prec_, n_, m_rr, k, 1.0, V, n_, Q, mmax_, 0.0, X_.get(), n_), "dft_block_update");
+        if (with_hx)
+            check(c_, dft_block_update(c_, prec_, n_, m_rr, k, 1.0, W, n_, Q, mmax_, 0.0, HX_.get(), n_),
+                  "dft_block_update");
+        x_fresh = true;
+    };
+    const bool px = P.needs_ritz_vectors();
     for (int it = 1; it <= opt.max_iter; ++it) {
         st.iterations = it;
         check(c_, dft_copy(c_, Q, G, sizeof(double) * (size_t)mmax_ * mmax_), "dft_copy");
         check(c_, dft_syev(c_, m, Q, mmax_, theta), "dft_syev");
-        check(c_, dft_block_update(c_, prec_, n_, m, k, 1.0, V, n_, Q, mmax_, 0.0, X_.get(), n_), "dft_block_update");
-        check(c_, dft_block_update(c_, prec_, n_, m, k, 1.0, W, n_, Q, mmax_, 0.0, HX_.get(), n_), "dft_block_update");
-        check(c_, dft_residual(c_, prec_, n_, k, HX_.get(), n_, X_.get(), n_, theta, R_.get(), n_), "dft_residual");
-        check(c_, dft_column_norms2(c_, prec_, n_, k, R_.get(), n_, rn_.as<double>()), "dft_column_norms2");
+        m_rr = m;
+        x_fresh = false;
+        if (px) ritz_block(false);
+        check(c_, dft_ritz_residual(c_, prec_, n_, m, k, V, n_, W, n_, Q, mmax_, theta, R_.get(), n_, rn_.as<double>()),
+              "dft_ritz_residual");
         check(c_, dft_download(c_, theta_h_.data(), theta, sizeof(double) * k), "dft_download");
         check(c_, dft_download(c_, rn_h_.data(), rn_.get(), sizeof(double) * k), "dft_download");
         double maxres = 0.0;
@@ -242,6 +254,7 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             check(c_, dft_block_update(c_, prec_, n_, m, nt, -1.0, V, n_, Pm, mmax_, 1.0, T_.get(), n_), "dft_block_update");
         }
         if (m + nt > mmax_) {  // thick restart on [X, P]
+            ritz_block(true);
             m = mprev > 0 ? restart(m, mprev) : 0;
             if (m == 0) {
                 check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
@@ -264,6 +277,9 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
         check(c_, dft_transpose(c_, m, kept, G + (size_t)m * mmax_, mmax_, G + m, mmax_), "dft_transpose");
         m += kept;
     }
+    // the Ritz block of the last Rayleigh-Ritz step (V's first m_rr columns
+    // are unchanged unless a restart formed X_ itself)
+    if (!x_fresh) ritz_block(false);
     return st;
 }
 

@@ -4,7 +4,9 @@
 // products are the MFMA Gram kernels:
 //   basis V (N x m, orthonormal), W = H V, G = V^T W (FP64, m x m)
 //   Rayleigh-Ritz: G = Q diag(theta) Q^T (FP64 on the device)
-//   X = V Q_k, HX = W Q_k, R = HX - X theta
+//   R = W Q_k - V Q_k theta in one pass over [W | V], norms in the same pass;
+//   the Ritz block X = V Q_k (and HX = W Q_k) is formed only where it is
+//   read: before a restart, for a preconditioner that reads X, and at exit
 //   T = P(R) for unconverged columns (P = diagonal or learned preconditioner)
 //   T <- T - V (V^T T) twice, then CholQR twice, appended to V;
 //   restart to V = X when m would exceed the subspace limit.
@@ -36,6 +38,8 @@ struct Preconditioner {
     virtual void feedback(const std::vector<double>& /*before*/, const std::vector<double>& /*after*/) {}
     // Called when a solve starts (new operator).
     virtual void reset() {}
+    // True when apply() reads the Ritz block X (then it is formed every iteration).
+    virtual bool needs_ritz_vectors() const { return false; }
     virtual int64_t applied() const { return 0; }
     virtual int64_t fallbacks() const { return 0; }
 };

@@ -174,6 +174,16 @@ int dft_scale_columns(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, 
                       void* y, int64_t ldy);
 int dft_residual(dft_ctx* ctx, int prec, int64_t n, int nb, const void* hx, int64_t ldh, const void* x, int64_t ldx,
                  const double* theta, void* r, int64_t ldr);
+/* Davidson residuals from the basis: r[:,j] = w q[:,j] - theta[j] v q[:,j]
+ * (w = H v, N x m; q m x k device FP64, ldq >= m; theta device FP64) and
+ * rn2[j] = ||r[:,j]||^2 (device FP64, k values), in one pass over v and w
+ * when 2m x k coefficients fit the kernel's LDS budget (two passes and a
+ * norm pass otherwise).  Replaces the residual and norm loop of the
+ * reference's Davidson (src/dft/EigenvalueSolverMolecule.cpp:151-175, 893-903) for a
+ * block, without forming the Ritz vectors. */
+int dft_ritz_residual(dft_ctx* ctx, int prec, int64_t n, int m, int k, const void* v, int64_t ldv, const void* w,
+                      int64_t ldw, const double* q, int64_t ldq, const double* theta, void* r, int64_t ldr,
+                      double* rn2);
 /* Column norms^2 (FP64 device out, nb values) */
 int dft_column_norms2(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, int64_t ldx, double* out);
 

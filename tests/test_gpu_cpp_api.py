@@ -18,9 +18,19 @@ def rel(a, b):
 
 
 def test_reference_cpp_api_against_oracle(tmp_path, oracle):
+    """The program runs while this process holds a device context that has
+    made a cooperative (multi-sweep) Jacobi launch: the device's cooperative
+    queue can then be unavailable to the child, whose solves must fall back to
+    one launch per sweep and still match the oracle."""
+    import dft_amd
     from oracle.oracle import Grid
     exe = os.path.join(ROOT, "tests", "cpp", "bin", "molecule_api_check")  # built by __graft_entry__.build()
     assert os.path.exists(exe), "tests/cpp/bin/molecule_api_check is not built"
+    dev = dft_amd.Device(0)
+    gs = dft_amd.Grid.centered(20, 0.4)  # few tiles: the cooperative kernel applies
+    dev.poisson(gs, dev.array(np.full(gs.size, 1e-3)), dev.zeros((gs.size,)), max_sweeps=300, tol=0.0,
+                zero_start=True, batch=100)
+    dev.sync()
     og = Grid(21, 18, 16, -4.0, 4.2, -3.9, 3.6, -3.3, 3.5)
     n = og.size
     rng = np.random.default_rng(11)
@@ -33,6 +43,7 @@ def test_reference_cpp_api_against_oracle(tmp_path, oracle):
     for k, a in data.items():
         np.asarray(a, dtype=np.float64).tofile(tmp_path / f"{k}.bin")
     out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    dev.close()
     assert out.returncode == 0, out.stderr
     rd = lambda k: np.fromfile(tmp_path / f"{k}.out", dtype=np.float64)
     assert rel(rd("hpsi"), oracle.hpsi(og, data["v"], data["psi"])) <= 1e-12

@@ -265,7 +265,8 @@ def test_gram_modes(dev, n, ma, mb):
         assert rel(got, ref32) <= tol
 
 
-@pytest.mark.parametrize("n,m,k", [(1000, 8, 8), (50001, 37, 70), (200000, 192, 96), (70001, 29, 13)])
+@pytest.mark.parametrize("n,m,k", [(1000, 8, 8), (50001, 37, 70), (200000, 192, 96), (70001, 29, 13), (70000, 29, 13),
+                                   (4100, 90, 96), (64004, 61, 33), (64002, 5, 1)])
 def test_block_update(dev, n, m, k):
     rng = np.random.default_rng(m)
     A = rng.standard_normal((m, n))
@@ -282,6 +283,30 @@ def test_block_update(dev, n, m, k):
     dyn = dev.array(np.full((k, n), np.nan), dtype=np.float32)
     dev.block_update(dev.array(A, dtype=np.float32), dev.array(Cm), dyn, n, m, k, alpha=0.5, beta=0.0)
     assert rel(dyn.download(), 0.5 * (Cm @ A)) <= 1e-5
+
+
+@pytest.mark.parametrize("n,m,k", [(64000, 87, 29), (70001, 40, 29), (4096, 61, 96), (30002, 150, 40), (5000, 3, 1)])
+def test_ritz_residual(dev, n, m, k):
+    """R = W Q - V Q diag(theta) and its column norms^2, against numpy: one
+    fused pass (vector and scalar grid sizes) and the two-pass form (2m x k
+    coefficients beyond the LDS budget: m = 150 in FP64 and FP32)."""
+    rng = np.random.default_rng(n + m)
+    V = rng.standard_normal((m, n))
+    W = rng.standard_normal((m, n))
+    ldq = m + 3
+    Q = rng.standard_normal((k, ldq))  # column-major m x k with ldq >= m
+    th = rng.standard_normal(k)
+    Qm = Q[:, :m]
+    ref = Qm @ W - th[:, None] * (Qm @ V)
+    for dt, tol in ((np.float64, 1e-13), (np.float32, 3e-6)):
+        r = dev.array(np.full((k, n), np.nan), dtype=dt)  # every entry is written
+        rn = dev.zeros((k,))
+        dev.ritz_residual(dev.array(V, dtype=dt), dev.array(W, dtype=dt), dev.array(Q), dev.array(th), r, rn, n, m, k,
+                          ldq=ldq)
+        got = r.download()
+        assert rel(got, ref) <= tol * np.sqrt(m)
+        assert np.max(np.abs(rn.download() - np.sum(got.astype(np.float64) ** 2, axis=1))) <= 1e-12 * np.max(
+            np.sum(ref ** 2, axis=1))
 
 
 def test_syev_and_cholinv(dev):
