@@ -4,6 +4,8 @@
 // stream.
 #include "common.hpp"
 
+#include <atomic>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -254,6 +256,336 @@ __global__ __launch_bounds__(256) void k_neg_q_theta(int m, int k, const double*
     out[e] = -theta[j] * q[(int64_t)j * ldq + i];
 }
 
+// ---- small symmetric eigensolver (the Davidson's Rayleigh-Ritz step) -------
+// The k lowest eigenpairs of a symmetric m x m matrix in one workgroup
+// (rocSOLVER's syevd spends ~1.4 ms per call at m = 87 in dozens of
+// latency-bound launches; this is one launch with every array in LDS):
+//  1. Householder tridiagonalisation T = H^T A H (reflectors in the LAPACK
+//     dsytd2 convention: v[0] = 1, H_j = I - tau_j v_j v_j^T);
+//  2. the k lowest eigenvalues of T by Sturm-count multisection, 256 / k
+//     counts per level and step;
+//  3. eigenvectors of T by inverse iteration, one thread per level, four
+//     iterations from a fixed pseudo-random start.  T - lambda I = L D L^T
+//     without pivoting (the Sturm recurrence; pivots below eps ||T|| are
+//     raised to that size), the inverse pivots kept for the later solves.
+//     Levels closer than 1e-9 ||T|| (degenerate to working precision)
+//     iterate together in one thread, each vector orthogonalised against the
+//     earlier ones of its group every iteration (LAPACK dstein's scheme);
+//  4. one modified Gram-Schmidt pass over the k vectors: independently
+//     computed vectors of levels a gap g apart overlap by ~eps ||T|| / g,
+//     and removing that overlap changes each residual by ~eps ||T|| only;
+//  5. back-transform z = H_0 ... H_{m-3} x.
+// LDS: A (m x (m+1)), the k vectors and the k inverse-pivot rows (k x m
+// each) are dynamic; the fast path needs m (m+1) + 2 k m <= kEigLdsDoubles.
+constexpr int kEigMax = 96;
+constexpr int kEigLdsDoubles = 19000;  // + ~10 KB of static LDS <= 160 KB
+constexpr int kEigThreads = 1024;
+
+__device__ inline double eig_wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+// sum over an aligned group of G lanes (G a power of two <= 64)
+__device__ inline double eig_group_sum(double v, int G) {
+    for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(kEigThreads) void k_syev_small(int m, int k, const double* __restrict__ a, int64_t lda,
+                                                            double* __restrict__ z, int64_t ldz,
+                                                            double* __restrict__ w) {
+    constexpr int NT = kEigThreads;
+    extern __shared__ double eig_dyn[];
+    const int LD = m + 1;
+    double* A = eig_dyn;        // A[i * LD + j]
+    double* X = A + m * LD;     // eigenvectors of T, k x m
+    double* DI = X + k * m;     // inverse LDL^T pivots, one row per group
+    __shared__ double vb[kEigMax], pb[kEigMax];
+    __shared__ double d[kEigMax], e[kEigMax], e2[kEigMax], tau[kEigMax], lam[kEigMax];
+    __shared__ double blo[kEigMax], bhi[kEigMax], xs[256];
+    __shared__ int cnt[256];
+    __shared__ double sc[4];
+    __shared__ int nit_s;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int q = tid; q < m * m; q += NT) {
+        const int i = q % m, j = q / m;  // A[i][j] from the lower triangle
+        A[i * LD + j] = i >= j ? a[(int64_t)j * lda + i] : a[(int64_t)i * lda + j];
+    }
+    __syncthreads();
+    // 1. tridiagonalisation
+    for (int j = 0; j + 2 < m; ++j) {
+        const int r = m - j - 1;
+        if (wv == 0) {
+            double s = 0.0;
+            for (int i = 1 + lane; i < r; i += 64) {
+                const double x = A[(j + 1 + i) * LD + j];
+                s += x * x;
+            }
+            s = eig_wave_sum(s);
+            const double alpha = A[(j + 1) * LD + j];
+            double t = 0.0, beta = alpha, scale = 0.0;
+            if (s > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + s), alpha);
+                t = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            for (int i = lane; i < r; i += 64) vb[i] = i == 0 ? 1.0 : A[(j + 1 + i) * LD + j] * scale;
+            if (lane == 0) {
+                tau[j] = t;
+                d[j] = A[j * LD + j];
+                e[j] = beta;
+            }
+        }
+        __syncthreads();
+        const double t = tau[j];
+        if (t != 0.0) {
+            {  // p = tau A22 v, eight lanes per row
+                const int row = tid >> 3, part = tid & 7;
+                double s0 = 0.0, s1 = 0.0;
+                if (row < r) {
+                    const double* rp = &A[(j + 1 + row) * LD + j + 1];
+                    int l = part;
+                    for (; l + 8 < r; l += 16) {
+                        s0 += rp[l] * vb[l];
+                        s1 += rp[l + 8] * vb[l + 8];
+                    }
+                    if (l < r) s0 += rp[l] * vb[l];
+                }
+                const double s = eig_group_sum(s0 + s1, 8);
+                if (row < r && part == 0) pb[row] = t * s;
+            }
+            __syncthreads();
+            if (wv == 0) {
+                double s = 0.0;
+                for (int i = lane; i < r; i += 64) s += pb[i] * vb[i];
+                s = eig_wave_sum(s);
+                const double K = 0.5 * t * s;
+                for (int i = lane; i < r; i += 64) pb[i] -= K * vb[i];
+            }
+            __syncthreads();
+            {  // A22 -= v w^T + w v^T: a thread owns columns tl, tl + 32, tl + 64 (v, w in registers)
+                const int ti = tid >> 5, tl = tid & 31;
+                const bool c1 = tl + 32 < r, c2 = tl + 64 < r;
+                const double v0 = tl < r ? vb[tl] : 0.0, w0 = tl < r ? pb[tl] : 0.0;
+                const double v1 = c1 ? vb[tl + 32] : 0.0, w1 = c1 ? pb[tl + 32] : 0.0;
+                const double v2 = c2 ? vb[tl + 64] : 0.0, w2 = c2 ? pb[tl + 64] : 0.0;
+                if (tl < r)
+                    for (int i = ti; i < r; i += 32) {
+                        const double vi = vb[i], wi = pb[i];
+                        double* rp = &A[(j + 1 + i) * LD + j + 1 + tl];
+                        rp[0] -= vi * w0 + wi * v0;
+                        if (c1) rp[32] -= vi * w1 + wi * v1;
+                        if (c2) rp[64] -= vi * w2 + wi * v2;
+                    }
+            }
+        }
+        for (int i = tid; i < r; i += NT) A[(j + 1 + i) * LD + j] = vb[i];  // reflector, v[0] = 1
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int i = m >= 2 ? m - 2 : 0; i < m; ++i) d[i] = A[i * LD + i];
+        if (m >= 2) e[m - 2] = A[(m - 1) * LD + m - 2];
+        double lo = d[0], hi = d[0], emax2 = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double rad = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < m ? fabs(e[i]) : 0.0);
+            lo = fmin(lo, d[i] - rad);
+            hi = fmax(hi, d[i] + rad);
+            if (i + 1 < m) {
+                e2[i] = e[i] * e[i];
+                emax2 = fmax(emax2, e2[i]);
+            }
+        }
+        const double eps = 1.1102230246251565e-16;
+        // scale of T, floored so that tolerances and pivots derived from it stay normal numbers
+        const double tn = fmax(fmax(fabs(lo), fabs(hi)), 1e-150);
+        lo -= 4.0 * eps * tn;
+        hi += 4.0 * eps * tn;
+        sc[0] = lo;
+        sc[1] = hi;
+        sc[2] = tn;
+        sc[3] = 2.2250738585072014e-308 * fmax(1.0, emax2);  // pivmin
+        const int groups = 256 / k;
+        const double ratio = (hi - lo) / (eps * tn);
+        // bounded for any input (non-finite entries give garbage, never a long loop)
+        const double steps = ratio > 1.0 ? ceil(log(ratio) / log((double)(groups + 1))) + 2.0 : 0.0;
+        nit_s = steps >= 0.0 && steps <= 128.0 ? (int)steps : (steps > 128.0 ? 128 : 0);
+    }
+    __syncthreads();
+    // 2. multisection for eigenvalue indices 0..k-1: 256 / k points per level and
+    //    step (more counting threads only contend for the FP64 pipes)
+    const double tn = sc[2], pivmin = sc[3];
+    {
+        const int groups = 256 / k;
+        const int idx = tid % k, sgrp = tid / k;
+        const bool act = sgrp < groups;
+        if (tid < k) {
+            blo[tid] = sc[0];
+            bhi[tid] = sc[1];
+        }
+        __syncthreads();
+        const int nit = nit_s;
+        for (int it = 0; it < nit; ++it) {
+            if (act) {
+                const double L = blo[idx], U = bhi[idx];
+                const double x = L + (U - L) * (double)(sgrp + 1) / (double)(groups + 1);
+                double q = d[0] - x;
+                if (fabs(q) <= pivmin) q = -pivmin;
+                int c = q < 0.0;
+                for (int i = 1; i < m; ++i) {
+                    // e2 / q with the hardware reciprocal and one Newton step (relative
+                    // error ~1e-16: the count stays that of a matrix within rounding of T)
+                    double rq = __builtin_amdgcn_rcp(q);
+                    rq = fma(rq, fma(-q, rq, 1.0), rq);
+                    q = d[i] - x - e2[i - 1] * rq;
+                    if (fabs(q) <= pivmin) q = -pivmin;
+                    c += q < 0.0;
+                }
+                xs[tid] = x;
+                cnt[tid] = c;
+            }
+            __syncthreads();
+            if (tid < k) {
+                double L = blo[tid], U = bhi[tid];
+                for (int s = 0; s < groups; ++s) {
+                    const int t2 = s * k + tid;
+                    if (cnt[t2] <= tid) L = fmax(L, xs[t2]);
+                    else U = fmin(U, xs[t2]);
+                }
+                blo[tid] = L;
+                bhi[tid] = U;
+            }
+            __syncthreads();
+        }
+        if (tid < k) lam[tid] = 0.5 * (blo[tid] + bhi[tid]);
+        __syncthreads();
+    }
+    // 3. inverse iteration, one thread per group of (numerically) degenerate levels
+    const double eps = 1.1102230246251565e-16;
+    const double gtol = 1e-9 * tn, pertol = 10.0 * eps * tn, tiny = eps * tn;
+    if (tid < k && (tid == 0 || lam[tid] - lam[tid - 1] > gtol)) {
+        double* di = DI + tid * m;
+        int c1 = tid + 1;
+        while (c1 < k && lam[c1] - lam[c1 - 1] <= gtol) ++c1;
+        double prev = 0.0;
+        for (int jj = tid; jj < c1; ++jj) {
+            double sh = lam[jj];
+            if (jj > tid && sh - prev < pertol) sh = prev + pertol;
+            prev = sh;
+            double* x = X + jj * m;
+            uint64_t st = 0x9e3779b97f4a7c15ULL * (uint64_t)(jj + 1);
+            for (int i = 0; i < m; ++i) {
+                st ^= st >> 12;
+                st ^= st << 25;
+                st ^= st >> 27;
+                x[i] = (double)((st * 0x2545F4914F6CDD1DULL) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            }
+            double scale = 1.0;  // 1 / max|x| of the previous solve (|x| can reach ~1/tiny)
+            for (int itr = 0; itr < 3; ++itr) {
+                // forward: L y = scale x (the first pass also factors T - sh I)
+                x[0] *= scale;
+                if (itr == 0) {
+                    double dc = d[0] - sh;
+                    if (fabs(dc) < tiny) dc = dc < 0.0 ? -tiny : tiny;
+                    di[0] = 1.0 / dc;
+                    for (int i = 1; i < m; ++i) {
+                        const double l = e[i - 1] * di[i - 1];
+                        dc = (d[i] - sh) - l * e[i - 1];
+                        if (fabs(dc) < tiny) dc = dc < 0.0 ? -tiny : tiny;
+                        di[i] = 1.0 / dc;
+                        x[i] = x[i] * scale - l * x[i - 1];
+                    }
+                } else {
+#pragma unroll 4
+                    for (int i = 1; i < m; ++i) x[i] = x[i] * scale - (e[i - 1] * di[i - 1]) * x[i - 1];
+                }
+                // backward: L^T x = D^{-1} y, tracking max|x|
+                x[m - 1] *= di[m - 1];
+                double xm = fabs(x[m - 1]);
+#pragma unroll 4
+                for (int i = m - 2; i >= 0; --i) {
+                    x[i] = (x[i] - e[i] * x[i + 1]) * di[i];
+                    xm = fmax(xm, fabs(x[i]));
+                }
+                scale = xm > 0.0 ? 1.0 / xm : 1.0;
+                if (jj > tid) {  // earlier (normalised) vectors of the group
+                    for (int i = 0; i < m; ++i) x[i] *= scale;
+                    scale = 1.0;
+                    for (int c = tid; c < jj; ++c) {
+                        const double* y = X + c * m;
+                        double s = 0.0;
+                        for (int i = 0; i < m; ++i) s += y[i] * x[i];
+                        for (int i = 0; i < m; ++i) x[i] -= s * y[i];
+                    }
+                }
+            }
+            double nn = 0.0;
+            for (int i = 0; i < m; ++i) {
+                x[i] *= scale;
+                nn += x[i] * x[i];
+            }
+            const double inv = 1.0 / sqrt(fmax(nn, 1e-300));
+            for (int i = 0; i < m; ++i) x[i] *= inv;
+        }
+    }
+    __syncthreads();
+    // 4. modified Gram-Schmidt over the k vectors in level order, G lanes per vector
+    int G = 64;
+    while (G > 1 && G * k > NT) G >>= 1;
+    const int grp = tid / G, gl = tid % G;
+    for (int c = 0; c < k; ++c) {
+        double* xc = X + c * m;
+        if (grp == c) {
+            double s = 0.0;
+            for (int i = gl; i < m; i += G) s += xc[i] * xc[i];
+            s = eig_group_sum(s, G);
+            const double inv = 1.0 / sqrt(fmax(s, 1e-300));
+            for (int i = gl; i < m; i += G) xc[i] *= inv;
+        }
+        __syncthreads();
+        if (grp > c && grp < k) {
+            double* xj = X + grp * m;
+            double s = 0.0;
+            for (int i = gl; i < m; i += G) s += xc[i] * xj[i];
+            s = eig_group_sum(s, G);
+            for (int i = gl; i < m; i += G) xj[i] -= s * xc[i];
+        }
+        __syncthreads();
+    }
+    // 5. back-transform, G lanes per column (a column's lanes share one wave)
+    if (grp < k) {
+        double* x = X + grp * m;
+        for (int j = m - 3; j >= 0; --j) {
+            const double tj = tau[j];
+            if (tj == 0.0) continue;
+            const int r = m - j - 1;
+            double s = 0.0;
+            for (int i = gl; i < r; i += G) s += A[(j + 1 + i) * LD + j] * x[j + 1 + i];
+            s = tj * eig_group_sum(s, G);
+            for (int i = gl; i < r; i += G) x[j + 1 + i] -= s * A[(j + 1 + i) * LD + j];
+        }
+        // sign convention: the largest component positive
+        double best = 0.0;
+        int bi = 0;
+        for (int i = gl; i < m; i += G)
+            if (fabs(x[i]) > best) {
+                best = fabs(x[i]);
+                bi = i;
+            }
+        for (int o = G >> 1; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        const double sg = x[bi] < 0.0 ? -1.0 : 1.0;
+        for (int i = gl; i < m; i += G) z[(int64_t)grp * ldz + i] = sg * x[i];
+        if (gl == 0) w[grp] = lam[grp];
+    }
+}
+
 // ---- tall-skinny block update on the VALU --------------------------------
 // Y = alpha * A C + beta * Y with A: N x m (N grid points), C: m x k small.
 // Each thread owns two grid points and KMAX output columns in registers; the
@@ -417,6 +749,40 @@ int dft_chol_inv(dft_ctx* c, int m, double* a, int64_t lda) {
     k_lower_t<<<gb, 256, 0, c->stream>>>(m, t, a, lda);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
+}
+
+// The k lowest eigenpairs of the symmetric matrix a (lower triangle read):
+// eigenvalues ascending in w[0..k) (w has room for m values), eigenvectors in
+// z's first k columns; a is not modified.  One workgroup when m <= 96 and
+// m (m + 1) + 2 k m <= 19000 doubles (the LDS it needs); rocSOLVER dsyevd on
+// a copy otherwise.
+int dft_syev_lowest(dft_ctx* c, int m, int k, const double* a, int64_t lda, double* z, int64_t ldz, double* w) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, a && z && w && m > 0 && k > 0 && k <= m && lda >= m && ldz >= m && m <= 4096 && z != a,
+                "dft_syev_lowest: bad arguments");
+    const int64_t nd = (int64_t)m * (m + 1) + 2LL * k * m;
+    if (m <= kEigMax && nd <= kEigLdsDoubles) {
+        const size_t bytes = sizeof(double) * (size_t)nd;
+        // the kernel's dynamic LDS limit, raised once per device
+        static std::atomic<uint64_t> attr_done{0};
+        const uint64_t bit = 1ULL << (c->device & 63);
+        if (!(attr_done.load() & bit)) {
+            DFT_HIP(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_syev_small),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(sizeof(double) * kEigLdsDoubles)));
+            attr_done.fetch_or(bit);
+        }
+        k_syev_small<<<1, kEigThreads, bytes, c->stream>>>(m, k, a, lda, z, ldz, w);
+        DFT_CHECK_LAUNCH(c);
+        return DFT_OK;
+    }
+    if (ldz == lda) {
+        DFT_HIP(c, hipMemcpyAsync(z, a, sizeof(double) * (size_t)lda * m, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        DFT_HIP(c, hipMemcpy2DAsync(z, sizeof(double) * ldz, a, sizeof(double) * lda, sizeof(double) * m, m,
+                                    hipMemcpyDeviceToDevice, c->stream));
+    }
+    return dft_syev(c, m, z, ldz, w);
 }
 
 // Davidson residual block straight from the basis V (N x m) and W = H V:

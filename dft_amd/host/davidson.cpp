@@ -214,8 +214,7 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     const bool px = P.needs_ritz_vectors();
     for (int it = 1; it <= opt.max_iter; ++it) {
         st.iterations = it;
-        check(c_, dft_copy(c_, Q, G, sizeof(double) * (size_t)mmax_ * mmax_), "dft_copy");
-        check(c_, dft_syev(c_, m, Q, mmax_, theta), "dft_syev");
+        check(c_, dft_syev_lowest(c_, m, k, G, mmax_, Q, mmax_, theta), "dft_syev_lowest");
         m_rr = m;
         x_fresh = false;
         if (px) ritz_block(false);

@@ -191,6 +191,16 @@ int dft_column_norms2(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, 
 /* Symmetric eigensolve of an m x m FP64 device matrix (column-major, ld):
  * eigenvalues ascending into w (device), eigenvectors overwrite a. */
 int dft_syev(dft_ctx* ctx, int m, double* a, int64_t lda, double* w);
+/* The k lowest eigenpairs of a symmetric m x m FP64 device matrix (lower
+ * triangle read, a unchanged): eigenvalues ascending in w (room for m
+ * values), eigenvectors in the first k columns of z (ldz >= m, z != a).
+ * One launch, no host synchronisation, when m <= 96 and
+ * m (m + 1) + 2 k m <= 19000 (Householder tridiagonalisation, Sturm
+ * multisection, inverse iteration, all in LDS); rocSOLVER dsyevd otherwise.
+ * The Rayleigh-Ritz step of the block Davidson; replaces the reference's
+ * Jacobi `diagonalize` (src/dft/EigenvalueSolverMolecule.cpp:290-449), which
+ * returns the lowest pair of the projected matrix. */
+int dft_syev_lowest(dft_ctx* ctx, int m, int k, const double* a, int64_t lda, double* z, int64_t ldz, double* w);
 /* Cholesky S = L L^T of an SPD m x m device matrix, then a <- L^{-T}
  * (upper triangular, zeros below): for S = T^T T, T * a has orthonormal
  * columns (CholQR).  Returns -2 if S is not positive definite. */

@@ -309,6 +309,57 @@ def test_ritz_residual(dev, n, m, k):
             np.sum(ref ** 2, axis=1))
 
 
+def _sym_cases():
+    rng = np.random.default_rng(5)
+    out = []
+    # (96, 96) and (120, 29) exceed the one-workgroup LDS budget: the rocSOLVER path
+    for m, k in ((1, 1), (2, 2), (3, 1), (17, 9), (64, 29), (87, 29), (80, 64), (96, 96), (96, 40), (120, 29)):
+        M = rng.standard_normal((m, m))
+        out.append((f"random{m}x{k}", (M + M.T) / 2 + np.diag(np.linspace(-5, 3, m)), k))
+    # exact degeneracies (pairs and a triple) in a rotated basis, as in symmetric molecules
+    m = 60
+    lam = np.sort(np.concatenate([[-3.0, -3.0, -1.2, -1.2, -1.2, -0.5], np.linspace(0.1, 9.0, m - 6)]))
+    Qr, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    out.append(("degenerate60", (Qr * lam) @ Qr.T, 20))
+    # already diagonal with a zero block (the reduction steps are identities)
+    out.append(("diagonal40", np.diag(np.linspace(2.0, -1.0, 40)), 12))
+    out.append(("zero7", np.zeros((7, 7)), 3))
+    # Rayleigh-Ritz shaped: clustered negative levels and a spread-out top
+    m = 87
+    lam = np.concatenate([np.linspace(-10.0, -0.3, 29) + 1e-9 * rng.standard_normal(29), np.geomspace(0.5, 80.0, m - 29)])
+    Qr, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    out.append(("ritz87", (Qr * lam) @ Qr.T, 29))
+    return out
+
+
+@pytest.mark.parametrize("case", _sym_cases(), ids=lambda c: c[0])
+def test_syev_lowest(dev, case):
+    """k lowest eigenpairs against numpy's eigh: eigenvalues within 1e-13 ||A||,
+    residuals and orthonormality within 1e-12 ||A||, and each eigenspace
+    (through its projector) for degenerate or clustered levels."""
+    _, A, k = case
+    m = A.shape[0]
+    w, z = dev.syev_lowest(dev.array(A.T.copy()), m, k)  # column-major upload of a symmetric matrix
+    lam = w.download()[:k]
+    Z = z.download()[:k].T  # m x k
+    ref, V = np.linalg.eigh(A)
+    nrm = max(np.max(np.abs(ref)), 1e-100)  # the zero matrix: eigenvalues at the 1e-150 scale floor pass
+    assert np.max(np.abs(lam - ref[:k])) <= 1e-13 * nrm
+    assert np.max(np.abs(A @ Z - Z * lam)) <= 1e-12 * nrm
+    assert np.max(np.abs(Z.T @ Z - np.eye(k))) <= 1e-12
+    # eigenspaces: group levels closer than 1e-8 ||A|| (projectors are basis-free)
+    i = 0
+    while i < k:
+        j = i + 1
+        while j < m and ref[j] - ref[j - 1] <= 1e-8 * nrm:
+            j += 1
+        if j <= k:
+            Pr = V[:, i:j] @ V[:, i:j].T
+            Pg = Z[:, i:j] @ Z[:, i:j].T
+            assert np.max(np.abs(Pr - Pg)) <= 1e-10, (i, j)
+        i = j
+
+
 def test_syev_and_cholinv(dev):
     rng = np.random.default_rng(2)
     m = 40
