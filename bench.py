@@ -90,11 +90,11 @@ WORKLOADS = {
                functional=1, nstates=4, poisson_fixed=0, init=1),
     "c2": dict(desc="Ne atom, 96^3, h=0.2, PBE96, Davidson block 8, Jacobi 200 sweeps/step", atoms=[[10, 0.0, 0.0, 0.0]],
                n=96, h=0.2, functional=2, nstates=8, poisson_fixed=200, init=0),
-    "c3": dict(desc="benzene, 160^3, h=0.2, LSDA/PZ81, Davidson block 29, Chebyshev-accelerated Jacobi Poisson",
-               atoms=benzene(), n=160, h=0.2, functional=1, nstates=29, poisson_fixed=400, poisson="cheb", init=1),
+    "c3": dict(desc="benzene, 160^3, h=0.2, LSDA/PZ81, Davidson block 29, Poisson at the Jacobi fixed point",
+               atoms=benzene(), n=160, h=0.2, functional=1, nstates=29, poisson_fixed=400, poisson="direct", init=1),
     "c5": dict(desc="40-atom random cluster (seed 5, radius 7 bohr, 150-170 electrons), 256^3, h=0.18, PBE96, block 96, "
-                    "Chebyshev-accelerated Jacobi Poisson",
-               atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, poisson="cheb", init=1),
+                    "Poisson at the Jacobi fixed point",
+               atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, poisson="direct", init=1),
 }
 
 
@@ -106,8 +106,11 @@ def workload_atoms(w):
 
 def poisson_args(dft_amd, w):
     """Jacobi with a fixed number of warm sweeps per call (C2's definition), or
-    Chebyshev-accelerated Jacobi sweeps to max|dV| < 1e-8 (C3, C5: the
-    definition leaves the Poisson schedule open)."""
+    the converged Jacobi fixed point (C3, C5: the definition leaves the Poisson
+    schedule open), by sine transforms ("direct") or Chebyshev-accelerated
+    sweeps to max|dV| < 1e-8 ("cheb")."""
+    if w.get("poisson", "jacobi") == "direct":
+        return dict(poisson_method=dft_amd.POISSON_DIRECT, poisson_fixed=0)
     if w.get("poisson", "jacobi") == "cheb":
         return dict(poisson_method=dft_amd.POISSON_CHEBYSHEV, poisson_tol=1e-8, poisson_fixed=0)
     return dict(poisson_method=dft_amd.POISSON_JACOBI, poisson_fixed=w["poisson_fixed"])
@@ -388,8 +391,9 @@ def main():
         "dtype": "f32+bf16x3 (f64 subspace, f64 refinement)", "data": "synthetic",
         "config": {"workload": f"{a.workload}: {w['desc']}", "grid": w["n"], "h": w["h"], "atoms": len(atoms),
                    "electrons": ne, "davidson_block": w["nstates"],
-                   "poisson": ("chebyshev to max|dV|<1e-8" if w.get("poisson") == "cheb"
-                               else f"{w['poisson_fixed']} warm Jacobi sweeps per call"),
+                   "poisson": {"cheb": "chebyshev to max|dV|<1e-8",
+                               "direct": "Jacobi fixed point by sine transforms"}.get(
+                                   w.get("poisson"), f"{w['poisson_fixed']} warm Jacobi sweeps per call"),
                    "stop": "|dE| < 1e-5 Ha",
                    "preconditioner": {"diag": "diagonal (the reference's)", "kinetic": "kinetic smoothing",
                                       "cnn": f"learned ({a.cnn_model}) with diagonal fallback"}[a.precond],

@@ -229,6 +229,10 @@ class Device:
                                                C.c_int(batch), C.byref(sw), C.byref(md)))
         return sw.value, md.value
 
+    def poisson_direct(self, g, rho, v):
+        """V_H at the Jacobi fixed point by sine transforms (boundary values set by the reference's rule)."""
+        self._chk(self.L.dft_poisson_direct(self.ctx, C.byref(g), C.c_int(rho.prec), _vp(rho.ptr), _vp(v.ptr)))
+
     def hartree_energy(self, g, rho, vh):
         e = C.c_double()
         self._chk(self.L.dft_hartree_energy(self.ctx, C.byref(g), C.c_int(rho.prec), _vp(rho.ptr), _vp(vh.ptr), C.byref(e)))
@@ -400,7 +404,7 @@ SCF_FP64, SCF_MIXED = 0, 1
 STOP_REFERENCE, STOP_DE = 0, 1
 INIT_BARE, INIT_HYDROGENIC = 0, 1
 PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC = 0, 1, 2
-POISSON_JACOBI, POISSON_CHEBYSHEV = 0, 1
+POISSON_JACOBI, POISSON_CHEBYSHEV, POISSON_DIRECT = 0, 1, 2
 
 
 class ScfConfig(C.Structure):

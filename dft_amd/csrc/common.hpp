@@ -66,6 +66,9 @@ inline int set_err(dft_ctx* c, int code, const std::string& msg) {
 int ensure_partials(dft_ctx* c, size_t n);
 // Destroy the rocBLAS handle (linalg.hip).
 void release_blas(dft_ctx* c);
+// The context's rocBLAS handle (a rocblas_handle), created on first use and
+// bound to the context stream (linalg.hip).
+int rocblas_for(dft_ctx* c, void** handle);
 // Grow ctx->scratch to at least bytes.
 int ensure_scratch(dft_ctx* c, size_t bytes);
 // Sum of n FP64 device values already written to ctx->partials -> host.

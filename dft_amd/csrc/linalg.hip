@@ -669,6 +669,13 @@ static int blas(dft_ctx* c, rocblas_handle* h) {
     return DFT_OK;
 }
 
+int rocblas_for(dft_ctx* c, void** handle) {
+    rocblas_handle h;
+    if (int rc = blas(c, &h)) return rc;
+    *handle = (void*)h;
+    return 0;
+}
+
 }  // namespace dft
 
 extern "C" {

@@ -11,6 +11,7 @@
 #include "common.hpp"
 
 #include <hip/hip_cooperative_groups.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -179,7 +180,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_boundary(int nx, int ny, int nz, double xmin, double xmax, double ymin,
                                                   double ymax, double zmin, double zmax, double dx, double dy,
                                                   double dz, double q, T* __restrict__ a,
-                                                  T* __restrict__ b) {
+                                                  T* __restrict__ b /* may be null */) {
     const int64_t n = (int64_t)nx * ny * nz;
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (p >= n) return;
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(256) void k_boundary(int nx, int ny, int nz, double
         if (r > 0.0) val = q / r;
     }
     a[p] = (T)val;
-    b[p] = (T)val;
+    if (b) b[p] = (T)val;
 }
 
 template <typename T>
@@ -438,6 +439,131 @@ int poisson_cheb_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t ma
     return DFT_OK;
 }
 
+// ---- direct solve of the Jacobi fixed point -------------------------------
+// The interior equations of the Jacobi iteration at convergence are
+// Lap_h V = -4 pi rho with the boundary values fixed (HartreePotentialMolecule.cpp:
+// the update V = (sum_a (V_a+ + V_a-)/h_a^2 + 4 pi rho) / (2 sum_a 1/h_a^2)).
+// With zero boundary values moved to the right-hand side, the interior
+// operator is separable and each 1D factor is diagonalised by the
+// orthogonal DST-I matrix S[p][i] = sqrt(2/(n+1)) sin(pi (p+1)(i+1)/(n+1)),
+// eigenvalue -4 sin^2(pi (p+1) / (2 (n+1))) / h^2.  So
+//   V = S_x S_y S_z [ (S_x S_y S_z b) / (mu_x + mu_y + mu_z) ],
+// each 1D transform a dense FP64 GEMM along one axis (rocBLAS, MFMA):
+// 2 n flops per point per transform, 12 n per solve (n <= a few hundred).
+__global__ __launch_bounds__(256) void k_dst_matrix(int n, double* __restrict__ S) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int p = (int)(e % n), i = (int)(e / n);
+    const int64_t t = ((int64_t)(p + 1) * (i + 1)) % (2LL * (n + 1));  // exact argument reduction
+    S[e] = sqrt(2.0 / (n + 1)) * sinpi((double)t / (double)(n + 1));
+}
+
+// b (interior points, x fastest) = -4 pi rho - (boundary neighbours) / h^2
+template <typename T>
+__global__ __launch_bounds__(256) void k_dst_rhs(int nx, int ny, int nz, double ix2, double iy2, double iz2,
+                                                 const T* __restrict__ rho, const T* __restrict__ v,
+                                                 double* __restrict__ b) {
+    const int mx = nx - 2, my = ny - 2, mz = nz - 2;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)mx * my * mz) return;
+    const int i = (int)(e % mx) + 1, j = (int)((e / mx) % my) + 1, k = (int)(e / ((int64_t)mx * my)) + 1;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int64_t p = k * sz + j * sy + i;
+    double r = -4.0 * kPi * (double)rho[p];
+    if (i == 1) r -= (double)v[p - 1] * ix2;
+    if (i == nx - 2) r -= (double)v[p + 1] * ix2;
+    if (j == 1) r -= (double)v[p - sy] * iy2;
+    if (j == ny - 2) r -= (double)v[p + sy] * iy2;
+    if (k == 1) r -= (double)v[p - sz] * iz2;
+    if (k == nz - 2) r -= (double)v[p + sz] * iz2;
+    b[e] = r;
+}
+
+// divide the transformed right-hand side by the operator's eigenvalues
+__global__ __launch_bounds__(256) void k_dst_scale(int mx, int my, int mz, double ix2, double iy2, double iz2,
+                                                   double* __restrict__ B) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)mx * my * mz) return;
+    const int p = (int)(e % mx), q = (int)((e / mx) % my), r = (int)(e / ((int64_t)mx * my));
+    const double sx = sinpi((double)(p + 1) / (2.0 * (mx + 1)));
+    const double sy = sinpi((double)(q + 1) / (2.0 * (my + 1)));
+    const double sz = sinpi((double)(r + 1) / (2.0 * (mz + 1)));
+    const double mu = -4.0 * (sx * sx * ix2 + sy * sy * iy2 + sz * sz * iz2);
+    B[e] /= mu;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dst_store(int nx, int ny, int nz, const double* __restrict__ U,
+                                                   T* __restrict__ v) {
+    const int mx = nx - 2, my = ny - 2, mz = nz - 2;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)mx * my * mz) return;
+    const int i = (int)(e % mx) + 1, j = (int)((e / mx) % my) + 1, k = (int)(e / ((int64_t)mx * my)) + 1;
+    v[((int64_t)k * ny + j) * nx + i] = (T)U[e];
+}
+
+// out = the 1D transform of in along x, y, z in turn (S symmetric, so the
+// same three products apply the inverse)
+int dst3(dft_ctx* c, rocblas_handle h, int mx, int my, int mz, const double* Sx, const double* Sy, const double* Sz,
+         double* a, double* b) {
+    const double one = 1.0, zero = 0.0;
+    rocblas_status st;
+    st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, mx, my * mz, mx, &one, Sx, mx, a, mx, &zero,
+                       b, mx);
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "dft_poisson_direct: x transform failed");
+    st = rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, mx, my, my, &one, b, mx,
+                                       (rocblas_stride)mx * my, Sy, my, 0, &zero, a, mx, (rocblas_stride)mx * my, mz);
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "dft_poisson_direct: y transform failed");
+    st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, mx * my, mz, mz, &one, a, mx * my, Sz, mz,
+                       &zero, b, mx * my);
+    if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "dft_poisson_direct: z transform failed");
+    return DFT_OK;
+}
+
+template <typename T>
+int poisson_direct_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v) {
+    const int64_t n = grid_points(g);
+    const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
+    // total charge and the boundary values (HartreePotentialMolecule.cpp:17-35 and the boundary rule)
+    const int nblk = (int)std::min<int64_t>((n + 255) / 256, 4 * (int64_t)c->num_cus);
+    if (int rc = ensure_partials(c, (size_t)nblk + 2)) return rc;
+    k_charge<T><<<nblk, 256, 0, c->stream>>>(rho, n, c->partials);
+    DFT_CHECK_LAUNCH(c);
+    double q = 0.0;
+    if (int rc = finish_sum(c, nblk, &q)) return rc;
+    const double qd = q * (g->dx * g->dy * g->dz);
+    k_boundary<T><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(nx, ny, nz, g->xmin, g->xmax, g->ymin, g->ymax,
+                                                                        g->zmin, g->zmax, g->dx, g->dy, g->dz, qd, v,
+                                                                        (T*)nullptr);
+    DFT_CHECK_LAUNCH(c);
+    const int mx = nx - 2, my = ny - 2, mz = nz - 2;
+    if (mx < 1 || my < 1 || mz < 1) return DFT_OK;  // no interior
+    const int64_t ni = (int64_t)mx * my * mz;
+    const size_t need = sizeof(double) * ((size_t)2 * ni + (size_t)mx * mx + (size_t)my * my + (size_t)mz * mz);
+    if (int rc = ensure_scratch(c, need)) return rc;
+    double* w1 = (double*)c->scratch;
+    double* w2 = w1 + ni;
+    double* Sx = w2 + ni;
+    double* Sy = Sx + (size_t)mx * mx;
+    double* Sz = Sy + (size_t)my * my;
+    k_dst_matrix<<<(unsigned)(((int64_t)mx * mx + 255) / 256), 256, 0, c->stream>>>(mx, Sx);
+    k_dst_matrix<<<(unsigned)(((int64_t)my * my + 255) / 256), 256, 0, c->stream>>>(my, Sy);
+    k_dst_matrix<<<(unsigned)(((int64_t)mz * mz + 255) / 256), 256, 0, c->stream>>>(mz, Sz);
+    const double ix2 = 1.0 / (g->dx * g->dx), iy2 = 1.0 / (g->dy * g->dy), iz2 = 1.0 / (g->dz * g->dz);
+    const unsigned gb = (unsigned)((ni + 255) / 256);
+    k_dst_rhs<T><<<gb, 256, 0, c->stream>>>(nx, ny, nz, ix2, iy2, iz2, rho, v, w1);
+    DFT_CHECK_LAUNCH(c);
+    rocblas_handle h;
+    if (int rc = dft::rocblas_for(c, reinterpret_cast<void**>(&h))) return rc;
+    if (int rc = dst3(c, h, mx, my, mz, Sx, Sy, Sz, w1, w2)) return rc;  // transform -> w2
+    k_dst_scale<<<gb, 256, 0, c->stream>>>(mx, my, mz, ix2, iy2, iz2, w2);
+    DFT_CHECK_LAUNCH(c);
+    if (int rc = dst3(c, h, mx, my, mz, Sx, Sy, Sz, w2, w1)) return rc;  // inverse -> w1
+    k_dst_store<T><<<gb, 256, 0, c->stream>>>(nx, ny, nz, w1, v);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -454,6 +580,16 @@ int dft_poisson(dft_ctx* c, const dft_grid* g, int prec, const void* rho, void* 
         return poisson_t<float>(c, g, (const float*)rho, (float*)v, max_sweeps, tol, zero_start, sweeps_per_launch,
                                 sweeps_done, last_maxdiff);
     return set_err(c, DFT_EINVAL, "dft_poisson: bad precision");
+}
+
+int dft_poisson_direct(dft_ctx* c, const dft_grid* g, int prec, const void* rho, void* v) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, g && rho && v && g->nx >= 1 && g->ny >= 1 && g->nz >= 1 && g->nx <= 4096 && g->ny <= 4096 &&
+                       g->nz <= 4096,
+                "dft_poisson_direct: bad arguments");
+    if (prec == DFT_F64) return poisson_direct_t<double>(c, g, (const double*)rho, (double*)v);
+    if (prec == DFT_F32) return poisson_direct_t<float>(c, g, (const float*)rho, (float*)v);
+    return set_err(c, DFT_EINVAL, "dft_poisson_direct: bad precision");
 }
 
 int dft_poisson_chebyshev(dft_ctx* c, const dft_grid* g, int prec, const void* rho, void* v, int64_t max_sweeps,

@@ -149,7 +149,9 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     auto poisson = [&](Fields& f, const void* rho, void* v) {
         int64_t sw = 0;
         double md = 0;
-        if (cfg_.poisson_method == DFT_POISSON_CHEBYSHEV) {
+        if (cfg_.poisson_method == DFT_POISSON_DIRECT) {
+            check(c_, dft_poisson_direct(c_, &g_, f.prec, rho, v), "dft_poisson_direct");
+        } else if (cfg_.poisson_method == DFT_POISSON_CHEBYSHEV) {
             // warm start from the last V_H; converged to tol every call
             const double tol = std::max(cfg_.poisson_tol > 0 ? cfg_.poisson_tol : 1.0e-8, f.prec == DFT_F64 ? 0.0 : 1.0e-5);
             // O(n log(1/tol)) sweeps suffice; the cap only guards against a non-converging setup

@@ -30,7 +30,7 @@ enum { DFT_INIT_BARE = 0, DFT_INIT_HYDROGENIC = 1 };
  * the diagonal one as fallback; KINETIC: 4 damped-Jacobi steps on
  * (-1/2 Lap + 1) t = r (a smoothing correction, no reference counterpart). */
 enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1, DFT_PRECOND_KINETIC = 2 };
-enum { DFT_POISSON_JACOBI = 0, DFT_POISSON_CHEBYSHEV = 1 };
+enum { DFT_POISSON_JACOBI = 0, DFT_POISSON_CHEBYSHEV = 1, DFT_POISSON_DIRECT = 2 };
 
 typedef struct {
     int functional;        /* DFT_XC_PZ81 / DFT_XC_PBE96 */
@@ -42,7 +42,7 @@ typedef struct {
     int max_iterations;    /* SCF steps; 0: 250 (DFTConstants.h:15) */
     int poisson_fixed;     /* 0: reference rule (tol 1e-8, cap 10000, zero start); k > 0: k warm sweeps */
     int poisson_batch;     /* sweeps between host checks; 0: 64 */
-    int poisson_method;    /* DFT_POISSON_JACOBI (the reference's sweeps) or DFT_POISSON_CHEBYSHEV */
+    int poisson_method;    /* DFT_POISSON_JACOBI (the reference's sweeps), DFT_POISSON_CHEBYSHEV or DFT_POISSON_DIRECT */
     double poisson_tol;    /* Chebyshev stop: max|dV| per sweep (0: 1e-8 FP64; FP32 uses >= 1e-5) */
     int stop_rule;         /* DFT_STOP_REFERENCE (dE, dRho, KS triple) or DFT_STOP_DE */
     double energy_tol;     /* |dE| threshold for DFT_STOP_DE */

@@ -124,6 +124,12 @@ int dft_poisson(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, void
  * after the batch (sweeps between host checks) in which max|dV| < tol. */
 int dft_poisson_chebyshev(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, void* v, int64_t max_sweeps,
                           double tol, int zero_start, int batch, int64_t* sweeps_done, double* last_maxdiff);
+/* The same fixed point solved directly: interior equations Lap_h V = -4 pi rho
+ * with the boundary values of the reference's rule, diagonalised by 1D
+ * discrete sine transforms (dense FP64 GEMMs along each axis, the operator's
+ * eigenvalues divided out).  The converged limit of dft_poisson's sweeps to
+ * rounding; FP64 internally for either precision of rho/v. */
+int dft_poisson_direct(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, void* v);
 /* Hartree energy 1/2 sum rho V dV (HartreePotentialMolecule.cpp:326-379). */
 int dft_hartree_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* rho, const void* vh, double* e_host);
 

@@ -121,6 +121,56 @@ def test_poisson_chebyshev_reaches_jacobi_fixed_point(dev, oracle, prec):
     assert sw < jsw / 5, (sw, jsw)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_poisson_direct_is_the_jacobi_fixed_point(dev, oracle, prec):
+    """Sine-transform solve against the oracle's Jacobi iteration (the
+    reference's splitting and boundary rule) run to convergence."""
+    from oracle.oracle import Grid
+    og = Grid(26, 22, 19, -3.9, 3.6, -3.1, 3.2, -2.8, 2.6)
+    xs, ys, zs = og.coords()
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    rho = (np.exp(-1.5 * ((X - 0.2) ** 2 + Y ** 2 + (Z + 0.1) ** 2)) * 0.7).ravel()
+    ref, _, jmd = oracle.poisson(og, rho, v0=np.zeros(og.size), max_sweeps=400000, tol=1e-15)
+    assert jmd < 1e-15
+    g = dgrid(og)
+    dt = np.float64 if prec == "f64" else np.float32
+    d_v = dev.array(np.full(og.size, 7.0), dtype=dt)  # every point is written
+    dev.poisson_direct(g, dev.array(rho, dtype=dt), d_v)
+    assert rel(d_v.download(), ref) <= (1e-12 if prec == "f64" else 2e-6)
+
+
+def test_poisson_direct_fixed_point_and_sine_mode(dev):
+    """At a larger anisotropic grid: one Jacobi sweep (the reference's update)
+    from the direct solution changes it by rounding only, and a discrete sine
+    mode source has the closed-form solution -4 pi rho / mu with
+    mu = -sum_a 4 sin^2(pi / (2 (n_a - 1))) / h_a^2 (zero boundary values)."""
+    import dft_amd
+    n = (96, 80, 72)
+    g = dft_amd.Grid.make(*n, -9.5, 9.5, -7.1, 7.4, -6.0, 6.6)
+    i, j, k = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    mode = (np.sin(np.pi * i / (n[0] - 1)) * np.sin(np.pi * j / (n[1] - 1)) * np.sin(np.pi * k / (n[2] - 1)))
+    rho = mode.transpose(2, 1, 0).ravel()  # x fastest
+    rho[np.abs(rho) < 1e-300] = 0.0
+    h = (g.dx, g.dy, g.dz)
+    mu = -sum(4.0 * np.sin(np.pi / (2.0 * (na - 1))) ** 2 / ha ** 2 for na, ha in zip(n, h))
+    v = dev.zeros((g.size,))
+    dev.poisson_direct(g, dev.array(rho), v)
+    got = v.download()
+    assert rel(got, -4.0 * np.pi * rho / mu) <= 1e-12
+    # a smooth, non-separable source: one Jacobi sweep is (nearly) the identity
+    xs = g.xmin + g.dx * np.arange(n[0])
+    ys = g.ymin + g.dy * np.arange(n[1])
+    zs = g.zmin + g.dz * np.arange(n[2])
+    Zc, Yc, Xc = np.meshgrid(zs, ys, xs, indexing="ij")
+    rho2 = (np.exp(-0.4 * ((Xc - 1.0) ** 2 + 2 * Yc ** 2 + (Zc + 0.5) ** 2)) + 0.5 * np.exp(-((Xc + 2) ** 2 + Yc ** 2 + Zc ** 2))).ravel()
+    d_rho = dev.array(rho2)
+    dev.poisson_direct(g, d_rho, v)
+    v0 = v.download()
+    sw, md = dev.poisson(g, d_rho, v, max_sweeps=1, tol=0.0, zero_start=False, batch=1)
+    assert sw == 1
+    assert md <= 1e-13 * np.max(np.abs(v0)), md
+
+
 @pytest.mark.parametrize("sweeps,nb", [(4, 5), (3, 2)])
 def test_precond_kinetic_steps(dev, oracle, sweeps, nb):
     """The fused kinetic smoothing steps equal the same damped-Jacobi

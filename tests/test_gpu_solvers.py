@@ -172,27 +172,30 @@ def test_davidson_many_states(oracle, case):
         assert err <= 3e-5 * np.max(np.abs(ref["eigenvalues"][:84])), err
 
 
-def test_scf_chebyshev_poisson_matches_converged_jacobi(oracle):
-    """SCF with Chebyshev-accelerated Poisson converges to the SCF whose
-    Jacobi Poisson is run to full convergence every call (5000 warm sweeps on
-    16^3), both under the reference's triple stop criterion (dE < 1e-11,
-    dRho < 1e-9, KS < 1e-7); also in mixed precision with the kinetic
-    correction (FP32 phase, then FP64 refinement to the same criterion)."""
+@pytest.mark.parametrize("method", ["chebyshev", "direct"])
+def test_scf_fast_poisson_matches_converged_jacobi(oracle, method):
+    """SCF with Chebyshev-accelerated or direct (sine-transform) Poisson
+    converges to the SCF whose Jacobi Poisson is run to full convergence every
+    call (5000 warm sweeps on 16^3), both under the reference's triple stop
+    criterion (dE < 1e-11, dRho < 1e-9, KS < 1e-7); also in mixed precision
+    with the kinetic correction (FP32 phase, then FP64 refinement to the same
+    criterion)."""
     import dft_amd
     from oracle.oracle import Grid
     og = Grid.centered(16, 0.42)
     nuc = [[1, -0.71, 0.02, 0.0], [1, 0.69, 0.0, 0.03]]
     ref = oracle.scf(og, nuc, 1, poisson_fixed=5000, stop_rule=0, init_guess=1)
     assert ref["converged"]
-    common = dict(functional=dft_amd.PZ81, poisson_method=dft_amd.POISSON_CHEBYSHEV, poisson_tol=1e-13,
+    pm = dft_amd.POISSON_CHEBYSHEV if method == "chebyshev" else dft_amd.POISSON_DIRECT
+    common = dict(functional=dft_amd.PZ81, poisson_method=pm, poisson_tol=1e-13,
                   stop_rule=dft_amd.STOP_REFERENCE, init_guess=dft_amd.INIT_HYDROGENIC, davidson_tol=1e-9,
                   max_iterations=250)
     r = dft_amd.scf_run(dgrid(og), nuc, **common)
-    print("cheb scf", r["total"], ref["total"], r["iterations"], ref["iterations"])
+    print(method, "scf", r["total"], ref["total"], r["iterations"], ref["iterations"])
     # the total-energy bar of the FP64 path (1e-7 Ha); both runs stop at
     # dRho < 1e-9, which leaves ~1e-8 Ha in this non-variational expression
     assert r["converged"] and abs(r["total"] - ref["total"]) <= 1e-7
     m = dft_amd.scf_run(dgrid(og), nuc, precision=dft_amd.SCF_MIXED, refine_iterations=250,
                         precond=dft_amd.PRECOND_KINETIC, **common)
-    print("cheb scf mixed", m["total"], m["iterations"], m["refine_iterations"])
+    print(method, "scf mixed", m["total"], m["iterations"], m["refine_iterations"])
     assert m["converged"] and abs(m["total"] - ref["total"]) <= 1e-7
