@@ -355,6 +355,8 @@ def main():
         r = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_MIXED, verbose=a.verbose, precond=pc, cnn=model)
         print(json.dumps({"mixed_seconds": time.perf_counter() - t, "energy": r["total"], "steps": r["iterations"],
                           "refine_steps": r["refine_iterations"], "davidson_iterations": r["davidson_iterations"],
+                          "block_passes": r["block_passes"], "block_fallbacks": r["block_fallbacks"],
+                          "happly": r["happly_columns"],
                           "phase": {k: r[f"seconds_{k}"] for k in ("poisson", "xc", "davidson", "other")}}))
         return
     # plain FP64 GPU path (the accuracy reference and the speedup denominator)

@@ -426,7 +426,7 @@ class ScfReport(C.Structure):
                 ("seconds_to_first_de", C.c_double), ("seconds_poisson", C.c_double), ("seconds_xc", C.c_double),
                 ("seconds_davidson", C.c_double), ("seconds_other", C.c_double), ("happly_columns", C.c_int64),
                 ("poisson_sweeps", C.c_int64), ("davidson_iterations", C.c_int64), ("cnn_applied", C.c_int64),
-                ("cnn_fallbacks", C.c_int64)]
+                ("cnn_fallbacks", C.c_int64), ("block_passes", C.c_int64), ("block_fallbacks", C.c_int64)]
 
 
 def _core_err(rc):

@@ -51,6 +51,10 @@ __global__ __launch_bounds__(256) void k_copy_sq(int m, const double* __restrict
 // NORM: part[j * gridDim.x + blockIdx.x] receives the workgroup's FP64 sum of
 // squares of the stored column j of Y (the residual norms of the Davidson
 // iteration come out of the same pass).
+//
+// Y may alias columns of A or A2 (an in-place basis update): a wave reads
+// every input row at its points before it writes its outputs there, and no
+// other wave touches those points.
 template <typename T>
 struct UpdMfma;
 template <>
@@ -74,9 +78,9 @@ struct alignas(sizeof(T) * VEC) Pack {
 constexpr int kUpdLdsBytes = 48 * 1024;
 
 template <typename T, int NJT, int VEC, bool NORM>
-__global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m1, int m, int k, T alpha, const T* __restrict__ A,
-                                                     int64_t lda, const T* __restrict__ A2, int64_t lda2,
-                                                     const T* __restrict__ Cm, int ldc, T beta, T* __restrict__ Y,
+__global__ __launch_bounds__(256) void k_update_mfma(int64_t n, int m1, int m, int k, T alpha, const T* A,
+                                                     int64_t lda, const T* A2, int64_t lda2,
+                                                     const T* __restrict__ Cm, int ldc, T beta, T* Y,
                                                      int64_t ldy, double* __restrict__ part) {
     using U = UpdMfma<T>;
     using Acc = typename U::Acc;
@@ -594,9 +598,9 @@ __global__ __launch_bounds__(kEigThreads) void k_syev_small(int m, int k, const 
 // KMAX run in further passes over A.  Used for FP32 blocks with k <= 64,
 // where rocBLAS' SGEMM tiles waste most of their width.
 template <typename T, int KMAX>
-__global__ __launch_bounds__(256) void k_update_valu(int64_t n, int m, int kc, T alpha, const T* __restrict__ A,
+__global__ __launch_bounds__(256) void k_update_valu(int64_t n, int m, int kc, T alpha, const T* A,
                                                      int64_t lda, const T* __restrict__ Cm, int ldc, T beta,
-                                                     T* __restrict__ Y, int64_t ldy) {
+                                                     T* Y, int64_t ldy) {
     const int64_t p0 = (int64_t)blockIdx.x * 512 + threadIdx.x, p1 = p0 + 256;
     const bool ok0 = p0 < n, ok1 = p1 < n;
     T acc0[KMAX], acc1[KMAX];
@@ -687,32 +691,49 @@ extern "C" {
 int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
                      const double* cm, int64_t ldc, double beta, void* y, int64_t ldy) {
     if (!c) return DFT_EINVAL;
-    DFT_REQUIRE(c, a && cm && y && n > 0 && m > 0 && k >= 0 && lda >= n && ldy >= n && ldc >= m, "dft_block_update: bad arguments");
+    DFT_REQUIRE(c, a && cm && y && n > 0 && m > 0 && k >= 0 && lda >= n && ldy >= n && ldc >= m &&
+                       (prec == DFT_F64 || prec == DFT_F32),
+                "dft_block_update: bad arguments");
     if (k == 0) return DFT_OK;
     rocblas_handle h;
     if (int rc = blas(c, &h)) return rc;
-    rocblas_status st;
-    if (prec == DFT_F64 && update_mfma_fits<double>(m, k)) {
-        return update_mfma<double>(c, n, m, m, k, alpha, (const double*)a, lda, nullptr, 0, cm, (int)ldc, beta, (double*)y, ldy);
-    } else if (prec == DFT_F64) {
+    const size_t es = prec == DFT_F64 ? sizeof(double) : sizeof(float);
+    // y may overlap a (an in-place update of a basis block): the MFMA kernel and the
+    // single-launch VALU kernel handle that; a library GEMM writes through a staging copy
+    const char* ab = static_cast<const char*>(a);
+    char* yb = static_cast<char*>(y);
+    const bool alias = yb < ab + es * (size_t)m * lda && ab < yb + es * (size_t)k * ldy;
+    const bool mfma = prec == DFT_F64 ? update_mfma_fits<double>(m, k) : update_mfma_fits<float>(m, k);
+    const bool valu = prec == DFT_F32 && !mfma && k <= 64 && m <= 512;
+    const bool staged = alias && !mfma && !valu;
+    const size_t off = prec == DFT_F32 ? (sizeof(float) * (size_t)m * k + 255) / 256 * 256 : 0;
+    const size_t need = off + (staged ? es * (size_t)n * k : 0);
+    if (need)
+        if (int rc = ensure_scratch(c, need)) return rc;
+    void* out = staged ? static_cast<void*>(static_cast<char*>(c->scratch) + off) : y;
+    const int64_t ldo = staged ? n : ldy;
+    if (staged && beta != 0.0)
+        DFT_HIP(c, hipMemcpy2DAsync(out, es * ldo, y, es * ldy, es * n, k, hipMemcpyDeviceToDevice, c->stream));
+    rocblas_status st = rocblas_status_success;
+    if (prec == DFT_F64) {
+        if (mfma)
+            return update_mfma<double>(c, n, m, m, k, alpha, (const double*)a, lda, nullptr, 0, cm, (int)ldc, beta,
+                                       (double*)y, ldy);
         st = rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &alpha, (const double*)a, lda,
-                              cm, ldc, &beta, (double*)y, ldy);
-    } else if (prec == DFT_F32) {
-        if (int rc = ensure_scratch(c, sizeof(float) * (size_t)m * k)) return rc;
+                              cm, ldc, &beta, (double*)out, ldo);
+    } else {
         float* c32 = (float*)c->scratch;
         k_to_f32<<<(unsigned)(((int64_t)m * k + 255) / 256), 256, 0, c->stream>>>(m, k, cm, ldc, c32);
         DFT_CHECK_LAUNCH(c);
         const float al = (float)alpha, be = (float)beta;
-        if (update_mfma_fits<float>(m, k))
-            return update_mfma<float>(c, n, m, m, k, al, (const float*)a, lda, nullptr, 0, c32, m, be, (float*)y, ldy);
-        if (k <= 64 && m <= 512)
-            return update_valu<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
+        if (mfma) return update_mfma<float>(c, n, m, m, k, al, (const float*)a, lda, nullptr, 0, c32, m, be, (float*)y, ldy);
+        if (valu) return update_valu<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
         st = rocblas_sgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &al, (const float*)a, lda, c32,
-                              m, &be, (float*)y, ldy);
-    } else {
-        return set_err(c, DFT_EINVAL, "dft_block_update: bad precision");
+                              m, &be, (float*)out, ldo);
     }
     if (st != rocblas_status_success) return set_err(c, DFT_EHIP, "rocblas gemm failed");
+    if (staged)
+        DFT_HIP(c, hipMemcpy2DAsync(y, es * ldy, out, es * ldo, es * n, k, hipMemcpyDeviceToDevice, c->stream));
     return DFT_OK;
 }
 

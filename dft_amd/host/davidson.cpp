@@ -28,6 +28,7 @@ BlockDavidson::BlockDavidson(dft_ctx* c, const dft_grid& g, int prec, int nstate
     theta_ = Buf(c, sizeof(double) * (size_t)mmax_);
     rn_ = Buf(c, sizeof(double) * (size_t)k_);
     Qprev_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
+    Cb_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
     G_.zero();
 }
 
@@ -90,7 +91,98 @@ int BlockDavidson::orthonormalise(void* src, int cnt, void* dst, void* tmp) {
     return cnt;
 }
 
-int BlockDavidson::restart(int m, int mprev) {
+int BlockDavidson::append_block(int m, int nt, DavidsonStats& st) {
+    const size_t colb = (size_t)n_ * es_;
+    char* V = static_cast<char*>(V_.get());
+    char* T = V + (size_t)m * colb;
+    const int mt = m + nt;
+    double* G1 = P_.as<double>();  // (m + nt) x nt, ld m + nt (P_ holds mmax x k)
+    // accuracy of one Gram entry relative to the column norms (FP64, split-BF16 / FP32)
+    const double eps_g = prec_ == DFT_F64 ? 2.0e-16 : 1.0e-7;
+    std::vector<double> g((size_t)mt * nt), L((size_t)nt * nt), Ri((size_t)nt * nt), C((size_t)mt * nt);
+    for (int pass = 0; pass < 2; ++pass) {
+        gram(mt, nt, V, T, G1, mt);
+        check(c_, dft_download(c_, g.data(), G1, sizeof(double) * g.size()), "dft_download");
+        auto G = [&](int i, int j) { return g[(size_t)j * mt + i]; };  // rows 0..m-1: V^T T; m..: T^T T
+        // S' = T^T T - (V^T T)^T (V^T T), the Gram matrix of the projected block;
+        // column norms^2 before (sn) and after (sp) the projection
+        std::vector<double> sn(nt), sp(nt);
+        bool ok = true;
+        for (int j = 0; j < nt; ++j) {
+            sn[j] = G(m + j, j);
+            ok = ok && sn[j] > 0.0 && std::isfinite(sn[j]);
+        }
+        for (int j = 0; j < nt && ok; ++j)
+            for (int i = j; i < nt; ++i) {
+                double s = 0.5 * (G(m + i, j) + G(m + j, i));
+                for (int l = 0; l < m; ++l) s -= G(l, i) * G(l, j);
+                L[(size_t)j * nt + i] = s;  // lower triangle, column-major
+                if (i == j) sp[j] = s;
+            }
+        // Cholesky S' = L L^T; a pivot not clearly above the Gram noise of its own
+        // column (entries are accurate to eps_g times the column norms) means dependence
+        for (int j = 0; j < nt && ok; ++j) {
+            double d = L[(size_t)j * nt + j];
+            for (int l = 0; l < j; ++l) d -= L[(size_t)l * nt + j] * L[(size_t)l * nt + j];
+            if (!(d > 64.0 * eps_g * sn[j])) {
+                ok = false;
+                break;
+            }
+            d = std::sqrt(d);
+            L[(size_t)j * nt + j] = d;
+            for (int i = j + 1; i < nt; ++i) {
+                double s = L[(size_t)j * nt + i];
+                for (int l = 0; l < j; ++l) s -= L[(size_t)l * nt + i] * L[(size_t)l * nt + j];
+                L[(size_t)j * nt + i] = s / d;
+            }
+        }
+        if (!ok) break;
+        // R = L^T; Ri = R^-1 (upper triangular), column by column; rs2 = ||D R^-1||_F^2
+        // with D = diag(column norms before the projection): ~nt for an independent
+        // block that kept its norms
+        std::fill(Ri.begin(), Ri.end(), 0.0);
+        double rs2 = 0.0;
+        for (int j = 0; j < nt; ++j) {
+            Ri[(size_t)j * nt + j] = 1.0 / L[(size_t)j * nt + j];
+            for (int i = j - 1; i >= 0; --i) {  // (R Ri)[i][j] = 0: sum_{l=i..j} R[i][l] Ri[l][j]
+                double s = 0.0;
+                for (int l = i + 1; l <= j; ++l) s += L[(size_t)i * nt + l] * Ri[(size_t)j * nt + l];
+                Ri[(size_t)j * nt + i] = -s / L[(size_t)i * nt + i];
+            }
+            for (int i = 0; i <= j; ++i) rs2 += sn[i] * Ri[(size_t)j * nt + i] * Ri[(size_t)j * nt + i];
+        }
+        // C = [-(V^T T) Ri ; Ri]
+        for (int j = 0; j < nt; ++j) {
+            for (int l = 0; l < m; ++l) {
+                double s = 0.0;
+                for (int q = 0; q <= j; ++q) s += G(l, q) * Ri[(size_t)j * nt + q];
+                C[(size_t)j * mt + l] = -s;
+            }
+            for (int i = 0; i < nt; ++i) C[(size_t)j * mt + m + i] = i <= j ? Ri[(size_t)j * nt + i] : 0.0;
+        }
+        check(c_, dft_upload(c_, Cb_.get(), C.data(), sizeof(double) * C.size()), "dft_upload");
+        check(c_, dft_block_update(c_, prec_, n_, mt, nt, 1.0, V, n_, Cb_.as<double>(), mt, 0.0, T, n_),
+              "dft_block_update");
+        ++st.block_passes;
+        // the block's loss of orthogonality after a pass is ~ eps_g ||D R^-1||^2: one pass
+        // is enough when every column kept at least half its norm and the block is well
+        // conditioned; otherwise a second pass on the (now nearly orthonormal) block
+        bool single = pass == 0 && rs2 <= 4.0 * nt;
+        for (int j = 0; j < nt && single; ++j) single = sp[j] >= 0.25 * sn[j];
+        if (single || pass == 1) return nt;
+    }
+    // dependent block: copy it out, project twice, then the rank-revealing orthonormalisation
+    ++st.block_fallbacks;
+    check(c_, dft_copy(c_, T_.get(), T, colb * nt), "dft_copy");
+    for (int pass = 0; pass < 2; ++pass) {
+        gram(m, nt, V, T_.get(), P_.as<double>(), mmax_);
+        check(c_, dft_block_update(c_, prec_, n_, m, nt, -1.0, V, n_, P_.as<double>(), mmax_, 1.0, T_.get(), n_),
+              "dft_block_update");
+    }
+    return orthonormalise(T_.get(), nt, T, R_.get());
+}
+
+int BlockDavidson::restart(int m, int mprev, int pmax) {
     // Coefficients (in the current basis V, m columns) of the Ritz block X
     // (= Q[:, :k]) and of the previous Ritz block (Qprev padded with zeros).
     const int k = k_;
@@ -119,6 +211,7 @@ int BlockDavidson::restart(int m, int mprev) {
                 for (int i = 0; i < m; ++i) v[i] -= d * Pc[(size_t)a * m + i];
             }
         }
+        if (kept >= pmax) break;
         double nn = 0;
         for (double x : v) nn += x * x;
         nn = std::sqrt(nn);
@@ -132,15 +225,15 @@ int BlockDavidson::restart(int m, int mprev) {
     double* S = P_.as<double>();  // mmax x k scratch for the coefficients
     if (kept > 0) {
         check(c_, dft_upload(c_, S, Pc.data(), sizeof(double) * (size_t)m * kept), "dft_upload");
-        // P in PV_, HP in R_ (both free at this point)
+        // P in PV_, HP in T_ (both free at this point; R_ still holds the residuals)
         check(c_, dft_block_update(c_, prec_, n_, m, kept, 1.0, V, n_, S, m, 0.0, PV_.get(), n_), "dft_block_update");
-        check(c_, dft_block_update(c_, prec_, n_, m, kept, 1.0, W, n_, S, m, 0.0, R_.get(), n_), "dft_block_update");
+        check(c_, dft_block_update(c_, prec_, n_, m, kept, 1.0, W, n_, S, m, 0.0, T_.get(), n_), "dft_block_update");
     }
     check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
     check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
     if (kept > 0) {
         check(c_, dft_copy(c_, V + (size_t)k * colb, PV_.get(), colb * kept), "dft_copy");
-        check(c_, dft_copy(c_, W + (size_t)k * colb, R_.get(), colb * kept), "dft_copy");
+        check(c_, dft_copy(c_, W + (size_t)k * colb, T_.get(), colb * kept), "dft_copy");
     }
     const int mn = k + kept;
     gram(mn, mn, V, W, G_.as<double>(), mmax_);
@@ -237,24 +330,22 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
         }
         if (it == opt.max_iter) break;
         // corrections for the unconverged states that must converge, in contiguous runs
+        std::vector<std::pair<int, int>> runs;
         int nt = 0;
         for (int j = 0; j < nact;) {
-            if (rn_h_[j] < opt.tol) { ++j; continue; }
+            if (rn_h_[j] < opt.tol) {
+                ++j;
+                continue;
+            }
             int j1 = j;
             while (j1 < nact && rn_h_[j1] >= opt.tol) ++j1;
-            P.apply(c_, g_, prec_, veff, X_.get(), n_, static_cast<char*>(R_.get()) + j * colb, n_, j, j1 - j, theta + j,
-                    static_cast<char*>(T_.get()) + (size_t)nt * colb, n_);
+            runs.push_back({j, j1});
             nt += j1 - j;
             j = j1;
         }
-        // block Gram-Schmidt against the basis, twice
-        for (int pass = 0; pass < 2; ++pass) {
-            gram(m, nt, V, T_.get(), Pm, mmax_);
-            check(c_, dft_block_update(c_, prec_, n_, m, nt, -1.0, V, n_, Pm, mmax_, 1.0, T_.get(), n_), "dft_block_update");
-        }
-        if (m + nt > mmax_) {  // thick restart on [X, P]
+        if (m + nt > mmax_) {  // thick restart on [X, P] before the new block is formed
             ritz_block(true);
-            m = mprev > 0 ? restart(m, mprev) : 0;
+            m = mprev > 0 ? restart(m, mprev, std::max(0, mmax_ - k - nt)) : 0;
             if (m == 0) {
                 check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
                 check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
@@ -268,7 +359,24 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             check(c_, dft_copy(c_, Qprev_.get(), Q, sizeof(double) * (size_t)mmax_ * k), "dft_copy");
             mprev = m;
         }
-        const int kept = orthonormalise(T_.get(), nt, V + (size_t)m * colb, R_.get());
+        if (m + nt > mmax_) {  // a subspace limit below 3 k: only the lowest states' corrections fit
+            int room = mmax_ - m;
+            for (auto& r : runs) {
+                r.second = std::min(r.second, r.first + std::max(room, 0));
+                room -= r.second - r.first;
+            }
+            nt = mmax_ - m;
+        }
+        if (nt <= 0) break;
+        // the corrections go straight to V[:, m : m + nt]
+        int off = 0;
+        for (const auto& r : runs) {
+            if (r.second <= r.first) continue;
+            P.apply(c_, g_, prec_, veff, X_.get(), n_, static_cast<char*>(R_.get()) + r.first * colb, n_, r.first,
+                    r.second - r.first, theta + r.first, V + (size_t)(m + off) * colb, n_);
+            off += r.second - r.first;
+        }
+        const int kept = append_block(m, nt, st);
         if (kept == 0) break;  // no new direction: stagnation
         hpsi(veff, V + (size_t)m * colb, W + (size_t)m * colb, kept);
         st.happly_columns += kept;

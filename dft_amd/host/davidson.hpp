@@ -7,9 +7,12 @@
 //   R = W Q_k - V Q_k theta in one pass over [W | V], norms in the same pass;
 //   the Ritz block X = V Q_k (and HX = W Q_k) is formed only where it is
 //   read: before a restart, for a preconditioner that reads X, and at exit
-//   T = P(R) for unconverged columns (P = diagonal or learned preconditioner)
-//   T <- T - V (V^T T) twice, then CholQR twice, appended to V;
-//   restart to V = X when m would exceed the subspace limit.
+//   restart to V = [X, P] when the new block would exceed the subspace limit;
+//   T = P(R) for unconverged columns, written straight after V's columns;
+//   T orthonormalised against V and itself in place by Pythagorean block
+//   Gram-Schmidt: G = [V T]^T T in one pass, S' = T^T T - (V^T T)^T (V^T T)
+//   = R^T R, T <- [V T] [-(V^T T) R^-1 ; R^-1] in one pass (a second pass
+//   only when the first lost accuracy).
 // Restart keeps momentum (LOBPCG-style thick restart): the new basis is
 // [X, P] with P the previous Ritz block orthogonalised against X, formed in
 // the coefficient space of V (no extra H applications).  Corrections are
@@ -89,6 +92,8 @@ struct DavidsonStats {
     int64_t happly_columns = 0;
     double max_residual = 0.0;
     bool converged = false;
+    int block_passes = 0;     // Pythagorean Gram-Schmidt passes over new blocks
+    int block_fallbacks = 0;  // blocks orthonormalised by the rank-revealing path
 };
 
 class BlockDavidson {
@@ -116,8 +121,16 @@ class BlockDavidson {
     // Orthonormalise `cnt` columns at src into dst (CholQR twice, with an
     // eigenvalue filter if the block is rank deficient); returns kept count.
     int orthonormalise(void* src, int cnt, void* dst, void* tmp);
-    // Thick restart: V <- [X, P], W <- [HX, HP]; returns the new basis size.
-    int restart(int m, int mprev);
+    // Thick restart: V <- [X, P], W <- [HX, HP] with at most pmax columns of
+    // P; returns the new basis size.
+    int restart(int m, int mprev, int pmax);
+    // The nt correction columns at V[:, m:m+nt] made orthonormal and
+    // orthogonal to V[:, :m], in place: Pythagorean block Gram-Schmidt (one
+    // Gram pass [V T]^T T and one update per pass, a second pass only when
+    // the first lost accuracy); the rank-revealing path when the block is
+    // (numerically) dependent.  Returns the number of columns now at
+    // V[:, m:...], 0 when nothing independent remains.
+    int append_block(int m, int nt, DavidsonStats& st);
 
     dft_ctx* c_;
     dft_grid g_;
@@ -125,7 +138,7 @@ class BlockDavidson {
     int64_t n_;
     size_t es_;
     Buf V_, W_, X_, HX_, R_, T_, PV_;
-    Buf G_, Q_, P_, S_, theta_, rn_, Qprev_;
+    Buf G_, Q_, P_, S_, theta_, rn_, Qprev_, Cb_;
     std::vector<double> theta_h_, rn_h_;
     int gram_mode_ = DFT_GRAM_F64;
 };

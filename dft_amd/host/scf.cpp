@@ -139,6 +139,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             o.tol = std::max(o.tol, std::min({1.0e-3, 0.1 * dd_prev, 0.3 * std::sqrt(etol)}));
         DavidsonStats st = D.solve(veff, have_prev, o, *precond);
         rep.happly_columns += st.happly_columns;
+        rep.block_passes += st.block_passes;
+        rep.block_fallbacks += st.block_fallbacks;
         rep.davidson_iterations += st.iterations;
         if (!st.converged && f.prec == DFT_F64 && st.max_residual > 1e3 * o.tol)
             throw std::runtime_error("El orbital molecular no convergio. residual = " + std::to_string(st.max_residual));

@@ -71,6 +71,8 @@ typedef struct {
     int64_t poisson_sweeps;
     int64_t davidson_iterations;
     int64_t cnn_applied, cnn_fallbacks;
+    int64_t block_passes;     /* Davidson block Gram-Schmidt passes (one or two per new block) */
+    int64_t block_fallbacks;  /* blocks that needed the rank-revealing orthonormalisation */
 } dft_scf_report;
 
 const char* dft_core_last_error(void);

@@ -171,7 +171,8 @@ int dft_nuclear_potential(dft_ctx* ctx, const dft_grid* g, int prec, int natoms,
 int dft_gram(dft_ctx* ctx, int mode, int64_t n, int ma, int mb, double alpha, const void* a, int64_t lda,
              const void* b, int64_t ldb, double beta, double* out, int64_t ldo);
 /* Block update Y = alpha * A C + beta * Y: A N x m (prec), C m x k FP64
- * device column-major (ldc), Y N x k (prec). */
+ * device column-major (ldc), Y N x k (prec).  Y may overlap A (an in-place
+ * update of the last columns of a basis block). */
 int dft_block_update(dft_ctx* ctx, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
                      const double* c, int64_t ldc, double beta, void* y, int64_t ldy);
 /* Per-column scaling y[:,j] = x[:,j] * s[j] (s device FP64, nb values);

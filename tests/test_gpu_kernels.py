@@ -359,6 +359,29 @@ def test_ritz_residual(dev, n, m, k):
             np.sum(ref ** 2, axis=1))
 
 
+@pytest.mark.parametrize("n,m,k,path", [(70000, 40, 13, "mfma"), (70001, 29, 8, "mfma"), (30000, 300, 40, "valu"),
+                                        (30000, 300, 80, "library"), (30000, 200, 40, "library")])
+def test_block_update_in_place(dev, n, m, k, path):
+    """Y = the last k columns of A, overwritten with A C (the Davidson's
+    in-place block orthonormalisation), on each kernel path: the MFMA and
+    single-launch VALU kernels in place, the library GEMM through a staging
+    copy.  (FP32 VALU needs k <= 64 with C^T beyond the MFMA LDS budget.)"""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(n + k)
+    A = rng.standard_normal((m, n))
+    Cm = rng.standard_normal((k, m))
+    ref = Cm @ A
+    for dt, tol in ((np.float32, 1e-5), (np.float64, 1e-13)):
+        if path == "valu" and dt == np.float64:
+            continue  # no FP64 VALU kernel: the library path
+        dA = dev.array(A, dtype=dt)
+        y = SimpleNamespace(ptr=dA.col(m - k, n), prec=dA.prec)
+        dev.block_update(dA, dev.array(Cm), y, n, m, k, alpha=1.0, beta=0.0)
+        got = dA.download()
+        assert rel(got[m - k:], ref) <= tol * np.sqrt(m)
+        assert np.array_equal(got[:m - k], A[:m - k].astype(dt))  # the leading columns are untouched
+
+
 def _sym_cases():
     rng = np.random.default_rng(5)
     out = []
