@@ -209,7 +209,31 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
         rows_tag[0] = f"update_{tag}"
         ms = timed(lambda: dev.block_update(V, C, out, n, m, nb))
         rows.append(dict(kernel=f"update_{tag}", ms=ms, bytes=n * (m + nb) * es, flops=2.0 * n * m * nb, note=f"{m}x{nb}"))
+        # the Davidson residual [W | V] [Q ; -Q theta] with its norms: one pass over 2m columns
+        theta = dev.array(np.linspace(-1.0, 1.0, nb))
+        rn = dev.zeros((nb,))
+        rows_tag[0] = f"ritz_residual_{tag}"
+        ms = timed(lambda: dev.ritz_residual(V, V, C, theta, out, rn, n, m, nb))
+        rows.append(dict(kernel=f"ritz_residual_{tag}", ms=ms, bytes=n * (2 * m + nb) * es, flops=4.0 * n * m * nb,
+                         note=f"{m}x{nb}"))
         del V, G, C
+        # Poisson at the Jacobi fixed point by sine transforms (FP64 inside): 6 dense
+        # transforms of 2 n_a flops per interior point each, ~144 B per point moved
+        ni = (grid.nx - 2) * (grid.ny - 2) * (grid.nz - 2)
+        rows_tag[0] = f"poisson_direct_{tag}"
+        ms = timed(lambda: dev.poisson_direct(grid, rho, vh), reps=3)
+        rows.append(dict(kernel=f"poisson_direct_{tag}", ms=ms, bytes=144.0 * ni,
+                         flops=4.0 * (grid.nx + grid.ny + grid.nz - 6) * ni))
+    # the Rayleigh-Ritz eigensolve at the Davidson's subspace limit (FP64, k lowest pairs)
+    k = w["nstates"]
+    mm = max(3 * k, k + 16)
+    rng = np.random.default_rng(1)
+    M = rng.standard_normal((mm, mm))
+    A = dev.array(((M + M.T) / 2 + np.diag(np.linspace(-10, 50, mm))).T.copy())
+    ew, ez = dev.array(shape=(mm,)), dev.array(shape=(mm, mm))
+    rows_tag[0] = "syev_lowest"
+    ms = timed(lambda: dev.syev_lowest(A, mm, k, ew, ez))
+    rows.append(dict(kernel="syev_lowest", ms=ms, bytes=8.0 * mm * mm, flops=None, note=f"m={mm}, k={k}"))
     for r in rows:
         r["GBps"] = r["bytes"] / r["ms"] / 1e6
         r["hbm_frac"] = r["GBps"] / HBM_PEAK_GBS

@@ -292,10 +292,10 @@ class Device:
         self._chk(self.L.dft_syev(self.ctx, C.c_int(m), _vp(a.ptr), C.c_int64(m), _vp(w.ptr)))
         return w
 
-    def syev_lowest(self, a, m, k):
+    def syev_lowest(self, a, m, k, w=None, z=None):
         """k lowest eigenpairs of the symmetric m x m device matrix a: (w[:k], z[:, :k])."""
-        w = self.array(shape=(m,))
-        z = self.array(shape=(m, m))
+        w = w if w is not None else self.array(shape=(m,))
+        z = z if z is not None else self.array(shape=(m, m))
         self._chk(self.L.dft_syev_lowest(self.ctx, C.c_int(m), C.c_int(k), _vp(a.ptr), C.c_int64(m), _vp(z.ptr),
                                          C.c_int64(m), _vp(w.ptr)))
         return w, z
