@@ -307,6 +307,13 @@ class Device:
         self._chk(self.L.dft_seed_vectors(self.ctx, C.byref(g), C.c_int(out.prec), C.c_int(first), C.c_int(count),
                                           _vp(out.ptr), C.c_int64(g.size)))
 
+    def precond_kinetic_cheb(self, g, r, t, nb, sigma=1.0, sweeps=8):
+        tmp = DeviceArray(self, (nb, g.size), r.dtype)
+        self._chk(self.L.dft_precond_kinetic_cheb(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma), C.c_int(sweeps),
+                                                  _vp(r.ptr), C.c_int64(g.size), C.c_int(nb), _vp(t.ptr),
+                                                  C.c_int64(g.size), _vp(tmp.ptr)))
+        tmp.free()
+
     def precond_kinetic(self, g, r, t, nb, sigma=1.0, omega=2.0 / 3.0, sweeps=4):
         tmp = DeviceArray(self, (nb, g.size), r.dtype)
         self._chk(self.L.dft_precond_kinetic(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma), C.c_double(omega),
@@ -403,7 +410,7 @@ class Device:
 SCF_FP64, SCF_MIXED = 0, 1
 STOP_REFERENCE, STOP_DE = 0, 1
 INIT_BARE, INIT_HYDROGENIC = 0, 1
-PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC = 0, 1, 2
+PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC, PRECOND_KINETIC_CHEB = 0, 1, 2, 3
 POISSON_JACOBI, POISSON_CHEBYSHEV, POISSON_DIRECT = 0, 1, 2
 
 
@@ -416,7 +423,8 @@ class ScfConfig(C.Structure):
                 ("poisson_tol", C.c_double), ("stop_rule", C.c_int),
                 ("energy_tol", C.c_double), ("init_guess", C.c_int), ("davidson_tol", C.c_double),
                 ("davidson_max_iter", C.c_int), ("refine_iterations", C.c_int), ("precond", C.c_int),
-                ("cnn", C.c_void_p), ("verbose", C.c_int)]
+                ("cnn", C.c_void_p), ("verbose", C.c_int), ("precond_sweeps", C.c_int),
+                ("precond_shift", C.c_double)]
 
 
 class ScfReport(C.Structure):
@@ -439,13 +447,14 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
             max_iterations=250, poisson_fixed=0, poisson_batch=64, poisson_method=POISSON_JACOBI, poisson_tol=0.0,
             stop_rule=STOP_REFERENCE, energy_tol=1e-5,
             init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
-            precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False):
+            precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False, precond_sweeps=0, precond_shift=0.0):
     """Run the device-resident SCF; returns a dict of results."""
     L = core_lib()
     nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
                     poisson_fixed, poisson_batch, poisson_method, poisson_tol, stop_rule, energy_tol, init_guess, davidson_tol,
-                    davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0)
+                    davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0,
+                    precond_sweeps, precond_shift)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
     eig = np.zeros(ne + 2)

@@ -14,6 +14,9 @@
 // column bases, which keeps the kernel small enough for several waves per SIMD.
 #include "common.hpp"
 
+#include <cmath>
+#include <type_traits>
+
 #include <algorithm>
 #include <vector>
 
@@ -45,14 +48,18 @@ struct Coef {
     T cx, cy, cz, cc;  // 1/dx^2, 1/dy^2, 1/dz^2, -2(1/dx^2+1/dy^2+1/dz^2)
 };
 
-// SMOOTH: instead of H psi, one damped-Jacobi step of the kinetic smoothing
-// correction on psi = t: out = t + cf (r - (-1/2 Lap t + sigma t)), with r in
-// `R` (same layout as psi) and no V_eff read.
-template <typename T, int TY, int NB, bool SMOOTH = false>
+// SM = 0: H psi.  SM = 1, 2: instead of H psi, one step of the kinetic
+// smoothing correction on psi = t with r in `R` (same layout as psi) and no
+// V_eff read: J(t) = t + cf (r - (-1/2 Lap t + sigma t)), stored as
+// out = a J(t) (SM = 1) or out = a J(t) + b out (SM = 2: out holds the
+// previous iterate, read before it is overwritten: the Chebyshev recurrence).
+template <typename T, int TY, int NB, int SM = 0>
 __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, int nzc, Coef<T> co,
                                               const T* __restrict__ V, const T* __restrict__ psi, int64_t ld,
                                               T* __restrict__ out, int64_t ldo, const T* __restrict__ R = nullptr,
-                                              int64_t ldr = 0, T sigma = T(0), T cf = T(0)) {
+                                              int64_t ldr = 0, T sigma = T(0), T cf = T(0), T a = T(1),
+                                              T b = T(0)) {
+    constexpr bool SMOOTH = SM != 0;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t tile = xcd_tile(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z),
                                   (int64_t)gridDim.x * gridDim.y * gridDim.z);
@@ -133,7 +140,9 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
                 if (xin && yin[r]) {
                     if constexpr (SMOOTH) {
                         const T rv = R[(int64_t)(grp * NB + o) * ldr + kp + roff[r]];
-                        q[kp + roff[r]] = c + cf * (rv - (kin + pot));
+                        const T jt = c + cf * (rv - (kin + pot));
+                        if constexpr (SM == 2) q[kp + roff[r]] = a * jt + b * q[kp + roff[r]];
+                        else q[kp + roff[r]] = a * jt;
                     } else {
                         q[kp + roff[r]] = kin + pot;
                     }
@@ -148,9 +157,10 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
     }
 }
 
-template <typename T, int TY, int NB, bool SMOOTH = false>
+template <typename T, int TY, int NB, int SM = 0>
 int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int ngroups, T* out,
-                int64_t ldo, const T* R = nullptr, int64_t ldr = 0, T sigma = T(0), T cf = T(0)) {
+                int64_t ldo, const T* R = nullptr, int64_t ldr = 0, T sigma = T(0), T cf = T(0), T a = T(1),
+                T b = T(0)) {
     const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
     Coef<T> co;
     const double ix2 = 1.0 / (g->dx * g->dx), iy2 = 1.0 / (g->dy * g->dy), iz2 = 1.0 / (g->dz * g->dz);
@@ -166,7 +176,8 @@ int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
     nzc = (nz + zc - 1) / zc;
     DFT_REQUIRE(c, (int64_t)nzc * ngroups <= 65535 && gy <= 65535, "dft_hpsi: launch grid too large");
     dim3 grid(gx, gy, nzc * ngroups);
-    k_hpsi<T, TY, NB, SMOOTH><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo, R, ldr, sigma, cf);
+    k_hpsi<T, TY, NB, SM><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo, R, ldr, sigma, cf,
+                                                       a, b);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }
@@ -212,11 +223,62 @@ int kinetic_t(dft_ctx* c, const dft_grid* g, double sigma, double omega, int swe
         constexpr int TY = 4, NB = 2;
         const int ng = nb / NB;
         if (ng > 0)
-            if (int rc = launch_hpsi<T, TY, NB, true>(c, g, nullptr, src, lds, ng, dst, ldd, r, ldr, (T)sigma, (T)(omega / D)))
+            if (int rc = launch_hpsi<T, TY, NB, 1>(c, g, nullptr, src, lds, ng, dst, ldd, r, ldr, (T)sigma, (T)(omega / D)))
                 return rc;
         for (int o = ng * NB; o < nb; ++o)
-            if (int rc = launch_hpsi<T, TY, 1, true>(c, g, nullptr, src + (int64_t)o * lds, lds, 1, dst + (int64_t)o * ldd, ldd,
-                                                     r + (int64_t)o * ldr, ldr, (T)sigma, (T)(omega / D)))
+            if (int rc = launch_hpsi<T, TY, 1, 1>(c, g, nullptr, src + (int64_t)o * lds, lds, 1, dst + (int64_t)o * ldd, ldd,
+                                                  r + (int64_t)o * ldr, ldr, (T)sigma, (T)(omega / D)))
+                return rc;
+        std::swap(src, dst);
+        std::swap(lds, ldd);
+    }
+    if (src != t)
+        for (int o = 0; o < nb; ++o)
+            DFT_HIP(c, hipMemcpyAsync(t + (int64_t)o * ldt, src + (int64_t)o * lds, sizeof(T) * n, hipMemcpyDeviceToDevice,
+                                      c->stream));
+    return DFT_OK;
+}
+
+// Chebyshev-accelerated kinetic smoothing: the same steps J(t) = t + (1/D)
+// (r - A t), A = -1/2 Lap + sigma with zero ghosts, D = sum_a 1/h_a^2 + sigma
+// (the centre of A's spectrum, so I - A/D has eigenvalues in [-rho, rho] with
+// rho = sum_a cos(pi/(n_a+1))/h_a^2 / D), combined by the Chebyshev
+// semi-iteration t_1 = r/D, t_2 = w_2 J(t_1), t_{k+1} = w_{k+1} (J(t_k) -
+// t_{k-1}) + t_{k-1}, w_2 = 1/(1 - rho^2/2), w_{k+1} = 1/(1 - rho^2 w_k/4).
+// The ping-pong keeps t_{k-1} in the destination buffer, so a step reads
+// t_k, r and t_{k-1} and writes t_{k+1}.  The error of the solve of A t = r
+// falls by ~rho/(1 + sqrt(1 - rho^2)) per step instead of ~rho.
+template <typename T>
+int kinetic_cheb_t(dft_ctx* c, const dft_grid* g, double sigma, int sweeps, const T* r, int64_t ldr, int nb, T* t,
+                   int64_t ldt, T* tmp) {
+    const int64_t n = grid_points(g);
+    const double ix2 = 1.0 / (g->dx * g->dx), iy2 = 1.0 / (g->dy * g->dy), iz2 = 1.0 / (g->dz * g->dz);
+    const double D = ix2 + iy2 + iz2 + sigma;
+    const double kPiL = 3.141592653589793238462643383279502884;
+    const double rho = (std::cos(kPiL / (g->nx + 1)) * ix2 + std::cos(kPiL / (g->ny + 1)) * iy2 +
+                        std::cos(kPiL / (g->nz + 1)) * iz2) / D;
+    if (int rc = dft_smooth_step(c, sizeof(T) == 8 ? DFT_F64 : DFT_F32, n, nb, 1.0 / D, r, ldr, nullptr, n, t, ldt, 1))
+        return rc;
+    T* src = t;
+    T* dst = tmp;
+    int64_t lds = ldt, ldd = n;
+    double w = 1.0;
+    for (int s = 0; s < sweeps; ++s) {
+        w = s == 0 ? 1.0 / (1.0 - 0.5 * rho * rho) : 1.0 / (1.0 - 0.25 * rho * rho * w);
+        const T a = (T)w, b = (T)(1.0 - w), cf = (T)(1.0 / D);
+        constexpr int TY = 4, NB = 2;
+        const int ng = nb / NB;
+        // the first step has t_0 = 0: nothing to blend
+        auto step = [&](auto nbc, const T* in, int64_t li, int groups, T* out, int64_t lo, const T* rr) -> int {
+            constexpr int G = decltype(nbc)::value;
+            if (s == 0) return launch_hpsi<T, TY, G, 1>(c, g, nullptr, in, li, groups, out, lo, rr, ldr, (T)sigma, cf, a);
+            return launch_hpsi<T, TY, G, 2>(c, g, nullptr, in, li, groups, out, lo, rr, ldr, (T)sigma, cf, a, b);
+        };
+        if (ng > 0)
+            if (int rc = step(std::integral_constant<int, NB>(), src, lds, ng, dst, ldd, r)) return rc;
+        for (int o = ng * NB; o < nb; ++o)
+            if (int rc = step(std::integral_constant<int, 1>(), src + (int64_t)o * lds, lds, 1, dst + (int64_t)o * ldd, ldd,
+                              r + (int64_t)o * ldr))
                 return rc;
         std::swap(src, dst);
         std::swap(lds, ldd);
@@ -327,6 +389,21 @@ int dft_precond_kinetic(dft_ctx* c, const dft_grid* g, int prec, double sigma, d
     if (prec == DFT_F32)
         return kinetic_t<float>(c, g, sigma, omega, sweeps, (const float*)r, ldr, nb, (float*)t, ldt, (float*)tmp);
     return set_err(c, DFT_EINVAL, "dft_precond_kinetic: bad precision");
+}
+
+int dft_precond_kinetic_cheb(dft_ctx* c, const dft_grid* g, int prec, double sigma, int sweeps, const void* r,
+                             int64_t ldr, int nb, void* t, int64_t ldt, void* tmp) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, valid_grid(g) && r && t && tmp && nb >= 0 && nb <= 65535 && sweeps >= 0 && sigma > 0.0 &&
+                       ldr >= grid_points(g) && ldt >= grid_points(g),
+                "dft_precond_kinetic_cheb: bad arguments");
+    DFT_REQUIRE(c, g->nx * g->ny * g->nz < (1LL << 31), "dft_precond_kinetic_cheb: grid exceeds 2^31 points");
+    if (!nb) return DFT_OK;
+    if (prec == DFT_F64)
+        return kinetic_cheb_t<double>(c, g, sigma, sweeps, (const double*)r, ldr, nb, (double*)t, ldt, (double*)tmp);
+    if (prec == DFT_F32)
+        return kinetic_cheb_t<float>(c, g, sigma, sweeps, (const float*)r, ldr, nb, (float*)t, ldt, (float*)tmp);
+    return set_err(c, DFT_EINVAL, "dft_precond_kinetic_cheb: bad precision");
 }
 
 int dft_precond_diag(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr, int nb,

@@ -250,8 +250,12 @@ int KineticPreconditioner::apply(dft_ctx* c, const dft_grid& g, int prec, const 
         kt_cols_ = ncols;
         sig_prec_ = prec;
     }
-    check(c, dft_precond_kinetic(c, &g, prec, sigma_, omega_, sweeps_, R, ldr, ncols, T, ldt, kt_.get()),
-          "dft_precond_kinetic");
+    if (cheb_)
+        check(c, dft_precond_kinetic_cheb(c, &g, prec, sigma_, sweeps_, R, ldr, ncols, T, ldt, kt_.get()),
+              "dft_precond_kinetic_cheb");
+    else
+        check(c, dft_precond_kinetic(c, &g, prec, sigma_, omega_, sweeps_, R, ldr, ncols, T, ldt, kt_.get()),
+              "dft_precond_kinetic");
     return 0;
 }
 

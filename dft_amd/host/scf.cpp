@@ -39,12 +39,21 @@ struct ScfDriver::Fields {
     }
 };
 
-std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g) {
+std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g,
+                                                    int sweeps, double shift) {
     switch (kind) {
         case DFT_PRECOND_DIAG:
             return std::unique_ptr<Preconditioner>(new DiagonalPreconditioner());
         case DFT_PRECOND_KINETIC:
-            return std::unique_ptr<Preconditioner>(new KineticPreconditioner());
+            if (sweeps < 0 || sweeps > 64) throw std::invalid_argument("precond_sweeps must be in 0..64");
+            if (!(shift >= 0.0) || !std::isfinite(shift)) throw std::invalid_argument("precond_shift must be >= 0");
+            return std::unique_ptr<Preconditioner>(
+                new KineticPreconditioner(sweeps > 0 ? sweeps : 4, shift > 0.0 ? shift : 1.0));
+        case DFT_PRECOND_KINETIC_CHEB:
+            if (sweeps < 0 || sweeps > 64) throw std::invalid_argument("precond_sweeps must be in 0..64");
+            if (!(shift >= 0.0) || !std::isfinite(shift)) throw std::invalid_argument("precond_shift must be >= 0");
+            return std::unique_ptr<Preconditioner>(
+                new KineticPreconditioner(sweeps > 0 ? sweeps : 8, shift > 0.0 ? shift : 1.5, 2.0 / 3.0, true));
         case DFT_PRECOND_CNN: {
             auto p = make_cnn_preconditioner(c, model, g);
             if (!p) throw std::invalid_argument("learned preconditioner requested without a model");
@@ -95,7 +104,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         }
 
     std::unique_ptr<Preconditioner> own_precond;
-    if (!ext_precond_) own_precond = make_preconditioner(c_, cfg_.precond, cfg_.cnn, g_);
+    if (!ext_precond_) own_precond = make_preconditioner(c_, cfg_.precond, cfg_.cnn, g_, cfg_.precond_sweeps, cfg_.precond_shift);
     Preconditioner* precond = ext_precond_ ? ext_precond_ : own_precond.get();
 
     const bool mixed = cfg_.precision == DFT_SCF_MIXED;

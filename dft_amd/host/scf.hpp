@@ -36,7 +36,8 @@ struct ScfOutputs {
 // Factory for the learned preconditioner (cnn.cpp); null when unavailable.
 std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g);
 // DFT_PRECOND_* -> preconditioner (throws invalid_argument for CNN without a model)
-std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g);
+std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g,
+                                                    int sweeps = 0, double shift = 0.0);
 
 class ScfDriver {
    public:

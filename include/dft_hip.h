@@ -97,6 +97,13 @@ int dft_precond_diag(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff
  * stencil pass.  tmp: nb columns of N (leading dimension N) of scratch. */
 int dft_precond_kinetic(dft_ctx* ctx, const dft_grid* g, int prec, double sigma, double omega, int sweeps,
                         const void* r, int64_t ldr, int nb, void* t, int64_t ldt, void* tmp);
+/* The same correction with the steps combined by Chebyshev semi-iteration
+ * (D is the centre of the spectrum of -1/2 Lap + sigma, so the Jacobi
+ * iteration's spectral radius rho = sum cos(pi/(n+1))/h^2 / D is known):
+ * t_1 = r / D, then `sweeps` fused stencil passes of the three-term
+ * recurrence.  tmp: nb columns of N of scratch. */
+int dft_precond_kinetic_cheb(dft_ctx* ctx, const dft_grid* g, int prec, double sigma, int sweeps, const void* r,
+                             int64_t ldr, int nb, void* t, int64_t ldt, void* tmp);
 /* One damped-Jacobi step of the kinetic smoothing correction (no reference
  * counterpart; see dft_core.h DFT_PRECOND_KINETIC): first = 1: t = cf r;
  * otherwise t = t + cf (r - kt), per column. */

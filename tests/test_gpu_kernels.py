@@ -194,6 +194,52 @@ def test_precond_kinetic_steps(dev, oracle, sweeps, nb):
     assert rel(t32.download(), ref) <= 1e-5
 
 
+@pytest.mark.parametrize("sweeps,nb", [(5, 5), (1, 2), (0, 1)])
+def test_precond_kinetic_chebyshev_steps(dev, oracle, sweeps, nb):
+    """The Chebyshev-combined smoothing steps equal the three-term recurrence
+    written with the oracle's H psi on V = sigma: t1 = r/D, t2 = w2 J(t1),
+    t_{k+1} = w_{k+1} (J(t_k) - t_{k-1}) + t_{k-1}."""
+    from oracle.oracle import Grid
+    og = Grid(23, 19, 17, -2.9, 3.1, -2.2, 2.6, -2.0, 2.1)
+    g = dgrid(og)
+    rng = np.random.default_rng(40 + sweeps)
+    R = rng.standard_normal((nb, og.size))
+    sigma = 0.7
+    ix2, iy2, iz2 = 1 / og.dx ** 2, 1 / og.dy ** 2, 1 / og.dz ** 2
+    D = ix2 + iy2 + iz2 + sigma
+    rho = (np.cos(np.pi / (og.n[0] + 1)) * ix2 + np.cos(np.pi / (og.n[1] + 1)) * iy2 +
+           np.cos(np.pi / (og.n[2] + 1)) * iz2) / D
+    vs = np.full(og.size, sigma)
+    J = lambda t, r: t + (r - oracle.hpsi(og, vs, t)) / D
+    ref = []
+    for o in range(nb):
+        tp, t, w = np.zeros(og.size), R[o] / D, 1.0
+        for s in range(sweeps):
+            w = 1 / (1 - rho * rho / 2) if s == 0 else 1 / (1 - rho * rho * w / 4)
+            tp, t = t, w * (J(t, R[o]) - tp) + tp
+        ref.append(t)
+    ref = np.array(ref)
+    t = dev.array(shape=(nb, og.size))
+    dev.precond_kinetic_cheb(g, dev.array(R), t, nb, sigma, sweeps)
+    assert rel(t.download(), ref) <= 1e-13
+    t32 = dev.array(shape=(nb, og.size), dtype=np.float32)
+    dev.precond_kinetic_cheb(g, dev.array(R, dtype=np.float32), t32, nb, sigma, sweeps)
+    assert rel(t32.download(), ref) <= 2e-5
+
+
+def test_precond_kinetic_chebyshev_converges_to_the_inverse(dev, oracle):
+    """Many Chebyshev steps solve (-1/2 Lap + sigma) t = r (zero ghosts): the
+    spectral-radius formula is right, or the iteration would stall or diverge."""
+    from oracle.oracle import Grid
+    og = Grid(23, 19, 17, -2.9, 3.1, -2.2, 2.6, -2.0, 2.1)
+    g = dgrid(og)
+    r = np.random.default_rng(3).standard_normal(og.size)
+    t = dev.array(shape=(1, og.size))
+    dev.precond_kinetic_cheb(g, dev.array(r.reshape(1, -1)), t, 1, 0.7, 80)
+    res = oracle.hpsi(og, np.full(og.size, 0.7), t.download()[0]) - r
+    assert np.linalg.norm(res) <= 1e-8 * np.linalg.norm(r)
+
+
 def ulp_sensitivity(oracle, fid, pts):
     """Largest relative change of each output under a 1-ulp change of any input
     (computed with the CPU oracle): the conditioning floor of a pointwise test."""

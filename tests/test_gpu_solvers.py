@@ -131,13 +131,16 @@ def test_scf_step_parity(oracle, case):
     assert got["ks"] < 1e-7 and ref["ks"] < 1e-7
 
 
-def test_davidson_kinetic_correction(goldens):
-    """The kinetic smoothing correction reaches the reference's eigenvalues in
-    fewer iterations than the diagonal one."""
+@pytest.mark.parametrize("kind", ["kinetic", "kinetic_cheb"])
+def test_davidson_kinetic_correction(goldens, kind):
+    """The kinetic smoothing corrections (damped-Jacobi or Chebyshev-combined
+    steps) reach the reference's eigenvalues in fewer iterations than the
+    diagonal one."""
     import dft_amd
     og = grid_from(goldens["dav_grid"])
+    pc = dft_amd.PRECOND_KINETIC if kind == "kinetic" else dft_amd.PRECOND_KINETIC_CHEB
     rd = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4)
-    rk = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4, precond=dft_amd.PRECOND_KINETIC)
+    rk = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4, precond=pc)
     assert np.max(np.abs(rk["eigenvalues"][:4] - goldens["dav_eig"])) <= 1e-8
     assert rk["iterations"] <= rd["iterations"], (rk["iterations"], rd["iterations"])
 
