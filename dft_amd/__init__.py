@@ -314,6 +314,10 @@ class Device:
                                                   C.c_int64(g.size), _vp(tmp.ptr)))
         tmp.free()
 
+    def precond_kinetic_exact(self, g, r, t, nb, sigma=1.5):
+        self._chk(self.L.dft_precond_kinetic_exact(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma), _vp(r.ptr),
+                                                   C.c_int64(g.size), C.c_int(nb), _vp(t.ptr), C.c_int64(g.size)))
+
     def precond_kinetic(self, g, r, t, nb, sigma=1.0, omega=2.0 / 3.0, sweeps=4):
         tmp = DeviceArray(self, (nb, g.size), r.dtype)
         self._chk(self.L.dft_precond_kinetic(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma), C.c_double(omega),
@@ -410,7 +414,7 @@ class Device:
 SCF_FP64, SCF_MIXED = 0, 1
 STOP_REFERENCE, STOP_DE = 0, 1
 INIT_BARE, INIT_HYDROGENIC = 0, 1
-PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC, PRECOND_KINETIC_CHEB = 0, 1, 2, 3
+PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC, PRECOND_KINETIC_CHEB, PRECOND_KINETIC_EXACT = 0, 1, 2, 3, 4
 POISSON_JACOBI, POISSON_CHEBYSHEV, POISSON_DIRECT = 0, 1, 2
 
 

@@ -244,13 +244,17 @@ int KineticPreconditioner::apply(dft_ctx* c, const dft_grid& g, int prec, const 
                                  int64_t /*ldx*/, const void* R, int64_t ldr, int /*first*/, int ncols,
                                  const double* /*theta_dev*/, void* T, int64_t ldt) {
     if (ncols <= 0) return 0;
+    if (mode_ == kExact) {
+        check(c, dft_precond_kinetic_exact(c, &g, prec, sigma_, R, ldr, ncols, T, ldt), "dft_precond_kinetic_exact");
+        return 0;
+    }
     const int64_t n = g.nx * g.ny * g.nz;
     if (kt_cols_ < ncols || sig_prec_ != prec) {
         kt_ = Buf(c, (size_t)n * ncols * esize(prec));
         kt_cols_ = ncols;
         sig_prec_ = prec;
     }
-    if (cheb_)
+    if (mode_ == kChebyshev)
         check(c, dft_precond_kinetic_cheb(c, &g, prec, sigma_, sweeps_, R, ldr, ncols, T, ldt, kt_.get()),
               "dft_precond_kinetic_cheb");
     else

@@ -64,20 +64,23 @@ struct DiagonalPreconditioner : Preconditioner {
 // spectrum) are damped relative to the smooth ones, which the diagonal
 // correction (a near-constant scaling on these grids) cannot do.  No
 // reference counterpart; the diagonal correction stays the reference's choice.
-// With `chebyshev`, the same stencil steps combined by Chebyshev
-// semi-iteration (dft_precond_kinetic_cheb): a much closer approximation of
-// (-1/2 Lap + sigma)^{-1} r for the same number of passes.
+// Mode kChebyshev: the same stencil steps combined by Chebyshev
+// semi-iteration (dft_precond_kinetic_cheb), a much closer approximation of
+// (-1/2 Lap + sigma)^{-1} r for the same number of passes.  Mode kExact: that
+// inverse itself, by sine transforms on the matrix cores
+// (dft_precond_kinetic_exact).
 class KineticPreconditioner : public Preconditioner {
    public:
-    KineticPreconditioner(int sweeps = 4, double sigma = 1.0, double omega = 2.0 / 3.0, bool chebyshev = false)
-        : sweeps_(sweeps), sigma_(sigma), omega_(omega), cheb_(chebyshev) {}
+    enum Mode { kJacobi = 0, kChebyshev = 1, kExact = 2 };
+    KineticPreconditioner(int sweeps = 4, double sigma = 1.0, double omega = 2.0 / 3.0, Mode mode = kJacobi)
+        : sweeps_(sweeps), sigma_(sigma), omega_(omega), mode_(mode) {}
     int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* X, int64_t ldx, const void* R,
               int64_t ldr, int first, int ncols, const double* theta_dev, void* T, int64_t ldt) override;
 
    private:
     int sweeps_;
     double sigma_, omega_;
-    bool cheb_;
+    Mode mode_;
     Buf kt_;  // scratch columns for the ping-pong steps
     int64_t kt_cols_ = 0;
     int sig_prec_ = -1;

@@ -53,7 +53,12 @@ std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const 
             if (sweeps < 0 || sweeps > 64) throw std::invalid_argument("precond_sweeps must be in 0..64");
             if (!(shift >= 0.0) || !std::isfinite(shift)) throw std::invalid_argument("precond_shift must be >= 0");
             return std::unique_ptr<Preconditioner>(
-                new KineticPreconditioner(sweeps > 0 ? sweeps : 8, shift > 0.0 ? shift : 1.5, 2.0 / 3.0, true));
+                new KineticPreconditioner(sweeps > 0 ? sweeps : 8, shift > 0.0 ? shift : 1.5, 2.0 / 3.0,
+                                          KineticPreconditioner::kChebyshev));
+        case DFT_PRECOND_KINETIC_EXACT:
+            if (!(shift >= 0.0) || !std::isfinite(shift)) throw std::invalid_argument("precond_shift must be >= 0");
+            return std::unique_ptr<Preconditioner>(new KineticPreconditioner(
+                0, shift > 0.0 ? shift : 2.0, 2.0 / 3.0, KineticPreconditioner::kExact));
         case DFT_PRECOND_CNN: {
             auto p = make_cnn_preconditioner(c, model, g);
             if (!p) throw std::invalid_argument("learned preconditioner requested without a model");

@@ -29,7 +29,8 @@ enum { DFT_INIT_BARE = 0, DFT_INIT_HYDROGENIC = 1 };
 /* DIAG: the reference's diagonal correction; CNN: the learned correction with
  * the diagonal one as fallback; KINETIC: 4 damped-Jacobi steps on
  * (-1/2 Lap + 1) t = r (a smoothing correction, no reference counterpart). */
-enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1, DFT_PRECOND_KINETIC = 2, DFT_PRECOND_KINETIC_CHEB = 3 };
+enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1, DFT_PRECOND_KINETIC = 2, DFT_PRECOND_KINETIC_CHEB = 3,
+       DFT_PRECOND_KINETIC_EXACT = 4 };
 enum { DFT_POISSON_JACOBI = 0, DFT_POISSON_CHEBYSHEV = 1, DFT_POISSON_DIRECT = 2 };
 
 typedef struct {
@@ -50,11 +51,11 @@ typedef struct {
     double davidson_tol;   /* residual threshold; 0: 1e-7 (EigenvalueSolverMolecule.cpp:19) */
     int davidson_max_iter; /* 0: 200 */
     int refine_iterations; /* mixed mode: max FP64 refinement steps after the FP32 phase stops */
-    int precond;           /* DFT_PRECOND_DIAG, DFT_PRECOND_KINETIC(_CHEB) or DFT_PRECOND_CNN */
+    int precond;           /* DFT_PRECOND_DIAG, DFT_PRECOND_KINETIC(_CHEB, _EXACT) or DFT_PRECOND_CNN */
     const void* cnn;       /* trained network (const dft_cnn_model*) for DFT_PRECOND_CNN */
     int verbose;           /* 1: one line per SCF step on stderr */
     int precond_sweeps;    /* DFT_PRECOND_KINETIC(_CHEB): smoothing steps per correction; 0: 4 (8 for _CHEB) */
-    double precond_shift;  /* DFT_PRECOND_KINETIC(_CHEB): sigma in (-1/2 Lap + sigma)^-1; 0: 1.0 (1.5 for _CHEB) */
+    double precond_shift;  /* DFT_PRECOND_KINETIC*: sigma in (-1/2 Lap + sigma)^-1; 0: 1.0 (_CHEB 1.5, _EXACT 2.0) */
 } dft_scf_config;
 
 typedef struct {
@@ -109,7 +110,7 @@ int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_
  * FP64 values.  prec/gram_mode select the arithmetic.  eig: nstates values;
  * psi (optional): nconv orbitals, N each, normalised sum psi^2 dV = 1 like
  * the reference's MolecularOrbital::psi; resid (optional): nstates residuals.
- * precond: DFT_PRECOND_DIAG / CNN / KINETIC / KINETIC_CHEB; cnn: the trained network
+ * precond: DFT_PRECOND_DIAG / CNN / KINETIC / KINETIC_CHEB / KINETIC_EXACT; cnn: the trained network
  * (dft_cnn_model) for DFT_PRECOND_CNN.  stats (optional, 4 values): H-applied columns,
  * restarts, learned corrections applied, diagonal fallbacks. */
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,

@@ -104,6 +104,12 @@ int dft_precond_kinetic(dft_ctx* ctx, const dft_grid* g, int prec, double sigma,
  * recurrence.  tmp: nb columns of N of scratch. */
 int dft_precond_kinetic_cheb(dft_ctx* ctx, const dft_grid* g, int prec, double sigma, int sweeps, const void* r,
                              int64_t ldr, int nb, void* t, int64_t ldt, void* tmp);
+/* The limit of both: t = (-1/2 Lap + sigma)^{-1} r (zero ghosts) applied
+ * exactly by sine transforms along x, y, z, as BF16 GEMMs with FP32
+ * accumulation on the matrix cores (a correction direction needs ~2 digits).
+ * No tmp argument: the context's workspace holds the BF16 blocks. */
+int dft_precond_kinetic_exact(dft_ctx* ctx, const dft_grid* g, int prec, double sigma, const void* r, int64_t ldr,
+                              int nb, void* t, int64_t ldt);
 /* One damped-Jacobi step of the kinetic smoothing correction (no reference
  * counterpart; see dft_core.h DFT_PRECOND_KINETIC): first = 1: t = cf r;
  * otherwise t = t + cf (r - kt), per column. */

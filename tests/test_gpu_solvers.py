@@ -131,14 +131,15 @@ def test_scf_step_parity(oracle, case):
     assert got["ks"] < 1e-7 and ref["ks"] < 1e-7
 
 
-@pytest.mark.parametrize("kind", ["kinetic", "kinetic_cheb"])
+@pytest.mark.parametrize("kind", ["kinetic", "kinetic_cheb", "kinetic_exact"])
 def test_davidson_kinetic_correction(goldens, kind):
     """The kinetic smoothing corrections (damped-Jacobi or Chebyshev-combined
     steps) reach the reference's eigenvalues in fewer iterations than the
     diagonal one."""
     import dft_amd
     og = grid_from(goldens["dav_grid"])
-    pc = dft_amd.PRECOND_KINETIC if kind == "kinetic" else dft_amd.PRECOND_KINETIC_CHEB
+    pc = {"kinetic": dft_amd.PRECOND_KINETIC, "kinetic_cheb": dft_amd.PRECOND_KINETIC_CHEB,
+          "kinetic_exact": dft_amd.PRECOND_KINETIC_EXACT}[kind]
     rd = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4)
     rk = dft_amd.davidson_run(dgrid(og), goldens["dav_v"], nstates=6, nconv=4, precond=pc)
     assert np.max(np.abs(rk["eigenvalues"][:4] - goldens["dav_eig"])) <= 1e-8
@@ -199,6 +200,7 @@ def test_scf_fast_poisson_matches_converged_jacobi(oracle, method):
     # dRho < 1e-9, which leaves ~1e-8 Ha in this non-variational expression
     assert r["converged"] and abs(r["total"] - ref["total"]) <= 1e-7
     m = dft_amd.scf_run(dgrid(og), nuc, precision=dft_amd.SCF_MIXED, refine_iterations=250,
-                        precond=dft_amd.PRECOND_KINETIC, **common)
+                        precond=dft_amd.PRECOND_KINETIC_CHEB if method == "chebyshev" else dft_amd.PRECOND_KINETIC_EXACT,
+                        **common)
     print(method, "scf mixed", m["total"], m["iterations"], m["refine_iterations"])
     assert m["converged"] and abs(m["total"] - ref["total"]) <= 1e-7
