@@ -372,7 +372,7 @@ def main():
     print(f"[bench] workload {a.workload}: {w['desc']}; {len(atoms)} atoms, {ne} electrons", file=sys.stderr, flush=True)
     for i in range(a.warmup):  # loads the code objects (rocBLAS kernels included) the timed run uses
         dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=dft_amd.SCF_MIXED, nstates=w["nstates"],
-                        max_iterations=2, poisson_fixed=50, stop_rule=dft_amd.STOP_DE,
+                        max_iterations=2, poisson_batch=64, **poisson_args(dft_amd, w), stop_rule=dft_amd.STOP_DE,
                         energy_tol=1e-5, init_guess=w["init"], refine_iterations=1, precond=pc, cnn=model)
         print(f"[bench] warmup {i + 1} done", file=sys.stderr, flush=True)
     if a.mixed_only:

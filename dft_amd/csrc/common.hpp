@@ -30,6 +30,10 @@ struct dft_ctx {
     // cooperative queue can be held by another process); Jacobi then runs
     // one launch per sweep
     bool coop_refused = false;
+    // BF16 sine-transform matrices of the exact kinetic correction, for the
+    // grid shape dst_shape (built on first use)
+    void* dst_mats = nullptr;
+    int dst_shape[3] = {0, 0, 0};
     // rocBLAS handle for the dense FP64 subspace solvers (created on first use)
     void* blas = nullptr;
     int* dev_info = nullptr;

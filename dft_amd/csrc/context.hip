@@ -170,6 +170,7 @@ int dft_ctx_destroy(dft_ctx* c) {
     if (c->partials) (void)hipFree(c->partials);
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->sweep_max) (void)hipFree(c->sweep_max);
+    if (c->dst_mats) (void)hipFree(c->dst_mats);
     if (c->host_scalar) (void)hipHostFree(c->host_scalar);
     dft::release_blas(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);

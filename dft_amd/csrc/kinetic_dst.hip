@@ -38,18 +38,23 @@ __global__ __launch_bounds__(256) void k_cols_to_bf16(int64_t n, const T* __rest
     if (p < n) a[(int64_t)o * n + p] = (__bf16)(float)r[(int64_t)o * ldr + p];
 }
 
-// B[i + nx (j + ny (k + nz o))] /= l_x(i) + l_y(j) + l_z(k) + sigma
+// B[i + nx (j + ny (k + nz o))] /= l_x(i) + l_y(j) + l_z(k) + sigma: one
+// workgroup per (z plane k, column o), the x and y eigenvalues in LDS, so the
+// pass is a plain stream of the block
 __global__ __launch_bounds__(256) void k_kin_scale(int nx, int ny, int nz, float ix2, float iy2, float iz2,
                                                    float sigma, __bf16* __restrict__ B) {
-    const int64_t n = (int64_t)nx * ny * nz;
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n) return;
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
-    const float lx = (1.0f - cospif((float)(i + 1) / (float)(nx + 1))) * ix2;
-    const float ly = (1.0f - cospif((float)(j + 1) / (float)(ny + 1))) * iy2;
-    const float lz = (1.0f - cospif((float)(k + 1) / (float)(nz + 1))) * iz2;
-    const int64_t q = (int64_t)blockIdx.y * n + p;
-    B[q] = (__bf16)((float)B[q] / (lx + ly + lz + sigma));
+    __shared__ float lx[1024], ly[1024];
+    const int k = blockIdx.x, o = blockIdx.y;
+    for (int i = threadIdx.x; i < nx; i += 256) lx[i] = (1.0f - cospif((float)(i + 1) / (float)(nx + 1))) * ix2;
+    for (int j = threadIdx.x; j < ny; j += 256) ly[j] = (1.0f - cospif((float)(j + 1) / (float)(ny + 1))) * iy2;
+    const float lz = (1.0f - cospif((float)(k + 1) / (float)(nz + 1))) * iz2 + sigma;
+    __syncthreads();
+    __bf16* plane = B + ((int64_t)o * nz + k) * ((int64_t)nx * ny);
+    const int np = nx * ny;
+    for (int e = threadIdx.x; e < np; e += 256) {
+        const int j = e / nx, i = e - j * nx;
+        plane[e] = (__bf16)((float)plane[e] / (lx[i] + ly[j] + lz));
+    }
 }
 
 __global__ __launch_bounds__(256) void k_f32_to_f64_cols(int64_t n, const float* __restrict__ s, double* __restrict__ t,
@@ -93,21 +98,36 @@ int kinetic_exact_t(dft_ctx* c, const dft_grid* g, double sigma, const T* r, int
     const bool f64 = sizeof(T) == 8;
     auto al = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t sb = al(sizeof(__bf16) * (size_t)nb * n);
-    const size_t ss = al(sizeof(__bf16) * (size_t)nx * nx) + al(sizeof(__bf16) * (size_t)ny * ny) +
-                      al(sizeof(__bf16) * (size_t)nz * nz);
     const size_t sf = f64 ? al(sizeof(float) * (size_t)nb * n) : 0;
-    if (int rc = ensure_scratch(c, 2 * sb + ss + sf)) return rc;
+    if (int rc = ensure_scratch(c, 2 * sb + sf)) return rc;
     char* w = static_cast<char*>(c->scratch);
     __bf16* A = reinterpret_cast<__bf16*>(w);
     __bf16* B = reinterpret_cast<__bf16*>(w + sb);
-    __bf16* Sx = reinterpret_cast<__bf16*>(w + 2 * sb);
-    __bf16* Sy = reinterpret_cast<__bf16*>(w + 2 * sb + al(sizeof(__bf16) * (size_t)nx * nx));
-    __bf16* Sz = reinterpret_cast<__bf16*>(w + 2 * sb + al(sizeof(__bf16) * (size_t)nx * nx) +
-                                           al(sizeof(__bf16) * (size_t)ny * ny));
-    float* F = f64 ? reinterpret_cast<float*>(w + 2 * sb + ss) : nullptr;
-    k_dst_matrix_bf16<<<(unsigned)(((int64_t)nx * nx + 255) / 256), 256, 0, c->stream>>>(nx, Sx);
-    k_dst_matrix_bf16<<<(unsigned)(((int64_t)ny * ny + 255) / 256), 256, 0, c->stream>>>(ny, Sy);
-    k_dst_matrix_bf16<<<(unsigned)(((int64_t)nz * nz + 255) / 256), 256, 0, c->stream>>>(nz, Sz);
+    float* F = f64 ? reinterpret_cast<float*>(w + 2 * sb) : nullptr;
+    // the three transform matrices, built once per grid shape and kept by the context
+    const size_t sx = al(sizeof(__bf16) * (size_t)nx * nx), sy = al(sizeof(__bf16) * (size_t)ny * ny);
+    const size_t need = sx + sy + al(sizeof(__bf16) * (size_t)nz * nz);
+    if (c->dst_shape[0] != nx || c->dst_shape[1] != ny || c->dst_shape[2] != nz || !c->dst_mats) {
+        if (c->dst_mats) {
+            DFT_HIP(c, hipStreamSynchronize(c->stream));
+            DFT_HIP(c, hipFree(c->dst_mats));
+            c->dst_mats = nullptr;
+        }
+        DFT_HIP(c, hipMalloc(&c->dst_mats, need));
+        char* m = static_cast<char*>(c->dst_mats);
+        k_dst_matrix_bf16<<<(unsigned)(((int64_t)nx * nx + 255) / 256), 256, 0, c->stream>>>(nx, reinterpret_cast<__bf16*>(m));
+        k_dst_matrix_bf16<<<(unsigned)(((int64_t)ny * ny + 255) / 256), 256, 0, c->stream>>>(
+            ny, reinterpret_cast<__bf16*>(m + sx));
+        k_dst_matrix_bf16<<<(unsigned)(((int64_t)nz * nz + 255) / 256), 256, 0, c->stream>>>(
+            nz, reinterpret_cast<__bf16*>(m + sx + sy));
+        DFT_CHECK_LAUNCH(c);
+        c->dst_shape[0] = nx;
+        c->dst_shape[1] = ny;
+        c->dst_shape[2] = nz;
+    }
+    const __bf16* Sx = reinterpret_cast<const __bf16*>(c->dst_mats);
+    const __bf16* Sy = reinterpret_cast<const __bf16*>(static_cast<char*>(c->dst_mats) + sx);
+    const __bf16* Sz = reinterpret_cast<const __bf16*>(static_cast<char*>(c->dst_mats) + sx + sy);
     const dim3 gc((unsigned)((n + 255) / 256), (unsigned)nb);
     k_cols_to_bf16<T><<<gc, 256, 0, c->stream>>>(n, r, ldr, A);
     DFT_CHECK_LAUNCH(c);
@@ -115,8 +135,9 @@ int kinetic_exact_t(dft_ctx* c, const dft_grid* g, double sigma, const T* r, int
     if (int rc = rocblas_for(c, &hv)) return rc;
     rocblas_handle h = static_cast<rocblas_handle>(hv);
     if (int rc = dst3_bf16(c, h, nx, ny, nz, nb, Sx, Sy, Sz, A, B, B, rocblas_datatype_bf16_r, n)) return rc;
-    k_kin_scale<<<gc, 256, 0, c->stream>>>(nx, ny, nz, (float)(1.0 / (g->dx * g->dx)), (float)(1.0 / (g->dy * g->dy)),
-                                           (float)(1.0 / (g->dz * g->dz)), (float)sigma, B);
+    k_kin_scale<<<dim3((unsigned)nz, (unsigned)nb), 256, 0, c->stream>>>(
+        nx, ny, nz, (float)(1.0 / (g->dx * g->dx)), (float)(1.0 / (g->dy * g->dy)), (float)(1.0 / (g->dz * g->dz)),
+        (float)sigma, B);
     DFT_CHECK_LAUNCH(c);
     if (!f64) return dst3_bf16(c, h, nx, ny, nz, nb, Sx, Sy, Sz, B, A, t, rocblas_datatype_f32_r, ldt);
     if (int rc = dst3_bf16(c, h, nx, ny, nz, nb, Sx, Sy, Sz, B, A, F, rocblas_datatype_f32_r, n)) return rc;
@@ -136,8 +157,9 @@ int dft_precond_kinetic_exact(dft_ctx* c, const dft_grid* g, int prec, double si
                        ldt >= grid_points(g),
                 "dft_precond_kinetic_exact: bad arguments");
     const int64_t n = grid_points(g);
-    DFT_REQUIRE(c, n * (int64_t)nb / g->nx < (1LL << 31) && g->nx * g->ny < (1LL << 31),
-                "dft_precond_kinetic_exact: block too large for one transform");
+    DFT_REQUIRE(c, n * (int64_t)nb / g->nx < (1LL << 31) && g->nx * g->ny < (1LL << 31) && g->nx <= 1024 &&
+                       g->ny <= 1024 && g->nz <= 65535,
+                "dft_precond_kinetic_exact: grid or block too large for the transforms");
     if (!nb) return DFT_OK;
     if (prec == DFT_F64) return kinetic_exact_t<double>(c, g, sigma, (const double*)r, ldr, nb, (double*)t, ldt);
     if (prec == DFT_F32) return kinetic_exact_t<float>(c, g, sigma, (const float*)r, ldr, nb, (float*)t, ldt);
