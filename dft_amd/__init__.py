@@ -416,6 +416,7 @@ STOP_REFERENCE, STOP_DE = 0, 1
 INIT_BARE, INIT_HYDROGENIC = 0, 1
 PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC, PRECOND_KINETIC_CHEB, PRECOND_KINETIC_EXACT = 0, 1, 2, 3, 4
 POISSON_JACOBI, POISSON_CHEBYSHEV, POISSON_DIRECT = 0, 1, 2
+MIX_LINEAR, MIX_PULAY = 0, 1
 
 
 class ScfConfig(C.Structure):
@@ -428,7 +429,7 @@ class ScfConfig(C.Structure):
                 ("energy_tol", C.c_double), ("init_guess", C.c_int), ("davidson_tol", C.c_double),
                 ("davidson_max_iter", C.c_int), ("refine_iterations", C.c_int), ("precond", C.c_int),
                 ("cnn", C.c_void_p), ("verbose", C.c_int), ("precond_sweeps", C.c_int),
-                ("precond_shift", C.c_double)]
+                ("precond_shift", C.c_double), ("mixer", C.c_int), ("mix_depth", C.c_int), ("mix_beta", C.c_double)]
 
 
 class ScfReport(C.Structure):
@@ -451,14 +452,15 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
             max_iterations=250, poisson_fixed=0, poisson_batch=64, poisson_method=POISSON_JACOBI, poisson_tol=0.0,
             stop_rule=STOP_REFERENCE, energy_tol=1e-5,
             init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
-            precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False, precond_sweeps=0, precond_shift=0.0):
+            precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False, precond_sweeps=0, precond_shift=0.0,
+            mixer=MIX_LINEAR, mix_depth=0, mix_beta=0.0):
     """Run the device-resident SCF; returns a dict of results."""
     L = core_lib()
     nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
                     poisson_fixed, poisson_batch, poisson_method, poisson_tol, stop_rule, energy_tol, init_guess, davidson_tol,
                     davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0,
-                    precond_sweeps, precond_shift)
+                    precond_sweeps, precond_shift, mixer, mix_depth, mix_beta)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
     eig = np.zeros(ne + 2)

@@ -24,6 +24,122 @@ constexpr double kF32EnergyFloor = 3.0e-4, kF32DensityFloor = 1.0e-5;
 
 }  // namespace
 
+// Pulay (DIIS) density mixing (P. Pulay, Chem. Phys. Lett. 73, 393 (1980)):
+// with the last `depth` input densities rho_i and residuals R_i = rho_out,i -
+// rho_i, minimise |sum c_i R_i| subject to sum c_i = 1 and take
+// rho_next = sum c_i (rho_i + beta R_i).  Spin densities are mixed together
+// (one vector of both channels for open shells).  The history is one block
+// [rho_0 .. rho_{M-1} | R_0 .. R_{M-1}] (leading dimension L = n or 2 n), so
+// the new row of the DIIS matrix is one Gram pass and rho_next one block
+// update.  No reference counterpart: the reference's lagged linear mixing
+// (SelfConsistentFieldMolecule.cpp:447-484) stays the default.
+class PulayMixer {
+   public:
+    PulayMixer(dft_ctx* c, int prec, int64_t n, bool closed, int depth, double beta)
+        : c_(c), prec_(prec), n_(n), closed_(closed), depth_(depth), beta_(beta) {
+        L_ = closed ? n : 2 * n;
+        hist_ = Buf(c, (size_t)2 * depth * L_ * esize(prec));
+        hist_.zero();  // unused columns enter the update with coefficient 0
+        coef_ = Buf(c, sizeof(double) * (size_t)2 * depth);
+        dots_ = Buf(c, sizeof(double) * (size_t)depth);
+        A_.assign((size_t)depth * depth, 0.0);
+    }
+    void mix(void* ra, void* rb, const void* ora, const void* orb) {
+        const size_t es = esize(prec_), cb = (size_t)L_ * es;
+        char* h = static_cast<char*>(hist_.get());
+        char* rin = h + (size_t)head_ * cb;
+        char* res = h + (size_t)(depth_ + head_) * cb;
+        const size_t fb = (size_t)n_ * es;
+        check(c_, dft_copy(c_, rin, ra, fb), "dft_copy");
+        check(c_, dft_copy(c_, res, ora, fb), "dft_copy");
+        check(c_, dft_axpby(c_, prec_, n_, -1.0, ra, 1.0, res), "dft_axpby");
+        if (!closed_) {
+            check(c_, dft_copy(c_, rin + fb, rb, fb), "dft_copy");
+            check(c_, dft_copy(c_, res + fb, orb, fb), "dft_copy");
+            check(c_, dft_axpby(c_, prec_, n_, -1.0, rb, 1.0, res + fb), "dft_axpby");
+        }
+        count_ = std::min(count_ + 1, depth_);
+        // new row: <R_i, R_head> for every stored i (slots 0..count-1 hold the history)
+        const int mode = prec_ == DFT_F64 ? DFT_GRAM_F64 : DFT_GRAM_F32;
+        check(c_, dft_gram(c_, mode, L_, count_, 1, 1.0, h + (size_t)depth_ * cb, L_, res, L_, 0.0,
+                           dots_.as<double>(), count_),
+              "dft_gram");
+        std::vector<double> d(count_);
+        check(c_, dft_download(c_, d.data(), dots_.get(), sizeof(double) * count_), "dft_download");
+        for (int i = 0; i < count_; ++i) A_[(size_t)i * depth_ + head_] = A_[(size_t)head_ * depth_ + i] = d[i];
+        // bordered system [A 1; 1^T 0] [c; lambda] = [0; 1], scaled and lightly regularised
+        const int m = count_ + 1;
+        std::vector<double> M((size_t)m * m, 0.0), b(m, 0.0);
+        double scale = 0.0;
+        for (int i = 0; i < count_; ++i) scale = std::max(scale, A_[(size_t)i * depth_ + i]);
+        scale = scale > 0.0 ? 1.0 / scale : 1.0;
+        for (int i = 0; i < count_; ++i) {
+            for (int j = 0; j < count_; ++j) M[(size_t)i * m + j] = A_[(size_t)i * depth_ + j] * scale;
+            M[(size_t)i * m + i] *= 1.0 + 1e-10;
+            M[(size_t)i * m + count_] = M[(size_t)count_ * m + i] = 1.0;
+        }
+        b[count_] = 1.0;
+        std::vector<double> cvec;
+        if (!solve(M, b, m, cvec)) {  // singular history: restart it from the newest entry
+            cvec.assign(count_ + 1, 0.0);
+            cvec[head_] = 1.0;
+        }
+        // rho_next = sum c_i (rho_i + beta R_i), one pass over the history block
+        std::vector<double> C((size_t)2 * depth_, 0.0);
+        for (int i = 0; i < count_; ++i) {
+            C[i] = cvec[i];
+            C[depth_ + i] = beta_ * cvec[i];
+        }
+        check(c_, dft_upload(c_, coef_.get(), C.data(), sizeof(double) * C.size()), "dft_upload");
+        check(c_, dft_block_update(c_, prec_, n_, 2 * depth_, 1, 1.0, h, L_, coef_.as<double>(), 2 * depth_, 0.0, ra, n_),
+              "dft_block_update");
+        if (closed_) check(c_, dft_copy(c_, rb, ra, fb), "dft_copy");
+        else
+            check(c_, dft_block_update(c_, prec_, n_, 2 * depth_, 1, 1.0, h + fb, L_, coef_.as<double>(), 2 * depth_, 0.0,
+                                       rb, n_),
+                  "dft_block_update");
+        head_ = (head_ + 1) % depth_;
+    }
+
+   private:
+    // Gaussian elimination with partial pivoting; false when singular
+    static bool solve(std::vector<double> M, std::vector<double> b, int m, std::vector<double>& x) {
+        for (int col = 0; col < m; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < m; ++r)
+                if (std::fabs(M[(size_t)r * m + col]) > std::fabs(M[(size_t)piv * m + col])) piv = r;
+            if (!(std::fabs(M[(size_t)piv * m + col]) > 1e-300)) return false;
+            if (piv != col) {
+                for (int j = 0; j < m; ++j) std::swap(M[(size_t)piv * m + j], M[(size_t)col * m + j]);
+                std::swap(b[piv], b[col]);
+            }
+            for (int r = col + 1; r < m; ++r) {
+                const double f = M[(size_t)r * m + col] / M[(size_t)col * m + col];
+                for (int j = col; j < m; ++j) M[(size_t)r * m + j] -= f * M[(size_t)col * m + j];
+                b[r] -= f * b[col];
+            }
+        }
+        x.assign(m, 0.0);
+        for (int r = m - 1; r >= 0; --r) {
+            double s = b[r];
+            for (int j = r + 1; j < m; ++j) s -= M[(size_t)r * m + j] * x[j];
+            x[r] = s / M[(size_t)r * m + r];
+            if (!std::isfinite(x[r])) return false;
+        }
+        return true;
+    }
+
+    dft_ctx* c_;
+    int prec_;
+    int64_t n_, L_ = 0;
+    bool closed_;
+    int depth_;
+    double beta_;
+    Buf hist_, coef_, dots_;
+    std::vector<double> A_;
+    int count_ = 0, head_ = 0;
+};
+
 struct ScfDriver::Fields {
     int prec;
     int64_t n;
@@ -143,7 +259,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         if (f.prec == DFT_F32) {
             o.gram_mode = DFT_GRAM_SPLIT_BF16;
             o.tol = std::max(o.tol, 3.0e-4);
-            o.max_iter = std::min(o.max_iter, 60);
+            // a warm start needs few iterations; a cold start (seed vectors) keeps the full budget
+            if (have_prev) o.max_iter = std::min(o.max_iter, 60);
         }
         // |dE| < etol only means convergence if the orbitals are converged
         // well beyond it: energy errors go with the square of the residual,
@@ -223,6 +340,10 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     }
 
     double prevE = std::numeric_limits<double>::infinity(), prevDD = prevE, mixing = kMixing0;
+    if (cfg_.mixer != DFT_MIX_LINEAR && cfg_.mixer != DFT_MIX_PULAY) throw std::invalid_argument("unknown density mixer");
+    if (cfg_.mix_depth < 0 || cfg_.mix_depth > 32) throw std::invalid_argument("mix_depth must be in 0..32");
+    std::unique_ptr<PulayMixer> pulay;  // rebuilt for each precision phase
+    int pulay_prec = -1;
     bool refining = false;
     int refine_steps = 0;
     for (int it = 1; it <= maxit + (mixed ? std::max(cfg_.refine_iterations, 0) : 0); ++it) {
@@ -312,13 +433,22 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             rep.converged = 1;
             break;
         }
-        // :447-484 lagged adaptive mixing
-        if (std::isfinite(prevDD)) {
-            if (dd > prevDD * kOsc) mixing = std::max(kMinMix, mixing * kMixDown);
-            else if (dd < prevDD) mixing = std::min(kMaxMix, mixing * kMixUp);
+        if (cfg_.mixer == DFT_MIX_PULAY) {
+            if (!pulay || pulay_prec != p) {
+                pulay.reset(new PulayMixer(c_, p, n, closed, cfg_.mix_depth > 0 ? cfg_.mix_depth : 6,
+                                           cfg_.mix_beta > 0.0 ? cfg_.mix_beta : 0.3));
+                pulay_prec = p;
+            }
+            pulay->mix(f.ra.get(), f.rb.get(), f.ora.get(), f.orb.get());
+        } else {
+            // :447-484 lagged adaptive mixing
+            if (std::isfinite(prevDD)) {
+                if (dd > prevDD * kOsc) mixing = std::max(kMinMix, mixing * kMixDown);
+                else if (dd < prevDD) mixing = std::min(kMaxMix, mixing * kMixUp);
+            }
+            check(c_, dft_axpby(c_, p, n, imix, f.ora.get(), 1.0 - imix, f.ra.get()), "dft_axpby");
+            check(c_, dft_axpby(c_, p, n, imix, f.orb.get(), 1.0 - imix, f.rb.get()), "dft_axpby");
         }
-        check(c_, dft_axpby(c_, p, n, imix, f.ora.get(), 1.0 - imix, f.ra.get()), "dft_axpby");
-        check(c_, dft_axpby(c_, p, n, imix, f.orb.get(), 1.0 - imix, f.rb.get()), "dft_axpby");
         prevDD = dd;
         dd_prev = dd;
         prevE = E[5];

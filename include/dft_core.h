@@ -32,6 +32,7 @@ enum { DFT_INIT_BARE = 0, DFT_INIT_HYDROGENIC = 1 };
 enum { DFT_PRECOND_DIAG = 0, DFT_PRECOND_CNN = 1, DFT_PRECOND_KINETIC = 2, DFT_PRECOND_KINETIC_CHEB = 3,
        DFT_PRECOND_KINETIC_EXACT = 4 };
 enum { DFT_POISSON_JACOBI = 0, DFT_POISSON_CHEBYSHEV = 1, DFT_POISSON_DIRECT = 2 };
+enum { DFT_MIX_LINEAR = 0, DFT_MIX_PULAY = 1 };
 
 typedef struct {
     int functional;        /* DFT_XC_PZ81 / DFT_XC_PBE96 */
@@ -56,6 +57,9 @@ typedef struct {
     int verbose;           /* 1: one line per SCF step on stderr */
     int precond_sweeps;    /* DFT_PRECOND_KINETIC(_CHEB): smoothing steps per correction; 0: 4 (8 for _CHEB) */
     double precond_shift;  /* DFT_PRECOND_KINETIC*: sigma in (-1/2 Lap + sigma)^-1; 0: 1.0 (_CHEB 1.5, _EXACT 2.0) */
+    int mixer;             /* DFT_MIX_LINEAR (the reference's lagged linear mixing) or DFT_MIX_PULAY */
+    int mix_depth;         /* DFT_MIX_PULAY: history length; 0: 6 */
+    double mix_beta;       /* DFT_MIX_PULAY: residual weight; 0: 0.3 */
 } dft_scf_config;
 
 typedef struct {

@@ -204,3 +204,29 @@ def test_scf_fast_poisson_matches_converged_jacobi(oracle, method):
                         **common)
     print(method, "scf mixed", m["total"], m["iterations"], m["refine_iterations"])
     assert m["converged"] and abs(m["total"] - ref["total"]) <= 1e-7
+
+
+def test_scf_pulay_mixing_matches_linear(oracle):
+    """Pulay (DIIS) density mixing converges to the same SCF as the reference's
+    lagged linear mixing (the converged-Jacobi oracle SCF, reference stop rule),
+    closed and open shell, FP64 and mixed precision."""
+    import dft_amd
+    from oracle.oracle import Grid
+    og = Grid.centered(16, 0.42)
+    nuc = [[1, -0.71, 0.02, 0.0], [1, 0.69, 0.0, 0.03]]
+    ref = oracle.scf(og, nuc, 1, poisson_fixed=5000, stop_rule=0, init_guess=1)
+    common = dict(functional=dft_amd.PZ81, poisson_method=dft_amd.POISSON_DIRECT, stop_rule=dft_amd.STOP_REFERENCE,
+                  init_guess=dft_amd.INIT_HYDROGENIC, davidson_tol=1e-9, max_iterations=250, mixer=dft_amd.MIX_PULAY)
+    r = dft_amd.scf_run(dgrid(og), nuc, **common)
+    print("pulay", r["total"], ref["total"], r["iterations"], ref["iterations"])
+    assert r["converged"] and abs(r["total"] - ref["total"]) <= 1e-7
+    assert r["iterations"] <= ref["iterations"]
+    m = dft_amd.scf_run(dgrid(og), nuc, precision=dft_amd.SCF_MIXED, refine_iterations=250,
+                        precond=dft_amd.PRECOND_KINETIC_EXACT, **common)
+    assert m["converged"] and abs(m["total"] - ref["total"]) <= 1e-7
+    # open shell (doublet H3 chain): both spin channels mixed as one vector
+    h3 = [[1, -1.61, 0.02, 0.0], [1, 0.05, 0.0, 0.03], [1, 1.66, -0.02, 0.01]]
+    lin = dft_amd.scf_run(dgrid(og), h3, multiplicity=2, **dict(common, mixer=dft_amd.MIX_LINEAR))
+    pul = dft_amd.scf_run(dgrid(og), h3, multiplicity=2, **common)
+    print("doublet", lin["total"], pul["total"], lin["iterations"], pul["iterations"])
+    assert lin["converged"] and pul["converged"] and abs(lin["total"] - pul["total"]) <= 1e-7
