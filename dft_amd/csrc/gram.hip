@@ -2,14 +2,15 @@
 // CDNA4 matrix cores.
 //
 // A^T B with A: N x ma, B: N x mb, N (grid points) >> ma, mb (<= a few
-// hundred).  The long dimension N is split across workgroups; each
-// workgroup owns a 64 x 64 output tile (four waves, each 32 x 32 = 2 x 2
-// MFMA 16x16 tiles) over one K-range and streams its A and B columns
-// straight from HBM into MFMA operands: an operand lane holds a short
-// contiguous run of one column, so a wave touches a handful of 128-byte
-// lines per step and the next K steps hit the same lines.  Partial tiles
-// go to an FP64 buffer and a second kernel sums the K-splits in a fixed
-// order (deterministic).
+// hundred): bandwidth-bound.  Two kernels:
+//  * k_gram_stream (ma <= 96, mb <= 32, FP32 data: every Davidson product of
+//    the mixed phase): no LDS.  A wave owns a contiguous range of points and
+//    all output tiles; an operand lane loads 8 consecutive points of one
+//    column (32 bytes) per step straight into MFMA operands.
+//  * k_gram (any shape): 96 x 96 output tiles of 4 waves, 32-point K slabs
+//    staged through LDS with the next slab prefetched in registers.
+// The long dimension N is split across workgroups; partial tiles go to an
+// FP64 buffer and two kernels sum them in a fixed order (deterministic).
 //   F64:         v_mfma_f64_16x16x4_f64
 //   F32:         v_mfma_f32_16x16x4_f32
 //   SPLIT_BF16:  x = hi + lo (two bf16), A^T B ~ hi.hi + hi.lo + lo.hi with
@@ -257,6 +258,114 @@ __global__ __launch_bounds__(256) void k_gram(int64_t n, int ma, int mb, int ntm
 
 // First stage of the K-split reduction: segment s of `step` partials -> part2[s]
 // (enough workgroups to keep many loads in flight; fixed summation order).
+// Streaming Gram for small outputs (SPLIT_BF16 or F32 on FP32 data): no LDS
+// slabs.  Lane (c, q) = (l % 16, l / 16) loads points p + 8q .. p + 8q + 7 of
+// output row (A column) 16 ta + c and of output column (B column) 16 tb + c:
+// for SPLIT that is the K slice of one v_mfma_f32_16x16x32_bf16 (hi.hi +
+// hi.lo + lo.hi); for F32 MFMA t of eight v_mfma_f32_16x16x4_f32 takes point
+// p + 8q + t.  The 4 waves' tiles are summed in LDS and one FP64 partial per
+// workgroup goes to part[blockIdx.x * ma * mb + j * ma + i].
+template <int MODE, int TA, int TB>
+__global__ __launch_bounds__(256) void k_gram_stream(int64_t n, int ma, int mb, const float* __restrict__ A,
+                                                     int64_t lda, const float* __restrict__ B, int64_t ldb, int64_t kw,
+                                                     double* __restrict__ part) {
+    constexpr bool SPLIT = MODE == DFT_GRAM_SPLIT_BF16;
+    __shared__ float red[4][TA * TB * 256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int64_t k0 = ((int64_t)blockIdx.x * 4 + w) * kw, k1 = min(n, k0 + kw);
+    const float* ap[TA];
+    const float* bp[TB];
+    bool av[TA], bv[TB];
+#pragma unroll
+    for (int t = 0; t < TA; ++t) {
+        const int col = 16 * t + r16;
+        av[t] = col < ma;
+        ap[t] = A + (int64_t)(av[t] ? col : 0) * lda;
+    }
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+        const int col = 16 * t + r16;
+        bv[t] = col < mb;
+        bp[t] = B + (int64_t)(bv[t] ? col : 0) * ldb;
+    }
+    f4 acc[TA][TB];
+#pragma unroll
+    for (int a = 0; a < TA; ++a)
+#pragma unroll
+        for (int b = 0; b < TB; ++b) acc[a][b] = f4{0, 0, 0, 0};
+    for (int64_t p = k0; p < k1; p += 32) {
+        const int64_t pp = p + 8 * q;
+        const bool full = pp + 8 <= k1;
+        float xa[TA][8], xb[TB][8];
+        auto load8 = [&](const float* src, bool valid, float* x) {
+            if (valid && full) {
+                const float4 u = *reinterpret_cast<const float4*>(src + pp);
+                const float4 v = *reinterpret_cast<const float4*>(src + pp + 4);
+                x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w;
+                x[4] = v.x; x[5] = v.y; x[6] = v.z; x[7] = v.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x[e] = (valid && pp + e < k1) ? src[pp + e] : 0.0f;
+            }
+        };
+#pragma unroll
+        for (int t = 0; t < TA; ++t) load8(ap[t], av[t], xa[t]);
+#pragma unroll
+        for (int t = 0; t < TB; ++t) load8(bp[t], bv[t], xb[t]);
+        if constexpr (SPLIT) {
+            bf8 bh[TB], bl[TB];
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const __bf16 h = to_bf16(xb[t][e]);
+                    bh[t][e] = h;
+                    bl[t][e] = to_bf16(xb[t][e] - (float)h);
+                }
+#pragma unroll
+            for (int a = 0; a < TA; ++a) {
+                bf8 ah, al;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const __bf16 h = to_bf16(xa[a][e]);
+                    ah[e] = h;
+                    al[e] = to_bf16(xa[a][e] - (float)h);
+                }
+#pragma unroll
+                for (int b = 0; b < TB; ++b) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < TA; ++a)
+#pragma unroll
+                for (int b = 0; b < TB; ++b)
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[a][t], xb[b][t], acc[a][b], 0, 0, 0);
+        }
+    }
+    // sum the 4 waves, then one FP64 partial per output element
+#pragma unroll
+    for (int a = 0; a < TA; ++a)
+#pragma unroll
+        for (int b = 0; b < TB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[w][(a * TB + b) * 256 + (4 * q + r) * 16 + r16] = acc[a][b][r];
+    __syncthreads();
+    double* P = part + (int64_t)blockIdx.x * ma * mb;
+    for (int e = threadIdx.x; e < TA * TB * 256; e += 256) {
+        const int tile = e >> 8, a = tile / TB, b = tile % TB;
+        const int i = 16 * a + ((e & 255) >> 4), j = 16 * b + (e & 15);
+        if (i < ma && j < mb)
+            P[(int64_t)j * ma + i] = ((double)red[0][e] + (double)red[1][e]) + ((double)red[2][e] + (double)red[3][e]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_gram_reduce1(const double* __restrict__ part, int nks, int64_t tot, int step,
                                                       double* __restrict__ part2) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -280,6 +389,54 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ 
     *o = beta == 0.0 ? alpha * s : alpha * s + beta * *o;
 }
 
+template <int MODE, int TA, int TB>
+void launch_stream(dft_ctx* c, unsigned nblk, int64_t n, int ma, int mb, const float* a, int64_t lda, const float* b,
+                   int64_t ldb, int64_t kw) {
+    k_gram_stream<MODE, TA, TB><<<nblk, 256, 0, c->stream>>>(n, ma, mb, a, lda, b, ldb, kw, c->partials);
+}
+
+template <int MODE>
+void launch_stream_ta(dft_ctx* c, unsigned nblk, int64_t n, int ma, int mb, const float* a, int64_t lda, const float* b,
+                      int64_t ldb, int64_t kw) {
+    const int ta = (ma + 15) / 16, tb = (mb + 15) / 16;
+#define DFT_GS(TA)                                                                                     \
+    if (tb == 1) launch_stream<MODE, TA, 1>(c, nblk, n, ma, mb, a, lda, b, ldb, kw);                   \
+    else launch_stream<MODE, TA, 2>(c, nblk, n, ma, mb, a, lda, b, ldb, kw)
+    switch (ta) {
+        case 1: DFT_GS(1); break;
+        case 2: DFT_GS(2); break;
+        case 3: DFT_GS(3); break;
+        case 4: DFT_GS(4); break;
+        case 5: DFT_GS(5); break;
+        default: DFT_GS(6); break;
+    }
+#undef DFT_GS
+}
+
+int gram_stream(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, const float* a, int64_t lda,
+                const float* b, int64_t ldb, double beta, double* out, int64_t ldo) {
+    // two workgroups per CU, each wave a contiguous run of 32-point steps
+    const int64_t waves = 8LL * c->num_cus;
+    int64_t kw = (n + waves - 1) / waves;
+    kw = std::max<int64_t>(32, (kw + 31) / 32 * 32);
+    const int64_t nblk = (n + 4 * kw - 1) / (4 * kw);
+    const int64_t tot = (int64_t)ma * mb;
+    const int nseg = (int)std::min<int64_t>(nblk, 32), step = (int)((nblk + nseg - 1) / nseg);
+    if (int rc = ensure_partials(c, (size_t)(nblk + nseg) * tot + 8)) return rc;
+    if (mode == DFT_GRAM_SPLIT_BF16)
+        launch_stream_ta<DFT_GRAM_SPLIT_BF16>(c, (unsigned)nblk, n, ma, mb, a, lda, b, ldb, kw);
+    else
+        launch_stream_ta<DFT_GRAM_F32>(c, (unsigned)nblk, n, ma, mb, a, lda, b, ldb, kw);
+    DFT_CHECK_LAUNCH(c);
+    double* part2 = c->partials + nblk * tot;
+    k_gram_reduce1<<<dim3((unsigned)((tot + 255) / 256), nseg), 256, 0, c->stream>>>(c->partials, (int)nblk, tot, step,
+                                                                                    part2);
+    DFT_CHECK_LAUNCH(c);
+    k_gram_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(part2, nseg, ma, mb, alpha, beta, out, ldo);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -289,6 +446,9 @@ int dft_gram(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, cons
     if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, a && b && out && n > 0 && ma > 0 && mb > 0 && lda >= n && ldb >= n && ldo >= ma, "dft_gram: bad arguments");
     DFT_REQUIRE(c, mode >= DFT_GRAM_F64 && mode <= DFT_GRAM_SPLIT_BF16, "dft_gram: bad mode");
+    if (mode != DFT_GRAM_F64 && ma <= 96 && mb <= 32 && lda % 4 == 0 && ldb % 4 == 0 &&
+        reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0)
+        return gram_stream(c, mode, n, ma, mb, alpha, (const float*)a, lda, (const float*)b, ldb, beta, out, ldo);
     const int ntm = (ma + GT - 1) / GT, ntn = (mb + GT - 1) / GT;
     const int ntiles = ntm * ntn;
     const int ks = mode == DFT_GRAM_SPLIT_BF16 ? 32 : 16;

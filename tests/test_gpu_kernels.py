@@ -362,7 +362,8 @@ def test_nuclear_potential_singular(dev):
         dev.nuclear_potential(g, [[1, 0.0, 0.0, 0.0]], out)
 
 
-@pytest.mark.parametrize("n,ma,mb", [(5000, 8, 8), (70001, 45, 17), (300000, 96, 96)])
+@pytest.mark.parametrize("n,ma,mb", [(5000, 8, 8), (70001, 45, 17), (300000, 96, 96), (70000, 45, 17), (100004, 96, 32),
+                                     (400000, 87, 29), (49380, 1, 1), (333336, 17, 16)])
 def test_gram_modes(dev, n, ma, mb):
     import dft_amd
     rng = np.random.default_rng(n)
@@ -425,6 +426,23 @@ def test_ritz_residual(dev, n, m, k):
         assert rel(got, ref) <= tol * np.sqrt(m)
         assert np.max(np.abs(rn.download() - np.sum(got.astype(np.float64) ** 2, axis=1))) <= 1e-12 * np.max(
             np.sum(ref ** 2, axis=1))
+
+
+def test_gram_trailing_columns(dev):
+    """[V T]^T T with T the trailing columns of the same block (the Davidson's
+    Pythagorean Gram-Schmidt Gram), both FP32 modes."""
+    import dft_amd
+    from types import SimpleNamespace
+    n, m, nt = 200000, 60, 15
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((m + nt, n)).astype(np.float32)
+    ref = A.astype(np.float64) @ A[m:].astype(np.float64).T
+    dA = dev.array(A, dtype=np.float32)
+    T = SimpleNamespace(ptr=dA.col(m, n), prec=dA.prec)
+    out = dev.zeros((nt, m + nt))
+    for mode, tol in ((dft_amd.GRAM_F32, 1e-5), (dft_amd.GRAM_SPLIT_BF16, 1e-4)):
+        dev.gram(mode, dA, T, out, n, m + nt, nt)
+        assert rel(out.download().T, ref) <= tol
 
 
 @pytest.mark.parametrize("n,m,k,path", [(70000, 40, 13, "mfma"), (70001, 29, 8, "mfma"), (30000, 300, 40, "valu"),
