@@ -193,6 +193,18 @@ bool update_mfma_fits(int m, int k) {
     return k >= 1 && k <= 96 && (int64_t)((m + 3) / 4 * 4) * njt * 16 * (int64_t)sizeof(T) <= kUpdLdsBytes;
 }
 
+// largest inner-dimension chunk (a multiple of 4) whose C^T fits the LDS budget,
+// balanced over ceil(m / max) chunks; 0 when k itself is too wide
+template <typename T>
+int update_chunk(int m, int k) {
+    if (k < 1 || k > 96) return 0;
+    const int njt = (k + 15) / 16;
+    const int mx = (int)(kUpdLdsBytes / (njt * 16 * (int64_t)sizeof(T))) / 4 * 4;
+    if (mx < 4) return 0;
+    const int nch = (m + mx - 1) / mx;
+    return ((m + nch - 1) / nch + 3) / 4 * 4;
+}
+
 // Y = alpha [A | A2] C + beta Y (m1 inner rows from A); with norm2 != null,
 // norm2[j] = ||Y[:, j]||^2 (device FP64, k values) from the same pass.
 template <typename T>
@@ -685,9 +697,12 @@ int rocblas_for(dft_ctx* c, void** handle) {
 extern "C" {
 
 
-// Block update Y = alpha A C + beta Y (a plain tall-skinny GEMM, N x m times
-// m x k): rocBLAS' 64-bit-index GEMM on the context stream; FP32 blocks use
-// an FP32 copy of the small coefficient matrix.
+// Block update Y = alpha A C + beta Y (a tall-skinny GEMM, N x m times m x k).
+// k <= 96: the MFMA streaming kernel, in one pass when C^T fits the LDS budget,
+// else over chunks of the inner dimension accumulated into Y (Y apart from A);
+// FP32 blocks with k <= 64 and m <= 512 that alias A: the VALU kernel;
+// otherwise rocBLAS' 64-bit-index GEMM, through a staging copy when Y overlaps
+// A.  FP32 blocks use an FP32 copy of the small coefficient matrix.
 int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
                      const double* cm, int64_t ldc, double beta, void* y, int64_t ldy) {
     if (!c) return DFT_EINVAL;
@@ -704,7 +719,10 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
     char* yb = static_cast<char*>(y);
     const bool alias = yb < ab + es * (size_t)m * lda && ab < yb + es * (size_t)k * ldy;
     const bool mfma = prec == DFT_F64 ? update_mfma_fits<double>(m, k) : update_mfma_fits<float>(m, k);
-    const bool valu = prec == DFT_F32 && !mfma && k <= 64 && m <= 512;
+    // C beyond the LDS budget: the MFMA kernel over chunks of the inner dimension,
+    // accumulating into Y (not in place: a later chunk would read overwritten columns)
+    const int mc = mfma || alias ? 0 : (prec == DFT_F64 ? update_chunk<double>(m, k) : update_chunk<float>(m, k));
+    const bool valu = prec == DFT_F32 && !mfma && !mc && k <= 64 && m <= 512;
     const bool staged = alias && !mfma && !valu;
     const size_t off = prec == DFT_F32 ? (sizeof(float) * (size_t)m * k + 255) / 256 * 256 : 0;
     const size_t need = off + (staged ? es * (size_t)n * k : 0);
@@ -719,6 +737,15 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
         if (mfma)
             return update_mfma<double>(c, n, m, m, k, alpha, (const double*)a, lda, nullptr, 0, cm, (int)ldc, beta,
                                        (double*)y, ldy);
+        if (mc) {
+            for (int r0 = 0; r0 < m; r0 += mc) {
+                const int rows = std::min(mc, m - r0);
+                if (int rc = update_mfma<double>(c, n, rows, rows, k, alpha, (const double*)a + (int64_t)r0 * lda, lda,
+                                                 nullptr, 0, cm + r0, (int)ldc, r0 == 0 ? beta : 1.0, (double*)y, ldy))
+                    return rc;
+            }
+            return DFT_OK;
+        }
         st = rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &alpha, (const double*)a, lda,
                               cm, ldc, &beta, (double*)out, ldo);
     } else {
@@ -727,6 +754,15 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
         DFT_CHECK_LAUNCH(c);
         const float al = (float)alpha, be = (float)beta;
         if (mfma) return update_mfma<float>(c, n, m, m, k, al, (const float*)a, lda, nullptr, 0, c32, m, be, (float*)y, ldy);
+        if (mc) {
+            for (int r0 = 0; r0 < m; r0 += mc) {
+                const int rows = std::min(mc, m - r0);
+                if (int rc = update_mfma<float>(c, n, rows, rows, k, al, (const float*)a + (int64_t)r0 * lda, lda, nullptr,
+                                                0, c32 + r0, m, r0 == 0 ? be : 1.0f, (float*)y, ldy))
+                    return rc;
+            }
+            return DFT_OK;
+        }
         if (valu) return update_valu<float>(c, n, m, k, al, (const float*)a, lda, c32, m, be, (float*)y, ldy);
         st = rocblas_sgemm_64(h, rocblas_operation_none, rocblas_operation_none, n, k, m, &al, (const float*)a, lda, c32,
                               m, &be, (float*)out, ldo);
@@ -817,7 +853,8 @@ int dft_syev_lowest(dft_ctx* c, int m, int k, const double* a, int64_t lda, doub
 // R[:, j] = W Q[:, j] - theta[j] V Q[:, j] and rn2[j] = ||R[:, j]||^2 for
 // j < k.  One MFMA pass over [W | V] with the 2m x k coefficients
 // [Q ; -Q diag(theta)] when they fit the LDS budget (norms from the same
-// pass); otherwise R = W Q, then R += V (-Q theta), then the norms.
+// pass), else over chunks of those 2m rows (norms from the last chunk);
+// k > 96: R = W Q, then R += V (-Q theta), then the norms.
 int dft_ritz_residual(dft_ctx* c, int prec, int64_t n, int m, int k, const void* v, int64_t ldv, const void* w,
                       int64_t ldw, const double* q, int64_t ldq, const double* theta, void* r, int64_t ldr, double* rn2) {
     if (!c) return DFT_EINVAL;
@@ -841,6 +878,43 @@ int dft_ritz_residual(dft_ctx* c, int prec, int64_t n, int m, int k, const void*
         DFT_CHECK_LAUNCH(c);
         return update_mfma<float>(c, n, m, 2 * m, k, 1.0f, (const float*)w, ldw, (const float*)v, ldv, cf, 2 * m, 0.0f,
                                   (float*)r, ldr, rn2);
+    }
+    // 2m x k beyond the LDS budget: chunks of the [W | V] rows, the norms from the last chunk's epilogue
+    const int mc = prec == DFT_F64 ? update_chunk<double>(2 * m, k) : update_chunk<float>(2 * m, k);
+    if (mc) {
+        const size_t es = prec == DFT_F64 ? sizeof(double) : sizeof(float);
+        if (int rc = ensure_scratch(c, es * (size_t)2 * m * k)) return rc;
+        if (prec == DFT_F64) k_ritz_coef<double><<<gc, 256, 0, c->stream>>>(m, k, q, ldq, theta, (double*)c->scratch);
+        else k_ritz_coef<float><<<gc, 256, 0, c->stream>>>(m, k, q, ldq, theta, (float*)c->scratch);
+        DFT_CHECK_LAUNCH(c);
+        for (int r0 = 0; r0 < 2 * m; r0 += mc) {
+            const int rows = std::min(mc, 2 * m - r0), r1 = r0 + rows;
+            double* nrm = r1 == 2 * m ? rn2 : nullptr;
+            // rows r0..r1 of [W | V]: W's rows below m, V's from m
+            const int m1 = r0 >= m ? 0 : std::min(rows, m - r0);
+            int rc;
+            if (prec == DFT_F64) {
+                const double* a1 = r0 < m ? (const double*)w + (int64_t)r0 * ldw : (const double*)v + (int64_t)(r0 - m) * ldv;
+                const double* a2 = r0 < m ? (const double*)v : nullptr;
+                rc = m1 == 0 || m1 == rows
+                         ? update_mfma<double>(c, n, rows, rows, k, 1.0, a1, m1 == 0 ? ldv : ldw, nullptr, 0,
+                                               (const double*)c->scratch + r0, 2 * m, r0 == 0 ? 0.0 : 1.0, (double*)r, ldr,
+                                               nrm)
+                         : update_mfma<double>(c, n, m1, rows, k, 1.0, a1, ldw, a2, ldv, (const double*)c->scratch + r0,
+                                               2 * m, r0 == 0 ? 0.0 : 1.0, (double*)r, ldr, nrm);
+            } else {
+                const float* a1 = r0 < m ? (const float*)w + (int64_t)r0 * ldw : (const float*)v + (int64_t)(r0 - m) * ldv;
+                const float* a2 = r0 < m ? (const float*)v : nullptr;
+                rc = m1 == 0 || m1 == rows
+                         ? update_mfma<float>(c, n, rows, rows, k, 1.0f, a1, m1 == 0 ? ldv : ldw, nullptr, 0,
+                                              (const float*)c->scratch + r0, 2 * m, r0 == 0 ? 0.0f : 1.0f, (float*)r, ldr,
+                                              nrm)
+                         : update_mfma<float>(c, n, m1, rows, k, 1.0f, a1, ldw, a2, ldv, (const float*)c->scratch + r0,
+                                              2 * m, r0 == 0 ? 0.0f : 1.0f, (float*)r, ldr, nrm);
+            }
+            if (rc) return rc;
+        }
+        return DFT_OK;
     }
     // two passes; -Q theta sits past the FP32 coefficient copy dft_block_update keeps at the scratch start
     const size_t off = ((sizeof(float) * (size_t)m * k) + 255) / 256 * 256;

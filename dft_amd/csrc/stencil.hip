@@ -53,7 +53,12 @@ struct Coef {
 // V_eff read: J(t) = t + cf (r - (-1/2 Lap t + sigma t)), stored as
 // out = a J(t) (SM = 1) or out = a J(t) + b out (SM = 2: out holds the
 // previous iterate, read before it is overwritten: the Chebyshev recurrence).
-template <typename T, int TY, int NB, int SM = 0>
+// PF: the loads of plane k + 2 (centre) and k + 1 (halo, V_eff) are issued
+// before plane k is computed, so a wave keeps a plane of loads in flight.
+// Results are written with non-temporal stores: the output is not read again
+// by this launch, and streaming it past the caches raised the sustained
+// bandwidth by 5-15% on MI355X.
+template <typename T, int TY, int NB, int SM = 0, bool PF = false>
 __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, int nzc, Coef<T> co,
                                               const T* __restrict__ V, const T* __restrict__ psi, int64_t ld,
                                               T* __restrict__ out, int64_t ldo, const T* __restrict__ R = nullptr,
@@ -91,7 +96,22 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
     const T* P = psi + (int64_t)(grp * NB) * ld;
     T* Out = out + (int64_t)(grp * NB) * ldo;
 
+    // plane k's x-halo values, y-halo rows and V_eff (sigma when smoothing)
+    auto load_side = [&](int k, T (&h)[NB][TY], T (&lo)[NB], T (&hi)[NB], T (&vv)[TY]) {
+        const int kp = k * plane;
+#pragma unroll
+        for (int r = 0; r < TY; ++r) vv[r] = (!SMOOTH && xin && yin[r]) ? V[kp + roff[r]] : sigma;
+#pragma unroll
+        for (int o = 0; o < NB; ++o) {
+            const T* p = P + (int64_t)o * ld;
+#pragma unroll
+            for (int r = 0; r < TY; ++r) h[o][r] = (hin && yin[r]) ? p[kp + hoff[r]] : T(0);
+            lo[o] = (xin && ym_in) ? p[kp + ymo] : T(0);
+            hi[o] = (xin && yp_in) ? p[kp + ypo] : T(0);
+        }
+    };
     T prev[NB][TY], cur[NB][TY], nxt[NB][TY];
+    T hv[NB][TY], ym[NB], yp[NB], v[TY];
 #pragma unroll
     for (int o = 0; o < NB; ++o) {
         const T* p = P + (int64_t)o * ld;
@@ -100,33 +120,42 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
             const bool ok = xin && yin[r];
             prev[o][r] = (ok && k0 > 0) ? p[(k0 - 1) * plane + roff[r]] : T(0);
             cur[o][r] = ok ? p[k0 * plane + roff[r]] : T(0);
+            if constexpr (PF) nxt[o][r] = (ok && k0 + 1 < nz) ? p[(k0 + 1) * plane + roff[r]] : T(0);
         }
     }
+    if constexpr (PF) load_side(k0, hv, ym, yp, v);
     for (int k = k0; k < k1; ++k) {
         const int kp = k * plane;
-        const bool has_next = k + 1 < nz;
-        T v[TY];
+        T nn[NB][TY], hvn[NB][TY], ymn[NB], ypn[NB], vn[TY];
+        if constexpr (PF) {
+            const bool has_nn = k + 2 < nz && k + 1 < k1;
 #pragma unroll
-        for (int r = 0; r < TY; ++r) v[r] = (!SMOOTH && xin && yin[r]) ? V[kp + roff[r]] : sigma;
+            for (int o = 0; o < NB; ++o) {
+                const T* p = P + (int64_t)o * ld;
+#pragma unroll
+                for (int r = 0; r < TY; ++r) nn[o][r] = (xin && yin[r] && has_nn) ? p[kp + 2 * plane + roff[r]] : T(0);
+            }
+            if (k + 1 < k1) load_side(k + 1, hvn, ymn, ypn, vn);
+        } else {
+            const bool has_next = k + 1 < nz;
+#pragma unroll
+            for (int o = 0; o < NB; ++o) {
+                const T* p = P + (int64_t)o * ld;
+#pragma unroll
+                for (int r = 0; r < TY; ++r) nxt[o][r] = (xin && yin[r] && has_next) ? p[kp + plane + roff[r]] : T(0);
+            }
+            load_side(k, hv, ym, yp, v);
+        }
 #pragma unroll
         for (int o = 0; o < NB; ++o) {
-            const T* p = P + (int64_t)o * ld;
-            T hv[TY];
-#pragma unroll
-            for (int r = 0; r < TY; ++r) {
-                nxt[o][r] = (xin && yin[r] && has_next) ? p[kp + plane + roff[r]] : T(0);
-                hv[r] = (hin && yin[r]) ? p[kp + hoff[r]] : T(0);
-            }
-            const T ym = (xin && ym_in) ? p[kp + ymo] : T(0);
-            const T yp = (xin && yp_in) ? p[kp + ypo] : T(0);
             T* q = Out + (int64_t)o * ldo;
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
                 const T c = cur[o][r];
-                const T xm = shr1(c, hv[r]);
-                const T xp = shl1(c, hv[r]);
-                const T ymv = r > 0 ? cur[o][r > 0 ? r - 1 : 0] : ym;
-                const T ypv = r < TY - 1 ? cur[o][r < TY - 1 ? r + 1 : 0] : yp;
+                const T xm = shr1(c, hv[o][r]);
+                const T xp = shl1(c, hv[o][r]);
+                const T ymv = r > 0 ? cur[o][r > 0 ? r - 1 : 0] : ym[o];
+                const T ypv = r < TY - 1 ? cur[o][r < TY - 1 ? r + 1 : 0] : yp[o];
                 // KohnShamHamiltonianMolecule.cpp:88-187 order: centre, x-, x+, y-, y+, z-, z+
                 T lap = co.cc * c;
                 lap += xm * co.cx;
@@ -138,26 +167,44 @@ __global__ __launch_bounds__(256) void k_hpsi(int nx, int ny, int nz, int zc, in
                 const T kin = T(-0.5) * lap;
                 const T pot = v[r] * c;
                 if (xin && yin[r]) {
+                    T res;
                     if constexpr (SMOOTH) {
                         const T rv = R[(int64_t)(grp * NB + o) * ldr + kp + roff[r]];
                         const T jt = c + cf * (rv - (kin + pot));
-                        if constexpr (SM == 2) q[kp + roff[r]] = a * jt + b * q[kp + roff[r]];
-                        else q[kp + roff[r]] = a * jt;
+                        if constexpr (SM == 2) res = a * jt + b * q[kp + roff[r]];
+                        else res = a * jt;
                     } else {
-                        q[kp + roff[r]] = kin + pot;
+                        res = kin + pot;
                     }
+                    __builtin_nontemporal_store(res, q + kp + roff[r]);
                 }
             }
+        }
+#pragma unroll
+        for (int o = 0; o < NB; ++o)
 #pragma unroll
             for (int r = 0; r < TY; ++r) {
                 prev[o][r] = cur[o][r];
                 cur[o][r] = nxt[o][r];
+                if constexpr (PF) nxt[o][r] = nn[o][r];
+            }
+        if constexpr (PF) {
+            if (k + 1 < k1) {
+#pragma unroll
+                for (int o = 0; o < NB; ++o) {
+#pragma unroll
+                    for (int r = 0; r < TY; ++r) hv[o][r] = hvn[o][r];
+                    ym[o] = ymn[o];
+                    yp[o] = ypn[o];
+                }
+#pragma unroll
+                for (int r = 0; r < TY; ++r) v[r] = vn[r];
             }
         }
     }
 }
 
-template <typename T, int TY, int NB, int SM = 0>
+template <typename T, int TY, int NB, int SM = 0, bool PF = false>
 int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int ngroups, T* out,
                 int64_t ldo, const T* R = nullptr, int64_t ldr = 0, T sigma = T(0), T cf = T(0), T a = T(1),
                 T b = T(0)) {
@@ -176,7 +223,7 @@ int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
     nzc = (nz + zc - 1) / zc;
     DFT_REQUIRE(c, (int64_t)nzc * ngroups <= 65535 && gy <= 65535, "dft_hpsi: launch grid too large");
     dim3 grid(gx, gy, nzc * ngroups);
-    k_hpsi<T, TY, NB, SM><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo, R, ldr, sigma, cf,
+    k_hpsi<T, TY, NB, SM, PF><<<grid, 256, 0, c->stream>>>(nx, ny, nz, zc, nzc, co, V, psi, ld, out, ldo, R, ldr, sigma, cf,
                                                        a, b);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
@@ -184,16 +231,24 @@ int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
 
 // Tile shape: 4 y-rows per wave (a 64 x 16 column per workgroup, so the
 // y-halo is 2 rows per 16) and NB orbitals per group sharing each V_eff load.
-// Measured on MI355X at 160^3 x 32 and 256^3 x 96: NB = 2 (FP64, and FP32
-// for small blocks) or 4 (FP32 blocks of 64 and more) keeps the kernel at
-// 4-6 waves per SIMD; taller tiles or larger groups lose occupancy.
-template <typename T, int NB>
+// Measured on MI355X at 160^3 x 32 and 256^3 x 96 (non-temporal stores in
+// all variants): FP32 blocks below 64 orbitals run NB = 2 (4.4 TB/s at
+// 160^3 x 32); FP32 blocks of 64 and more NB = 4 with the one-plane prefetch
+// (4.7 TB/s at 256^3 x 96, against 4.0 without either); FP64 NB = 4 (4.2 and
+// 4.9 TB/s).  The remainder of a block runs in groups of 2, then 1.
+template <typename T, int NB, bool PF>
 int hpsi_groups(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int nb, T* out, int64_t ldo) {
     constexpr int TY = 4;
     const int ng = nb / NB;
     if (ng > 0)
-        if (int rc = launch_hpsi<T, TY, NB>(c, g, V, psi, ld, ng, out, ldo)) return rc;
-    for (int o = ng * NB; o < nb; ++o)
+        if (int rc = launch_hpsi<T, TY, NB, 0, PF>(c, g, V, psi, ld, ng, out, ldo)) return rc;
+    int o = ng * NB;
+    if (NB > 2 && nb - o >= 2) {
+        const int n2 = (nb - o) / 2;
+        if (int rc = launch_hpsi<T, TY, 2>(c, g, V, psi + (int64_t)o * ld, ld, n2, out + (int64_t)o * ldo, ldo)) return rc;
+        o += 2 * n2;
+    }
+    for (; o < nb; ++o)
         if (int rc = launch_hpsi<T, TY, 1>(c, g, V, psi + (int64_t)o * ld, ld, 1, out + (int64_t)o * ldo, ldo)) return rc;
     return DFT_OK;
 }
@@ -201,9 +256,11 @@ int hpsi_groups(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
 template <typename T>
 int hpsi_t(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int nb, T* out, int64_t ldo) {
     DFT_REQUIRE(c, g->nx * g->ny * g->nz < (1LL << 31), "dft_hpsi: grid exceeds 2^31 points");
-    if constexpr (sizeof(T) == 4)
-        if (nb >= 64) return hpsi_groups<T, 4>(c, g, V, psi, ld, nb, out, ldo);
-    return hpsi_groups<T, 2>(c, g, V, psi, ld, nb, out, ldo);
+    if constexpr (sizeof(T) == 4) {
+        if (nb >= 64) return hpsi_groups<T, 4, true>(c, g, V, psi, ld, nb, out, ldo);
+        return hpsi_groups<T, 2, false>(c, g, V, psi, ld, nb, out, ldo);
+    }
+    return hpsi_groups<T, 4, false>(c, g, V, psi, ld, nb, out, ldo);
 }
 
 // Kinetic smoothing correction: t = r / D, then `sweeps` damped-Jacobi steps

@@ -230,3 +230,26 @@ def test_scf_pulay_mixing_matches_linear(oracle):
     pul = dft_amd.scf_run(dgrid(og), h3, multiplicity=2, **common)
     print("doublet", lin["total"], pul["total"], lin["iterations"], pul["iterations"])
     assert lin["converged"] and pul["converged"] and abs(lin["total"] - pul["total"]) <= 1e-7
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32_split"])
+def test_davidson_free_box_full_size(prec):
+    """Block Davidson at the 40-atom cluster's shapes (256^3 grid, h = 0.18,
+    96 states, subspace up to 288 columns: basis offsets beyond 2^31 values)
+    with V_eff = 0, where the 7-point operator's spectrum is known in closed
+    form: sum over the axes of (1 - cos(pi p / (n + 1))) / h^2, p = 1..n.
+    The lowest 80 eigenvalues (degenerate clusters included) must match."""
+    import time
+    import dft_amd
+    n, h, ns, nc = 256, 0.18, 96, 80
+    g = dft_amd.Grid.centered(n, h)
+    l1 = (1.0 - np.cos(np.pi * np.arange(1, 13) / (n + 1))) / h ** 2
+    exact = np.sort((l1[:, None, None] + l1[None, :, None] + l1[None, None, :]).ravel())[:nc]
+    kw = dict(prec=dft_amd.F64, gram_mode=dft_amd.GRAM_F64, tol=1e-7) if prec == "f64" else \
+        dict(prec=dft_amd.F32, gram_mode=dft_amd.GRAM_SPLIT_BF16, tol=1e-4)
+    t = time.perf_counter()
+    r = dft_amd.davidson_run(g, np.zeros(g.size), nstates=ns, nconv=nc, max_subspace=3 * ns,
+                             precond=dft_amd.PRECOND_KINETIC_EXACT, **kw)
+    err = np.max(np.abs(r["eigenvalues"][:nc] - exact))
+    print(prec, "iterations", r["iterations"], "seconds", round(time.perf_counter() - t, 2), "max error", err)
+    assert err <= (1e-10 if prec == "f64" else 2e-6)
