@@ -194,6 +194,10 @@ class Device:
     def sync(self):
         self._chk(self.L.dft_sync(self.ctx))
 
+    def clamp_min(self, y, lo=0.0, n=None):
+        """y = max(y, lo) elementwise (NaN -> lo)."""
+        self._chk(self.L.dft_clamp_min(self.ctx, C.c_int(y.prec), C.c_int64(n or int(np.prod(y.shape))), C.c_double(lo), _vp(y.ptr)))
+
     def copy(self, dst, src, nbytes=None):
         """Device-to-device copy on the context stream."""
         nb = src.nbytes if nbytes is None else int(nbytes)
@@ -429,7 +433,8 @@ class ScfConfig(C.Structure):
                 ("energy_tol", C.c_double), ("init_guess", C.c_int), ("davidson_tol", C.c_double),
                 ("davidson_max_iter", C.c_int), ("refine_iterations", C.c_int), ("precond", C.c_int),
                 ("cnn", C.c_void_p), ("verbose", C.c_int), ("precond_sweeps", C.c_int),
-                ("precond_shift", C.c_double), ("mixer", C.c_int), ("mix_depth", C.c_int), ("mix_beta", C.c_double)]
+                ("precond_shift", C.c_double), ("mixer", C.c_int), ("mix_depth", C.c_int), ("mix_beta", C.c_double),
+                ("smearing", C.c_double), ("ortho_gram", C.c_int)]
 
 
 class ScfReport(C.Structure):
@@ -439,7 +444,8 @@ class ScfReport(C.Structure):
                 ("seconds_to_first_de", C.c_double), ("seconds_poisson", C.c_double), ("seconds_xc", C.c_double),
                 ("seconds_davidson", C.c_double), ("seconds_other", C.c_double), ("happly_columns", C.c_int64),
                 ("poisson_sweeps", C.c_int64), ("davidson_iterations", C.c_int64), ("cnn_applied", C.c_int64),
-                ("cnn_fallbacks", C.c_int64), ("block_passes", C.c_int64), ("block_fallbacks", C.c_int64)]
+                ("cnn_fallbacks", C.c_int64), ("block_passes", C.c_int64), ("block_fallbacks", C.c_int64),
+                ("fermi_level", C.c_double * 2), ("smearing_energy", C.c_double)]
 
 
 def _core_err(rc):
@@ -453,14 +459,15 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
             stop_rule=STOP_REFERENCE, energy_tol=1e-5,
             init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
             precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False, precond_sweeps=0, precond_shift=0.0,
-            mixer=MIX_LINEAR, mix_depth=0, mix_beta=0.0):
-    """Run the device-resident SCF; returns a dict of results."""
+            mixer=MIX_LINEAR, mix_depth=0, mix_beta=0.0, smearing=0.0, ortho_gram=0):
+    """Run the device-resident SCF; returns a dict of results.  smearing: Fermi-Dirac kT in Ha (0: whole
+    orbitals, the reference's rule); with smearing the total is the free energy E - TS."""
     L = core_lib()
     nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
                     poisson_fixed, poisson_batch, poisson_method, poisson_tol, stop_rule, energy_tol, init_guess, davidson_tol,
                     davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0,
-                    precond_sweeps, precond_shift, mixer, mix_depth, mix_beta)
+                    precond_sweeps, precond_shift, mixer, mix_depth, mix_beta, smearing, ortho_gram)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
     eig = np.zeros(ne + 2)
@@ -473,7 +480,8 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
     if rc != 0:
         _core_err(rc)
     it = rep.iterations
-    out = {k: getattr(rep, k) for k, _ in ScfReport._fields_ if k != "energy"}
+    out = {k: getattr(rep, k) for k, _ in ScfReport._fields_ if k not in ("energy", "fermi_level")}
+    out["fermi_level"] = list(rep.fermi_level)
     out["energy"] = list(rep.energy)
     out["total"] = rep.energy[5]
     out["converged"] = bool(rep.converged)
@@ -484,6 +492,19 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
     if rho is not None:
         out["rho_alpha"], out["rho_beta"] = rho[: grid.size], rho[grid.size:]
     return out
+
+
+def fermi_occupations(eps, nelec, kT):
+    """Fermi-Dirac occupations (each in [0, 1]) of one spin channel summing to nelec; returns (occ, mu, entropy)."""
+    L = core_lib()
+    e = np.ascontiguousarray(eps, dtype=np.float64)
+    occ = np.zeros(e.size)
+    mu, S = C.c_double(), C.c_double()
+    rc = L.dft_fermi_occupations(e.ctypes.data_as(_dp), C.c_int(e.size), C.c_double(nelec), C.c_double(kT),
+                                 occ.ctypes.data_as(_dp), C.byref(mu), C.byref(S))
+    if rc != 0:
+        _core_err(rc)
+    return occ, mu.value, S.value
 
 
 def davidson_run(grid, veff, nstates, nconv=None, prec=F64, gram_mode=GRAM_F64, max_subspace=0, tol=1e-7,

@@ -108,6 +108,12 @@ __global__ __launch_bounds__(256) void k_axpby(int64_t n, double alpha, const T*
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_clamp_min(int64_t n, T lo, T* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && !(y[i] >= lo)) y[i] = lo;  // NaN becomes lo too
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_add3(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
                                               const T* __restrict__ c, T* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -304,6 +310,16 @@ int dft_axpby(dft_ctx* c, int prec, int64_t n, double alpha, const void* x, doub
     if (!n) return DFT_OK;
     if (prec == DFT_F64) k_axpby<double><<<grid1d(n), 256, 0, c->stream>>>(n, alpha, (const double*)x, beta, (double*)y);
     else k_axpby<float><<<grid1d(n), 256, 0, c->stream>>>(n, alpha, (const float*)x, beta, (float*)y);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+int dft_clamp_min(dft_ctx* c, int prec, int64_t n, double lo, void* y) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, n >= 0 && y && (prec == DFT_F64 || prec == DFT_F32), "dft_clamp_min: bad args");
+    if (!n) return DFT_OK;
+    if (prec == DFT_F64) k_clamp_min<double><<<grid1d(n), 256, 0, c->stream>>>(n, lo, (double*)y);
+    else k_clamp_min<float><<<grid1d(n), 256, 0, c->stream>>>(n, (float)lo, (float*)y);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }

@@ -408,19 +408,19 @@ int dft_hpsi(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const vo
     return set_err(c, DFT_EINVAL, "dft_hpsi: bad precision");
 }
 
-int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
-                       const int* electrons_host, double* t_host) {
+int dft_kinetic_energy_occ(dft_ctx* c, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
+                           const double* occ_host, double* t_host) {
     if (!c) return DFT_EINVAL;
-    DFT_REQUIRE(c, valid_grid(g) && psi && t_host && nb >= 0 && (nb == 0 || electrons_host), "dft_kinetic_energy: bad arguments");
+    DFT_REQUIRE(c, valid_grid(g) && psi && t_host && nb >= 0 && (nb == 0 || occ_host) && (prec == DFT_F64 || prec == DFT_F32),
+                "dft_kinetic_energy: bad arguments");
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, ld >= n, "dft_kinetic_energy: ld < grid size");
     if (nb == 0) { *t_host = 0.0; return DFT_OK; }
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4 * (int64_t)c->num_cus);
     if (int rc = ensure_partials(c, (size_t)blocks + 1 + (size_t)nb)) return rc;
     double* el = c->partials + blocks + 1;
-    std::vector<double> elh(nb);
-    for (int o = 0; o < nb; ++o) elh[o] = (double)electrons_host[o];
-    DFT_HIP(c, hipMemcpyAsync(el, elh.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    // the copy is synchronised below (finish_sum) before the host array can go away
+    DFT_HIP(c, hipMemcpyAsync(el, occ_host, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
     const double dx2 = g->dx * g->dx, dy2 = g->dy * g->dy, dz2 = g->dz * g->dz;
     if (prec == DFT_F64)
         k_kinetic<double><<<blocks, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const double*)psi, ld, nb, el, c->partials);
@@ -431,6 +431,15 @@ int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi,
     if (int rc = finish_sum(c, blocks, &s)) return rc;
     *t_host = s * (g->dx * g->dy * g->dz);
     return DFT_OK;
+}
+
+int dft_kinetic_energy(dft_ctx* c, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
+                       const int* electrons_host, double* t_host) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, nb >= 0 && (nb == 0 || electrons_host), "dft_kinetic_energy: bad arguments");
+    std::vector<double> elh(nb);
+    for (int o = 0; o < nb; ++o) elh[o] = (double)electrons_host[o];
+    return dft_kinetic_energy_occ(c, g, prec, psi, ld, nb, elh.data(), t_host);
 }
 
 int dft_precond_kinetic(dft_ctx* c, const dft_grid* g, int prec, double sigma, double omega, int sweeps, const void* r,

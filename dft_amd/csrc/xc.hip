@@ -55,7 +55,9 @@ __global__ __launch_bounds__(256) void k_xc_point(GridDev g, int functional, con
             const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / plane);
             const T ga = functional == DFT_XC_PBE96 ? gradmag(na, p, i, j, k, g) : T(0);
             const T gb = functional == DFT_XC_PBE96 ? gradmag(nb, p, i, j, k, g) : T(0);
-            if (sizeof(T) == 4 && (double)tot < xc::kF32Floor) {
+            // FP64 evaluation for small densities and for extreme relative gradients
+            // (the GGA's t^6 terms overflow FP32 beyond |grad n| / n ~ 1e6)
+            if (sizeof(T) == 4 && ((double)tot < xc::kF32Floor || (double)ga + (double)gb > 1.0e4 * (double)tot)) {
                 const xc::Out<double> r = xc::evaluate<double>(functional, (double)a, (double)b, (double)ga, (double)gb);
                 oa = (T)r.va; ob = (T)r.vb; oca = (T)r.ca; ocb = (T)r.cb;
                 e += ((double)tot * r.eps) * dv;

@@ -4,7 +4,8 @@
 // agrees with the CPU reference to a few ulps of the math library.  With
 // T = float, log(1 + 1/q1) in the PW92 interpolation is evaluated as
 // log1p(1/q1), the only change needed to keep FP32 finite at low density;
-// callers fall back to T = double below kF32Floor.
+// callers fall back to T = double below kF32Floor and for extreme relative
+// gradients (|grad n| / n > 1e4, where the GGA's t^6 terms overflow FP32).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -44,6 +44,16 @@ extern "C" {
 
 const char* dft_core_last_error(void) { return g_err.c_str(); }
 
+int dft_fermi_occupations(const double* eps, int n, double nelec, double kT, double* occ, double* mu, double* entropy) {
+    return guarded([&] {
+        if (!eps || !occ || n <= 0 || !(kT > 0.0) || !(nelec >= 0.0) || nelec > n)
+            throw std::invalid_argument("dft_fermi_occupations: bad arguments");
+        const double m = dftc::fermi_occupations(eps, n, nelec, kT, occ);
+        if (mu) *mu = m;
+        if (entropy) *entropy = dftc::fermi_entropy(occ, n);
+    });
+}
+
 int dft_scf_run(const dft_grid* g, int natoms, const double* nuclei, const dft_scf_config* cfg, dft_scf_report* rep,
                 double* eig, double* trace, int trace_cap, double* rho) {
     return guarded([&] {

@@ -41,6 +41,11 @@ void BlockDavidson::gram(int ma, int mb, const void* A, const void* B, double* o
     check(c_, dft_gram(c_, mode, n_, ma, mb, 1.0, A, n_, B, n_, beta, out, ldo), "dft_gram");
 }
 
+void BlockDavidson::ogram(int ma, int mb, const void* A, const void* B, double* out, int64_t ldo) {
+    const int mode = prec_ == DFT_F64 ? DFT_GRAM_F64 : ortho_mode_;
+    check(c_, dft_gram(c_, mode, n_, ma, mb, 1.0, A, n_, B, n_, 0.0, out, ldo), "dft_gram");
+}
+
 void BlockDavidson::load_X(const void* src, int src_prec, int ncols) {
     ncols = std::min(ncols, k_);
     check(c_, dft_convert(c_, src, src_prec, X_.get(), prec_, n_ * (int64_t)ncols), "dft_convert");
@@ -58,13 +63,13 @@ int BlockDavidson::orthonormalise(void* src, int cnt, void* dst, void* tmp) {
     for (int pass = 0; pass < 2; ++pass) {
         void* out = pass == 0 ? tmp : dst;
         double* S = S_.as<double>();
-        gram(cnt, cnt, in, in, S, cnt);
+        ogram(cnt, cnt, in, in, S, cnt);
         const int rc = dft_chol_inv(c_, cnt, S, cnt);
         if (rc == DFT_OK) {
             check(c_, dft_block_update(c_, prec_, n_, cnt, cnt, 1.0, in, n_, S, cnt, 0.0, out, n_), "dft_block_update");
         } else if (rc == DFT_ENUMERIC) {
             // rank-deficient block: keep eigen-directions above a relative floor
-            gram(cnt, cnt, in, in, S, cnt);
+            ogram(cnt, cnt, in, in, S, cnt);
             check(c_, dft_syev(c_, cnt, S, cnt, theta_.as<double>() /* scratch */), "dft_syev");
             std::vector<double> lam(cnt), U((size_t)cnt * cnt);
             check(c_, dft_download(c_, lam.data(), theta_.get(), sizeof(double) * cnt), "dft_download");
@@ -97,11 +102,13 @@ int BlockDavidson::append_block(int m, int nt, DavidsonStats& st) {
     char* T = V + (size_t)m * colb;
     const int mt = m + nt;
     double* G1 = P_.as<double>();  // (m + nt) x nt, ld m + nt (P_ holds mmax x k)
-    // accuracy of one Gram entry relative to the column norms (FP64, split-BF16 / FP32)
-    const double eps_g = prec_ == DFT_F64 ? 2.0e-16 : 1.0e-7;
+    // accuracy of one Gram entry relative to the column norms: FP64; FP32 products;
+    // split-BF16 (operands kept to ~2^-17, which for orbitals concentrated on a few
+    // grid points is also the entry's relative accuracy)
+    const double eps_g = prec_ == DFT_F64 ? 2.0e-16 : ortho_mode_ == DFT_GRAM_SPLIT_BF16 ? 1.0e-5 : 1.0e-7;
     std::vector<double> g((size_t)mt * nt), L((size_t)nt * nt), Ri((size_t)nt * nt), C((size_t)mt * nt);
     for (int pass = 0; pass < 2; ++pass) {
-        gram(mt, nt, V, T, G1, mt);
+        ogram(mt, nt, V, T, G1, mt);
         check(c_, dft_download(c_, g.data(), G1, sizeof(double) * g.size()), "dft_download");
         auto G = [&](int i, int j) { return g[(size_t)j * mt + i]; };  // rows 0..m-1: V^T T; m..: T^T T
         // S' = T^T T - (V^T T)^T (V^T T), the Gram matrix of the projected block;
@@ -175,7 +182,7 @@ int BlockDavidson::append_block(int m, int nt, DavidsonStats& st) {
     ++st.block_fallbacks;
     check(c_, dft_copy(c_, T_.get(), T, colb * nt), "dft_copy");
     for (int pass = 0; pass < 2; ++pass) {
-        gram(m, nt, V, T_.get(), P_.as<double>(), mmax_);
+        ogram(m, nt, V, T_.get(), P_.as<double>(), mmax_);
         check(c_, dft_block_update(c_, prec_, n_, m, nt, -1.0, V, n_, P_.as<double>(), mmax_, 1.0, T_.get(), n_),
               "dft_block_update");
     }
@@ -266,6 +273,7 @@ int KineticPreconditioner::apply(dft_ctx* c, const dft_grid& g, int prec, const 
 DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOptions& opt, Preconditioner& P) {
     DavidsonStats st;
     gram_mode_ = opt.gram_mode;
+    ortho_mode_ = opt.ortho_gram_mode >= 0 ? opt.ortho_gram_mode : opt.gram_mode;
     const int k = k_;
     const size_t colb = (size_t)n_ * es_;
     char* V = static_cast<char*>(V_.get());
@@ -285,7 +293,7 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
         check(c_, dft_random_vectors(c_, prec_, n_, need, 0x5eedULL + (uint64_t)m, T_.get(), n_), "dft_random_vectors");
         if (m > 0)
             for (int pass = 0; pass < 2; ++pass) {
-                gram(m, need, V, T_.get(), P_.as<double>(), mmax_);
+                ogram(m, need, V, T_.get(), P_.as<double>(), mmax_);
                 check(c_, dft_block_update(c_, prec_, n_, m, need, -1.0, V, n_, P_.as<double>(), mmax_, 1.0, T_.get(), n_),
                       "dft_block_update");
             }
@@ -315,7 +323,24 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     const bool px = P.needs_ritz_vectors();
     for (int it = 1; it <= opt.max_iter; ++it) {
         st.iterations = it;
-        check(c_, dft_syev_lowest(c_, m, k, G, mmax_, Q, mmax_, theta), "dft_syev_lowest");
+        if (int rc = dft_syev_lowest(c_, m, k, G, mmax_, Q, mmax_, theta)) {
+            // say what the projected matrix looked like
+            std::vector<double> gh((size_t)mmax_ * m);
+            check(c_, dft_download(c_, gh.data(), G, sizeof(double) * gh.size()), "dft_download");
+            int bad = 0;
+            double big = 0.0, asym = 0.0;
+            for (int j = 0; j < m; ++j)
+                for (int i = 0; i < m; ++i) {
+                    const double x = gh[(size_t)j * mmax_ + i];
+                    if (!std::isfinite(x)) ++bad;
+                    else big = std::max(big, std::fabs(x));
+                    if (i > j) asym = std::max(asym, std::fabs(x - gh[(size_t)i * mmax_ + j]));
+                }
+            throw std::runtime_error("Davidson: Rayleigh-Ritz eigensolve failed (iteration " + std::to_string(it) +
+                                     ", subspace " + std::to_string(m) + ", " + std::to_string(bad) +
+                                     " non-finite entries, max |G| " + std::to_string(big) + ", max asymmetry " +
+                                     std::to_string(asym) + "): " + std::string(dft_last_error(c_)));
+        }
         m_rr = m;
         x_fresh = false;
         if (px) ritz_block(false);

@@ -91,6 +91,9 @@ struct DavidsonOptions {
     double tol = 1.0e-7;   // EigenvalueSolverMolecule.cpp:19
     int max_iter = 200;
     int gram_mode = DFT_GRAM_F64;
+    // Gram mode of the orthonormalisation products (Pythagorean block
+    // Gram-Schmidt, CholQR) of FP32 blocks; -1: gram_mode
+    int ortho_gram_mode = -1;
 };
 
 struct DavidsonStats {
@@ -125,6 +128,8 @@ class BlockDavidson {
    private:
     void hpsi(const void* veff, const void* in, void* out, int ncols);
     void gram(int ma, int mb, const void* A, const void* B, double* out, int64_t ldo, double beta = 0.0);
+    // the orthonormalisation products (mode ortho_mode_ for FP32 blocks)
+    void ogram(int ma, int mb, const void* A, const void* B, double* out, int64_t ldo);
     // Orthonormalise `cnt` columns at src into dst (CholQR twice, with an
     // eigenvalue filter if the block is rank deficient); returns kept count.
     int orthonormalise(void* src, int cnt, void* dst, void* tmp);
@@ -148,6 +153,7 @@ class BlockDavidson {
     Buf G_, Q_, P_, S_, theta_, rn_, Qprev_, Cb_;
     std::vector<double> theta_h_, rn_h_;
     int gram_mode_ = DFT_GRAM_F64;
+    int ortho_mode_ = DFT_GRAM_F64;
 };
 
 }  // namespace dftc

@@ -21,8 +21,59 @@ constexpr double kMixing0 = 0.25, kMinMix = 0.10, kMaxMix = 0.50, kMixUp = 1.10,
 constexpr double kEnergyTol = 1.0e-11, kDensityTol = 1.0e-9, kKsTol = 1.0e-7;
 // mixed mode: where the FP32 phase hands over to FP64
 constexpr double kF32EnergyFloor = 3.0e-4, kF32DensityFloor = 1.0e-5;
+// ... or, below this relative density change, once three FP32 steps in a row
+// have not lowered it by 10% (the FP32 phase has reached its own noise floor)
+constexpr double kF32StallDensity = 1.0e-3;
 
 }  // namespace
+
+double fermi_occupations(const double* eps, int n, double nelec, double kT, double* occ) {
+    auto fill = [&](double mu) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double x = (eps[i] - mu) / kT;
+            // both branches stay finite: exp of a non-positive argument
+            occ[i] = x > 0.0 ? std::exp(-x) / (1.0 + std::exp(-x)) : 1.0 / (1.0 + std::exp(x));
+            s += occ[i];
+        }
+        return s;
+    };
+    const double lo0 = *std::min_element(eps, eps + n), hi0 = *std::max_element(eps, eps + n);
+    double lo = lo0 - 50.0 * kT - 1.0, hi = hi0 + 50.0 * kT + 1.0;
+    if (nelec >= n) {
+        std::fill(occ, occ + n, 1.0);
+        return hi;
+    }
+    if (nelec <= 0.0) {
+        std::fill(occ, occ + n, 0.0);
+        return lo;
+    }
+    // The electron count rises monotonically with mu but, in floating point,
+    // is flat across a gap much wider than kT: take the midpoint of the
+    // smallest mu with count >= nelec and the largest with count <= nelec
+    // (equal to rounding without a gap; mid-gap, with whole orbitals, across one).
+    auto bisect = [&](bool upper) {
+        double a = lo, b = hi;
+        for (int it = 0; it < 200 && b - a > 1e-15 * std::max(1.0, std::fabs(a) + std::fabs(b)); ++it) {
+            const double mid = 0.5 * (a + b);
+            const double cnt = fill(mid);
+            ((upper ? cnt <= nelec : cnt < nelec) ? a : b) = mid;
+        }
+        return 0.5 * (a + b);
+    };
+    const double mu = 0.5 * (bisect(false) + bisect(true));
+    fill(mu);
+    return mu;
+}
+
+double fermi_entropy(const double* occ, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double o = occ[i];
+        if (o > 0.0 && o < 1.0) s -= o * std::log(o) + (1.0 - o) * std::log(1.0 - o);
+    }
+    return s;
+}
 
 // Pulay (DIIS) density mixing (P. Pulay, Chem. Phys. Lett. 73, 393 (1980)):
 // with the last `depth` input densities rho_i and residuals R_i = rho_out,i -
@@ -93,11 +144,15 @@ class PulayMixer {
         check(c_, dft_upload(c_, coef_.get(), C.data(), sizeof(double) * C.size()), "dft_upload");
         check(c_, dft_block_update(c_, prec_, n_, 2 * depth_, 1, 1.0, h, L_, coef_.as<double>(), 2 * depth_, 0.0, ra, n_),
               "dft_block_update");
+        // the extrapolation can take the density below zero in its tails
+        check(c_, dft_clamp_min(c_, prec_, n_, 0.0, ra), "dft_clamp_min");
         if (closed_) check(c_, dft_copy(c_, rb, ra, fb), "dft_copy");
-        else
+        else {
             check(c_, dft_block_update(c_, prec_, n_, 2 * depth_, 1, 1.0, h + fb, L_, coef_.as<double>(), 2 * depth_, 0.0,
                                        rb, n_),
                   "dft_block_update");
+            check(c_, dft_clamp_min(c_, prec_, n_, 0.0, rb), "dft_clamp_min");
+        }
         head_ = (head_ + 1) % depth_;
     }
 
@@ -209,6 +264,13 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     if (nbe <= 0) throw std::invalid_argument("El numero de orbitales debe ser mayor que cero.");
     const int nocc_max = std::max(nal, nbe);
     const int k = std::max(cfg_.nstates > 0 ? cfg_.nstates : nocc_max, nocc_max);
+    const double kT = cfg_.smearing;
+    const bool smear = kT > 0.0;
+    if (!(kT >= 0.0) || !std::isfinite(kT)) throw std::invalid_argument("smearing must be >= 0");
+    if (cfg_.ortho_gram != 0 && cfg_.ortho_gram != DFT_GRAM_F32 && cfg_.ortho_gram != DFT_GRAM_SPLIT_BF16)
+        throw std::invalid_argument("ortho_gram must be 0, DFT_GRAM_F32 or DFT_GRAM_SPLIT_BF16");
+    if (smear && k < nocc_max + 2)
+        throw std::invalid_argument("smearing needs empty states: nstates must exceed the occupied count by 2 or more");
     const int mmax = cfg_.max_subspace > 0 ? cfg_.max_subspace : std::max(3 * k, k + 16);
     const int maxit = cfg_.max_iterations > 0 ? cfg_.max_iterations : 250;
     const int batch = cfg_.poisson_batch > 0 ? cfg_.poisson_batch : 64;
@@ -241,6 +303,36 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
 
     std::vector<double> occ_a(nal, 1.0 / dv), occ_b(nbe, 1.0 / dv);  // unit-norm vectors -> psi^2 = u^2/dV
     std::vector<int> el_a(nal, 1), el_b(nbe, 1);
+    // smearing: per-channel occupations of the k states (0..1), the states the
+    // density and the energy take (up to the last with occ > 1e-14) and the
+    // states the next Davidson must converge (occ > 1e-6, leaving the top 4 of
+    // the block as buffer where their occupations are below 1e-3)
+    struct Occ {
+        std::vector<double> o, w;  // occupations; o / dV for the density kernel
+        int used = 0, conv = 0;
+        double mu = 0.0, entropy = 0.0;
+    };
+    Occ sm_a, sm_b;
+    sm_a.conv = nal;
+    sm_b.conv = nbe;
+    auto occupy = [&](Occ& q, const std::vector<double>& eps, int nel) {
+        q.o.assign(k, 0.0);
+        q.mu = fermi_occupations(eps.data(), k, (double)nel, kT, q.o.data());
+        q.entropy = fermi_entropy(q.o.data(), k);
+        int must = 0;
+        q.used = q.conv = 0;
+        for (int i = 0; i < k; ++i) {
+            if (q.o[i] > 1e-14) q.used = i + 1;
+            if (q.o[i] > 1e-6) q.conv = i + 1;
+            if (q.o[i] > 1e-3) must = i + 1;
+        }
+        // keep a few unconverged states at the top of the block as Davidson's
+        // buffer, unless they hold more than 1e-3 electrons
+        q.conv = std::max({std::min(q.conv, k - 4), must, nel});
+        q.used = std::max(q.used, nel);
+        q.w.resize(q.used);
+        for (int i = 0; i < q.used; ++i) q.w[i] = q.o[i] / dv;
+    };
     DavidsonOptions dop;
     dop.tol = cfg_.davidson_tol > 0 ? cfg_.davidson_tol : 1.0e-7;
     dop.max_iter = cfg_.davidson_max_iter > 0 ? cfg_.davidson_max_iter : 200;
@@ -258,6 +350,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         o.nconv = nocc;
         if (f.prec == DFT_F32) {
             o.gram_mode = DFT_GRAM_SPLIT_BF16;
+            o.ortho_gram_mode = cfg_.ortho_gram == DFT_GRAM_SPLIT_BF16 ? DFT_GRAM_SPLIT_BF16 : DFT_GRAM_F32;
             o.tol = std::max(o.tol, 3.0e-4);
             // a warm start needs few iterations; a cold start (seed vectors) keeps the full budget
             if (have_prev) o.max_iter = std::min(o.max_iter, 60);
@@ -266,8 +359,13 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         // well beyond it: energy errors go with the square of the residual,
         // so the tolerance never exceeds 0.3 sqrt(etol) (else a loose solve
         // that leaves the orbitals unchanged gives dE = 0 by itself)
-        if (cfg_.stop_rule == DFT_STOP_DE)
-            o.tol = std::max(o.tol, std::min({1.0e-3, 0.1 * dd_prev, 0.3 * std::sqrt(etol)}));
+        if (cfg_.stop_rule == DFT_STOP_DE) {
+            // With smeared occupations a residual r turns orbitals within ~kT of
+            // each other by ~r / gap while their occupations differ by ~gap / 4kT:
+            // the density moves by ~r / 4kT, so r must stay below ~0.4 kT of dRho
+            const double dd_tol = smear ? std::min(0.1, 0.4 * kT) * dd_prev : 0.1 * dd_prev;
+            o.tol = std::max(o.tol, std::min({1.0e-3, dd_tol, 0.3 * std::sqrt(etol)}));
+        }
         DavidsonStats st = D.solve(veff, have_prev, o, *precond);
         rep.happly_columns += st.happly_columns;
         rep.block_passes += st.block_passes;
@@ -345,6 +443,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     std::unique_ptr<PulayMixer> pulay;  // rebuilt for each precision phase
     int pulay_prec = -1;
     bool refining = false;
+    double best_dd = std::numeric_limits<double>::infinity();  // FP32 phase: lowest dRho so far
+    int stall = 0;
     int refine_steps = 0;
     for (int it = 1; it <= maxit + (mixed ? std::max(cfg_.refine_iterations, 0) : 0); ++it) {
         if (!refining && it > maxit) break;
@@ -361,17 +461,34 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         check(c_, dft_xc(c_, &g_, p, cfg_.functional, f.ra.get(), f.rb.get(), f.vxa.get(), closed ? nullptr : f.vxb.get(),
                          nullptr), "dft_xc");
         tick(t_xc, t);
+        // a non-finite point anywhere in V_eff would poison every Davidson product: name the field
+        {
+            double sr = 0, sh = 0, sx = 0;
+            check(c_, dft_reduce(c_, DFT_RED_SUM, p, f.rho.get(), nullptr, n, &sr), "dft_reduce");
+            check(c_, dft_reduce(c_, DFT_RED_SUM, p, f.vh.get(), nullptr, n, &sh), "dft_reduce");
+            check(c_, dft_reduce(c_, DFT_RED_SUM, p, f.vxa.get(), nullptr, n, &sx), "dft_reduce");
+            if (!std::isfinite(sr) || !std::isfinite(sh) || !std::isfinite(sx))
+                throw std::runtime_error("SCF step " + std::to_string(it) + ": non-finite input field (sum rho " +
+                                         std::to_string(sr) + ", sum V_H " + std::to_string(sh) + ", sum V_xc " +
+                                         std::to_string(sx) + ")");
+        }
         // :92-115
         check(c_, dft_add3(c_, p, n, f.vnuc.get(), f.vh.get(), f.vxa.get(), f.va.get()), "dft_add3");
         if (!closed) check(c_, dft_add3(c_, p, n, f.vnuc.get(), f.vh.get(), f.vxb.get(), f.vb.get()), "dft_add3");
         // :117-144
-        DavidsonStats sa = davidson(f, *f.da, f.va.get(), nal), sb;
-        if (!closed) sb = davidson(f, *f.db, f.vb.get(), nbe);
+        DavidsonStats sa = davidson(f, *f.da, f.va.get(), smear ? sm_a.conv : nal), sb;
+        if (!closed) sb = davidson(f, *f.db, f.vb.get(), smear ? sm_b.conv : nbe);
         have_prev = true;
+        if (smear) {
+            occupy(sm_a, f.da->eigenvalues(), nal);
+            if (!closed) occupy(sm_b, f.db->eigenvalues(), nbe);
+        }
         tick(t_dav, t);
         // :163-193
-        check(c_, dft_density(c_, p, f.da->X(), n, nal, occ_a.data(), n, f.ora.get(), 0), "dft_density");
+        if (smear) check(c_, dft_density(c_, p, f.da->X(), n, sm_a.used, sm_a.w.data(), n, f.ora.get(), 0), "dft_density");
+        else check(c_, dft_density(c_, p, f.da->X(), n, nal, occ_a.data(), n, f.ora.get(), 0), "dft_density");
         if (closed) check(c_, dft_copy(c_, f.orb.get(), f.ora.get(), fb), "dft_copy");
+        else if (smear) check(c_, dft_density(c_, p, f.db->X(), n, sm_b.used, sm_b.w.data(), n, f.orb.get(), 0), "dft_density");
         else check(c_, dft_density(c_, p, f.db->X(), n, nbe, occ_b.data(), n, f.orb.get(), 0), "dft_density");
         check(c_, dft_add3(c_, p, n, f.ora.get(), f.orb.get(), nullptr, f.orho.get()), "dft_add3");
         poisson(f, f.orho.get(), f.ovh.get());
@@ -380,8 +497,10 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         double E[6];
         {
             double ta = 0, tb = 0;
-            check(c_, dft_kinetic_energy(c_, &g_, p, f.da->X(), n, nal, el_a.data(), &ta), "dft_kinetic_energy");
+            if (smear) check(c_, dft_kinetic_energy_occ(c_, &g_, p, f.da->X(), n, sm_a.used, sm_a.o.data(), &ta), "dft_kinetic_energy");
+            else check(c_, dft_kinetic_energy(c_, &g_, p, f.da->X(), n, nal, el_a.data(), &ta), "dft_kinetic_energy");
             if (closed) tb = ta;
+            else if (smear) check(c_, dft_kinetic_energy_occ(c_, &g_, p, f.db->X(), n, sm_b.used, sm_b.o.data(), &tb), "dft_kinetic_energy");
             else check(c_, dft_kinetic_energy(c_, &g_, p, f.db->X(), n, nbe, el_b.data(), &tb), "dft_kinetic_energy");
             E[0] = (ta + tb) / dv;
             double s = 0;
@@ -391,6 +510,13 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             check(c_, dft_xc(c_, &g_, p, cfg_.functional, f.ora.get(), f.orb.get(), f.ta.get(), f.tb.get(), &E[3]), "dft_xc");
             E[4] = enn;
             E[5] = E[0] + E[1] + E[2] + E[3] + E[4];
+            if (smear) {
+                // free energy E - TS, TS = kT (S_alpha + S_beta)
+                rep.smearing_energy = -kT * (sm_a.entropy + (closed ? sm_a.entropy : sm_b.entropy));
+                rep.fermi_level[0] = sm_a.mu;
+                rep.fermi_level[1] = closed ? sm_a.mu : sm_b.mu;
+                E[5] += rep.smearing_energy;
+            }
         }
         // :207-241
         const double ddt = ddiff(p, f.rho.get(), f.orho.get());
@@ -398,9 +524,9 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         const double dd = std::max(ddt, dds);
         // :243-249: max KS residual over occupied orbitals (Davidson's final residuals)
         double ks = 0.0;
-        for (int j = 0; j < nal; ++j) ks = std::max(ks, f.da->residuals()[j]);
+        for (int j = 0; j < (smear ? std::min(sm_a.conv, k) : nal); ++j) ks = std::max(ks, f.da->residuals()[j]);
         if (!closed)
-            for (int j = 0; j < nbe; ++j) ks = std::max(ks, f.db->residuals()[j]);
+            for (int j = 0; j < (smear ? std::min(sm_b.conv, k) : nbe); ++j) ks = std::max(ks, f.db->residuals()[j]);
         // SelfConsistentFieldMolecule.cpp:297-400
         const double de = std::isfinite(prevE) ? std::fabs(E[5] - prevE) : std::numeric_limits<double>::infinity();
         rep.iterations = it;
@@ -417,17 +543,29 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             rep.first_de_iteration = it;
             rep.seconds_to_first_de = secs(t0, Clock::now());
         }
-        if (cfg_.verbose)
-            std::fprintf(stderr, "[scf] %3d %s E=%.12f dE=%.3e dRho=%.3e KS=%.3e mix=%.3f eps0=%.8f dav=%d/%d\n", it,
-                         p == DFT_F64 ? "f64" : "f32", E[5], de, dd, ks, imix, out.eig[0], sa.iterations,
+        if (cfg_.verbose) {
+            // HOMO and the lowest empty Ritz value of the alpha channel (approximate: only occupied states are converged)
+            const auto& ea = f.da->eigenvalues();
+            const double homo = ea[nal - 1], lumo = (int)ea.size() > nal ? ea[nal] : homo;
+            std::fprintf(stderr,
+                         "[scf] %3d %s E=%.12f dE=%.3e dRho=%.3e KS=%.3e mix=%.3f eps0=%.8f homo=%.5f lumo=%.5f dav=%d/%d\n",
+                         it, p == DFT_F64 ? "f64" : "f32", E[5], de, dd, ks, imix, out.eig[0], homo, lumo, sa.iterations,
                          closed ? 0 : sb.iterations);
+        }
         bool conv = cfg_.stop_rule == DFT_STOP_DE ? de < etol : (de < kEnergyTol && dd < kDensityTol && ks < kKsTol);
         if (refining) ++refine_steps;
         // The FP32 phase stops at its own noise floor (FP32 fields carry ~1e-4 Ha
         // of energy noise and ~1e-6 of relative density noise) and hands over to
         // the FP64 refinement, which then applies the configured rule.
-        if (mixed && !refining && f.prec == DFT_F32)
+        if (mixed && !refining && f.prec == DFT_F32) {
             conv = cfg_.stop_rule == DFT_STOP_DE ? de < std::max(etol, kF32EnergyFloor) : dd < kF32DensityFloor;
+            if (dd < 0.9 * best_dd) {
+                best_dd = dd;
+                stall = 0;
+            } else if (++stall >= 3 && best_dd < kF32StallDensity) {
+                conv = true;
+            }
+        }
         const bool hand_over = conv && mixed && !refining && cfg_.refine_iterations > 0;
         if (conv && !hand_over) {
             rep.converged = 1;

@@ -33,6 +33,13 @@ struct ScfOutputs {
     const dft_scf_fields* export_fields = nullptr;
 };
 
+// Fermi-Dirac occupations of one channel (each in [0, 1], summing to nelec);
+// returns the chemical potential.  Bisection on mu over a bracket that holds
+// every eigenvalue +- 50 kT, to rounding.
+double fermi_occupations(const double* eps, int n, double nelec, double kT, double* occ);
+// S = -sum [o ln o + (1 - o) ln(1 - o)] (0 ln 0 = 0)
+double fermi_entropy(const double* occ, int n);
+
 // Factory for the learned preconditioner (cnn.cpp); null when unavailable.
 std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g);
 // DFT_PRECOND_* -> preconditioner (throws invalid_argument for CNN without a model)

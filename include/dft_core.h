@@ -60,6 +60,11 @@ typedef struct {
     int mixer;             /* DFT_MIX_LINEAR (the reference's lagged linear mixing) or DFT_MIX_PULAY */
     int mix_depth;         /* DFT_MIX_PULAY: history length; 0: 6 */
     double mix_beta;       /* DFT_MIX_PULAY: residual weight; 0: 0.3 */
+    double smearing;       /* Fermi-Dirac width kT in Ha; 0: the reference's whole-orbital (aufbau) occupations.
+                              > 0: fractional occupations over the nstates block (needs nstates > occupied count),
+                              and the energy reported and tested by the stop rule is the free energy E - TS */
+    int ortho_gram;        /* mixed mode: Gram products of the FP32 orthonormalisation (block Gram-Schmidt, CholQR):
+                              0 or DFT_GRAM_F32: FP32 products; DFT_GRAM_SPLIT_BF16: as the Rayleigh-Ritz products */
 } dft_scf_config;
 
 typedef struct {
@@ -80,6 +85,8 @@ typedef struct {
     int64_t cnn_applied, cnn_fallbacks;
     int64_t block_passes;     /* Davidson block Gram-Schmidt passes (one or two per new block) */
     int64_t block_fallbacks;  /* blocks that needed the rank-revealing orthonormalisation */
+    double fermi_level[2];    /* smearing > 0: chemical potential of the alpha and beta channels */
+    double smearing_energy;   /* smearing > 0: -TS, included in energy[5] */
 } dft_scf_report;
 
 const char* dft_core_last_error(void);
@@ -117,6 +124,12 @@ int dft_scf_step(const dft_grid* g, int natoms, const double* nuclei, const dft_
  * precond: DFT_PRECOND_DIAG / CNN / KINETIC / KINETIC_CHEB / KINETIC_EXACT; cnn: the trained network
  * (dft_cnn_model) for DFT_PRECOND_CNN.  stats (optional, 4 values): H-applied columns,
  * restarts, learned corrections applied, diagonal fallbacks. */
+/* Fermi-Dirac occupations of one spin channel: occ[i] = 1 / (1 + exp((eps[i]
+ * - mu) / kT)) with mu such that sum occ = nelec (bisection to rounding),
+ * and the entropy S = -sum [occ ln occ + (1 - occ) ln(1 - occ)].  kT > 0,
+ * 0 <= nelec <= n.  No reference counterpart (whole orbitals only there). */
+int dft_fermi_occupations(const double* eps, int n, double nelec, double kT, double* occ, double* mu, double* entropy);
+
 int dft_davidson_run(const dft_grid* g, int prec, int gram_mode, const double* veff_host, int nstates, int nconv,
                      int max_subspace, double tol, int max_iter, int precond, const void* cnn, double* eig,
                      double* psi, double* resid, int* iterations, int64_t* stats);

@@ -86,6 +86,12 @@ int dft_hpsi(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const 
 int dft_kinetic_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
                        const int* electrons_host, double* t_host);
 
+/* The same sum with fractional weights (occ_host: nb doubles, electrons per
+ * orbital), for smeared occupations.  No reference counterpart: the
+ * reference occupies whole orbitals only. */
+int dft_kinetic_energy_occ(dft_ctx* ctx, const dft_grid* g, int prec, const void* psi, int64_t ld, int nb,
+                           const double* occ_host, double* t_host);
+
 /* Davidson diagonal preconditioner (EigenvalueSolverMolecule.cpp:483-555):
  * t[:,o] = r[:,o] / (eps_o - (sum 1/h^2 + veff)), or r where |denominator| < 1e-10
  * or not finite.  eps_dev: nb FP64 values on the device. */
@@ -168,6 +174,10 @@ int dft_density(dft_ctx* ctx, int prec, const void* psi, int64_t ld, int nb, con
 int dft_reduce(dft_ctx* ctx, int op, int prec, const void* a, const void* b, int64_t n, double* result_host);
 /* Elementwise: y = alpha*x + beta*y */
 int dft_axpby(dft_ctx* ctx, int prec, int64_t n, double alpha, const void* x, double beta, void* y);
+/* Elementwise y = max(y, lo) (a NaN becomes lo): keeps an extrapolated
+ * (Pulay-mixed) density non-negative.  No reference counterpart: the
+ * reference's linear mixing of non-negative densities needs none. */
+int dft_clamp_min(dft_ctx* ctx, int prec, int64_t n, double lo, void* y);
 /* out = a + b + c (c may be NULL) -- V_eff assembly (MolecularSCFIteration.cpp:92-115) */
 int dft_add3(dft_ctx* ctx, int prec, int64_t n, const void* a, const void* b, const void* c, void* out);
 /* NuclearPotential::calculate (NuclearPotential.cpp:6-76): nuclei_host =

@@ -82,3 +82,44 @@ def test_no_gpu_context_creation_fails_cleanly(libs):
         pytest.skip("a GPU is visible")
     ctx = C.c_void_p()
     assert libs[0].dft_ctx_create(0, C.byref(ctx)) == -3
+
+
+def test_fermi_occupations(libs):
+    """Host occupation solver of the smeared SCF (no GPU): the electron count,
+    the chemical potential against an independent root find, and cases the
+    mathematics fixes: a degenerate level shares its electrons equally, a
+    particle-hole symmetric spectrum at half filling puts mu at its centre,
+    a state at mu holds 1/2, a wide gap gives whole orbitals, and the entropy
+    of k equal occupations is k times the binary entropy."""
+    import numpy as np
+    from scipy.optimize import brentq
+    import dft_amd
+    rng = np.random.default_rng(3)
+    eps = np.sort(rng.uniform(-2.0, 0.5, 40))
+    for nel, kT in ((17.0, 0.01), (3.5, 0.2), (39.2, 0.05)):
+        occ, mu, S = dft_amd.fermi_occupations(eps, nel, kT)
+        assert abs(occ.sum() - nel) <= 1e-11
+        assert np.all(np.diff(occ) <= 1e-15) and np.all((occ >= 0) & (occ <= 1))
+        mu_ref = brentq(lambda m: np.sum(0.5 * (1.0 - np.tanh((eps - m) / (2 * kT)))) - nel, -10, 10, xtol=1e-15)
+        assert abs(mu - mu_ref) <= 1e-9
+    # degenerate triple holding 1 electron above a gap, below a gap
+    e3 = np.array([-5.0, -5.0, -1.0, -1.0, -1.0, 2.0, 3.0])
+    occ, mu, S = dft_amd.fermi_occupations(e3, 3.0, 0.01)
+    assert np.allclose(occ[:2], 1.0, atol=1e-12) and np.allclose(occ[2:5], 1.0 / 3.0, atol=1e-12)
+    assert occ[5] < 1e-100
+    h = -(1 / 3 * np.log(1 / 3) + 2 / 3 * np.log(2 / 3))
+    assert abs(S - 3 * h) <= 1e-12
+    # symmetric spectrum at half filling: mu at 0, the state at mu holds 1/2
+    es = np.array([-1.0, -0.3, 0.0, 0.3, 1.0])
+    occ, mu, S = dft_amd.fermi_occupations(es, 2.5, 0.1)
+    assert abs(mu) <= 1e-12 and abs(occ[2] - 0.5) <= 1e-12 and abs(occ[0] + occ[4] - 1.0) <= 1e-12
+    # a wide gap: whole orbitals, no entropy
+    occ, mu, S = dft_amd.fermi_occupations(np.array([-3.0, -2.0, 1.0, 2.0]), 2.0, 0.01)
+    assert np.array_equal(occ[:2], [1.0, 1.0]) and occ[2] < 1e-60 and S < 1e-60  # mid-gap mu: e^-150
+    # full and empty channels; bad arguments raise
+    assert np.all(dft_amd.fermi_occupations(es, 5.0, 0.1)[0] == 1.0)
+    assert np.all(dft_amd.fermi_occupations(es, 0.0, 0.1)[0] == 0.0)
+    with pytest.raises(dft_amd.DftError):
+        dft_amd.fermi_occupations(es, 6.0, 0.1)
+    with pytest.raises(dft_amd.DftError):
+        dft_amd.fermi_occupations(es, 1.0, 0.0)

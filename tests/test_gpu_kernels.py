@@ -335,6 +335,35 @@ def test_xc_f32_vs_oracle(dev, oracle, fid):
     assert abs(e - ref_e) <= 1e-5 * abs(ref_e)
 
 
+def test_xc_f32_extreme_gradients(dev, oracle):
+    """FP32 PBE96 on a density whose values jump by up to 11 decades between
+    neighbours (an extrapolated density's tails): finite everywhere and, where
+    the gradient is extreme, equal to the oracle's FP64 values to FP32
+    rounding of the inputs (those points are evaluated in FP64)."""
+    from oracle.oracle import Grid
+    og = Grid.centered(12, 0.3)
+    rng = np.random.default_rng(11)
+    na = (10.0 ** rng.uniform(-8.0, 3.0, og.size)).astype(np.float32).astype(np.float64)
+    nb = na.copy()
+    ref_a = oracle.xc_potential(og, 2, na, nb, 0)
+    g = dgrid(og)
+    a32, b32 = dev.array(na, dtype=np.float32), dev.array(nb, dtype=np.float32)
+    va, vb = dev.array(shape=(og.size,), dtype=np.float32), dev.array(shape=(og.size,), dtype=np.float32)
+    dev.xc(g, 2, a32, b32, va, vb)
+    got = va.download().astype(np.float64)
+    assert np.all(np.isfinite(got))
+    print("xc f32 extreme", rel(got, ref_a))
+    assert rel(got, ref_a) <= 1e-3
+
+
+def test_clamp_min(dev):
+    x = np.array([1.0, -2.0, 0.0, np.nan, 3.5, -1e-30])
+    for dt in (np.float64, np.float32):
+        d = dev.array(x, dtype=dt)
+        dev.clamp_min(d, 0.0)
+        assert np.array_equal(d.download(), np.array([1.0, 0.0, 0.0, 0.0, 3.5, 0.0], dtype=dt))
+
+
 def test_density_and_energies(dev, goldens, oracle):
     og = grid_from(goldens["misc_grid"])
     g = dgrid(og)

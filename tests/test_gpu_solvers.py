@@ -253,3 +253,39 @@ def test_davidson_free_box_full_size(prec):
     err = np.max(np.abs(r["eigenvalues"][:nc] - exact))
     print(prec, "iterations", r["iterations"], "seconds", round(time.perf_counter() - t, 2), "max error", err)
     assert err <= (1e-10 if prec == "f64" else 2e-6)
+
+
+def test_scf_smearing():
+    """Fermi-Dirac smearing (no reference counterpart).  A wide-gap molecule
+    (H2) with small kT gives the whole-orbital SCF's energy, since every
+    occupation is 0 or 1 to rounding.  A spin-unpolarised carbon atom (a
+    partly filled, degenerate 2p level: no whole-orbital ground state)
+    converges with fractional occupations: -TS lies between 0 and the bound
+    -kT 2 k ln 2 of k half-filled states per spin, and the mixed-precision run
+    reaches the FP64 free energy.  The occupation solver itself is pinned on
+    the CPU (degenerate levels, symmetric spectra)."""
+    import dft_amd
+    g = dft_amd.Grid.centered(16, 0.42)
+    nuc = [[1, -0.71, 0.02, 0.0], [1, 0.69, 0.0, 0.03]]
+    common = dict(functional=dft_amd.PZ81, poisson_method=dft_amd.POISSON_DIRECT, stop_rule=dft_amd.STOP_DE,
+                  energy_tol=1e-9, init_guess=dft_amd.INIT_HYDROGENIC, davidson_tol=1e-9, max_iterations=200,
+                  mixer=dft_amd.MIX_PULAY)
+    tight = dict(common, stop_rule=dft_amd.STOP_REFERENCE)  # both runs at the reference's tight stop rule
+    r0 = dft_amd.scf_run(g, nuc, nstates=4, **tight)
+    r1 = dft_amd.scf_run(g, nuc, nstates=4, smearing=1e-3, **tight)
+    print("h2", r0["total"], r1["total"], r1["smearing_energy"], r1["fermi_level"])
+    assert r0["converged"] and r1["converged"]
+    assert abs(r1["total"] - r0["total"]) <= 1e-9 and abs(r1["smearing_energy"]) <= 1e-100
+    assert r1["fermi_level"][0] > r0["eigenvalues"][0]
+    gc = dft_amd.Grid.centered(40, 0.3)
+    kT = 0.01
+    c = [[6, 0.0, 0.0, 0.0]]
+    ce = dict(common, energy_tol=1e-8, nstates=8, smearing=kT)
+    f64 = dft_amd.scf_run(gc, c, **ce)
+    print("carbon", f64["total"], f64["smearing_energy"], f64["iterations"], f64["fermi_level"])
+    assert f64["converged"]
+    assert -kT * 2 * 8 * np.log(2.0) < f64["smearing_energy"] < 0.0
+    mx = dft_amd.scf_run(gc, c, precision=dft_amd.SCF_MIXED, refine_iterations=100,
+                         precond=dft_amd.PRECOND_KINETIC_EXACT, **ce)
+    print("carbon mixed", mx["total"], mx["iterations"], mx["refine_iterations"])
+    assert mx["converged"] and abs(mx["total"] - f64["total"]) <= 1e-6
