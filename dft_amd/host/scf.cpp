@@ -303,10 +303,9 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
 
     std::vector<double> occ_a(nal, 1.0 / dv), occ_b(nbe, 1.0 / dv);  // unit-norm vectors -> psi^2 = u^2/dV
     std::vector<int> el_a(nal, 1), el_b(nbe, 1);
-    // smearing: per-channel occupations of the k states (0..1), the states the
-    // density and the energy take (up to the last with occ > 1e-14) and the
-    // states the next Davidson must converge (occ > 1e-6, leaving the top 4 of
-    // the block as buffer where their occupations are below 1e-3)
+    // smearing: per-channel occupations of the block's states (0..1), the states
+    // the density and the energy take (up to the last with occ > 1e-14) and the
+    // states the next Davidson must converge (occ > 1e-6)
     struct Occ {
         std::vector<double> o, w;  // occupations; o / dV for the density kernel
         int used = 0, conv = 0;
@@ -315,20 +314,20 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     Occ sm_a, sm_b;
     sm_a.conv = nal;
     sm_b.conv = nbe;
+    // The top 4 states of the block (when it has 8 or more beyond the
+    // electrons) are Davidson's buffer: never occupied, never required to
+    // converge, so the occupied set is always made of converged states.
+    const int kocc = k - 4 >= nocc_max + 4 ? k - 4 : k;
     auto occupy = [&](Occ& q, const std::vector<double>& eps, int nel) {
         q.o.assign(k, 0.0);
-        q.mu = fermi_occupations(eps.data(), k, (double)nel, kT, q.o.data());
-        q.entropy = fermi_entropy(q.o.data(), k);
-        int must = 0;
+        q.mu = fermi_occupations(eps.data(), kocc, (double)nel, kT, q.o.data());
+        q.entropy = fermi_entropy(q.o.data(), kocc);
         q.used = q.conv = 0;
-        for (int i = 0; i < k; ++i) {
+        for (int i = 0; i < kocc; ++i) {
             if (q.o[i] > 1e-14) q.used = i + 1;
             if (q.o[i] > 1e-6) q.conv = i + 1;
-            if (q.o[i] > 1e-3) must = i + 1;
         }
-        // keep a few unconverged states at the top of the block as Davidson's
-        // buffer, unless they hold more than 1e-3 electrons
-        q.conv = std::max({std::min(q.conv, k - 4), must, nel});
+        q.conv = std::max(q.conv, nel);
         q.used = std::max(q.used, nel);
         q.w.resize(q.used);
         for (int i = 0; i < q.used; ++i) q.w[i] = q.o[i] / dv;

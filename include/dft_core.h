@@ -61,7 +61,8 @@ typedef struct {
     int mix_depth;         /* DFT_MIX_PULAY: history length; 0: 6 */
     double mix_beta;       /* DFT_MIX_PULAY: residual weight; 0: 0.3 */
     double smearing;       /* Fermi-Dirac width kT in Ha; 0: the reference's whole-orbital (aufbau) occupations.
-                              > 0: fractional occupations over the nstates block (needs nstates > occupied count),
+                              > 0: fractional occupations over the nstates block less its top 4 states (kept as
+                              the eigensolver's unoccupied buffer when nstates exceeds the occupied count by 8 or more),
                               and the energy reported and tested by the stop rule is the free energy E - TS */
     int ortho_gram;        /* mixed mode: Gram products of the FP32 orthonormalisation (block Gram-Schmidt, CholQR):
                               0 or DFT_GRAM_F32: FP32 products; DFT_GRAM_SPLIT_BF16: as the Rayleigh-Ritz products */
