@@ -194,10 +194,14 @@ bool update_mfma_fits(int m, int k) {
 }
 
 // largest inner-dimension chunk (a multiple of 4) whose C^T fits the LDS budget,
-// balanced over ceil(m / max) chunks; 0 when k itself is too wide
+// balanced over ceil(m / max) chunks; 0 when chunking does not pay.  Chunks
+// re-read and re-write Y, and past k = 48 the kernel's MFMA rate (~50 TF/s
+// FP32, ~20 TF/s FP64 at 288 x 96) falls behind rocBLAS' tuned GEMM on these
+// compute-bound shapes (measured on MI355X at N = 256^3: 19.7 / 54.9 ms chunked
+// against 17.7 / 17.6 ms for the library, FP32 / FP64).
 template <typename T>
 int update_chunk(int m, int k) {
-    if (k < 1 || k > 96) return 0;
+    if (k < 1 || k > 48) return 0;
     const int njt = (k + 15) / 16;
     const int mx = (int)(kUpdLdsBytes / (njt * 16 * (int64_t)sizeof(T))) / 4 * 4;
     if (mx < 4) return 0;
@@ -698,8 +702,9 @@ extern "C" {
 
 
 // Block update Y = alpha A C + beta Y (a tall-skinny GEMM, N x m times m x k).
-// k <= 96: the MFMA streaming kernel, in one pass when C^T fits the LDS budget,
-// else over chunks of the inner dimension accumulated into Y (Y apart from A);
+// k <= 96: the MFMA streaming kernel in one pass when C^T fits the LDS budget;
+// k <= 48 beyond it: the same kernel over chunks of the inner dimension
+// accumulated into Y (Y apart from A);
 // FP32 blocks with k <= 64 and m <= 512 that alias A: the VALU kernel;
 // otherwise rocBLAS' 64-bit-index GEMM, through a staging copy when Y overlaps
 // A.  FP32 blocks use an FP32 copy of the small coefficient matrix.
@@ -853,8 +858,8 @@ int dft_syev_lowest(dft_ctx* c, int m, int k, const double* a, int64_t lda, doub
 // R[:, j] = W Q[:, j] - theta[j] V Q[:, j] and rn2[j] = ||R[:, j]||^2 for
 // j < k.  One MFMA pass over [W | V] with the 2m x k coefficients
 // [Q ; -Q diag(theta)] when they fit the LDS budget (norms from the same
-// pass), else over chunks of those 2m rows (norms from the last chunk);
-// k > 96: R = W Q, then R += V (-Q theta), then the norms.
+// pass), else (k <= 48) over chunks of those 2m rows (norms from the last
+// chunk); otherwise R = W Q, then R += V (-Q theta), then the norms.
 int dft_ritz_residual(dft_ctx* c, int prec, int64_t n, int m, int k, const void* v, int64_t ldv, const void* w,
                       int64_t ldw, const double* q, int64_t ldq, const double* theta, void* r, int64_t ldr, double* rn2) {
     if (!c) return DFT_EINVAL;

@@ -414,11 +414,13 @@ def test_gram_modes(dev, n, ma, mb):
 
 
 @pytest.mark.parametrize("n,m,k", [(1000, 8, 8), (50001, 37, 70), (200000, 192, 96), (70001, 29, 13), (70000, 29, 13),
-                                   (4100, 90, 96), (64004, 61, 33), (64002, 5, 1), (30000, 288, 96), (30001, 131, 70)])
+                                   (4100, 90, 96), (64004, 61, 33), (64002, 5, 1), (30000, 288, 96), (30001, 131, 70),
+                                   (40000, 300, 40), (40001, 270, 17)])
 def test_block_update(dev, n, m, k):
-    """Y = alpha A C + beta Y against numpy, FP64 and FP32: one MFMA pass, and
-    C beyond the LDS budget (m = 192, 288, 131 at these k) as chunks of the
-    inner dimension accumulated into Y."""
+    """Y = alpha A C + beta Y against numpy, FP64 and FP32: one MFMA pass; C
+    beyond the LDS budget with k <= 48 (m = 300, 270) as chunks of the inner
+    dimension accumulated into Y; wider blocks (m = 192, 288, 131 at k = 96,
+    70) on the library GEMM."""
     rng = np.random.default_rng(m)
     A = rng.standard_normal((m, n))
     Cm = rng.standard_normal((k, m))  # column-major m x k
@@ -437,12 +439,13 @@ def test_block_update(dev, n, m, k):
 
 
 @pytest.mark.parametrize("n,m,k", [(64000, 87, 29), (70001, 40, 29), (4096, 61, 96), (30002, 150, 40), (5000, 3, 1),
-                                   (20000, 288, 96), (20001, 97, 70)])
+                                   (20000, 288, 96), (20001, 97, 70), (20003, 201, 33)])
 def test_ritz_residual(dev, n, m, k):
     """R = W Q - V Q diag(theta) and its column norms^2, against numpy: one
-    fused pass (vector and scalar grid sizes) and, with the 2m x k
-    coefficients beyond the LDS budget (m = 150, 288, 97), chunks of the
-    [W | V] rows, one of them straddling W and V, the norms from the last."""
+    fused pass (vector and scalar grid sizes); with the 2m x k coefficients
+    beyond the LDS budget and k <= 48 (m = 150, 201), chunks of the [W | V]
+    rows, one of them straddling W and V, the norms from the last; wider
+    blocks (m = 288, 97 at k = 96, 70) as two library passes and the norms."""
     rng = np.random.default_rng(n + m)
     V = rng.standard_normal((m, n))
     W = rng.standard_normal((m, n))
