@@ -86,14 +86,23 @@ double fermi_entropy(const double* occ, int n) {
 // (SelfConsistentFieldMolecule.cpp:447-484) stays the default.
 class PulayMixer {
    public:
-    PulayMixer(dft_ctx* c, int prec, int64_t n, bool closed, int depth, double beta)
-        : c_(c), prec_(prec), n_(n), closed_(closed), depth_(depth), beta_(beta) {
+    // q0 > 0: Kerker screening of the update, rho_next = sum c_i rho_i + beta K(sum c_i R_i)
+    // with K = -Lap (-Lap + q0^2)^-1 = 1 - q0^2 (-Lap + q0^2)^-1 (G. P. Kerker, Phys. Rev. B
+    // 23, 3082 (1981)), the inverse by the sine-transform kinetic solve: it damps the
+    // long-wavelength components of the residual, where a metallic system's density
+    // response is largest.  The DIIS coefficients still minimise the unscreened residual.
+    PulayMixer(dft_ctx* c, const dft_grid& g, int prec, int64_t n, bool closed, int depth, double beta, double q0)
+        : c_(c), g_(g), prec_(prec), n_(n), closed_(closed), depth_(depth), beta_(beta), q0_(q0) {
         L_ = closed ? n : 2 * n;
         hist_ = Buf(c, (size_t)2 * depth * L_ * esize(prec));
         hist_.zero();  // unused columns enter the update with coefficient 0
-        coef_ = Buf(c, sizeof(double) * (size_t)2 * depth);
+        coef_ = Buf(c, sizeof(double) * (size_t)4 * depth);
         dots_ = Buf(c, sizeof(double) * (size_t)depth);
         A_.assign((size_t)depth * depth, 0.0);
+        if (q0_ > 0.0) {
+            rbar_ = Buf(c, (size_t)L_ * esize(prec));
+            kr_ = Buf(c, (size_t)L_ * esize(prec));
+        }
     }
     void mix(void* ra, void* rb, const void* ora, const void* orb) {
         const size_t es = esize(prec_), cb = (size_t)L_ * es;
@@ -134,6 +143,39 @@ class PulayMixer {
         if (!solve(M, b, m, cvec)) {  // singular history: restart it from the newest entry
             cvec.assign(count_ + 1, 0.0);
             cvec[head_] = 1.0;
+        }
+        if (q0_ > 0.0) {
+            // [rho_bar | R_bar] = history * [c 0 ; 0 c], then rho_next = rho_bar + beta K(R_bar)
+            std::vector<double> C2((size_t)2 * depth_ * 2, 0.0);
+            for (int i = 0; i < count_; ++i) {
+                C2[i] = cvec[i];
+                C2[(size_t)2 * depth_ + depth_ + i] = cvec[i];
+            }
+            check(c_, dft_upload(c_, coef_k_(), C2.data(), sizeof(double) * C2.size()), "dft_upload");
+            const int ch = closed_ ? 1 : 2;
+            for (int s = 0; s < ch; ++s) {
+                char* dst = static_cast<char*>(s == 0 ? ra : rb);
+                check(c_, dft_block_update(c_, prec_, n_, 2 * depth_, 1, 1.0, h + s * fb, L_, coef_k_(), 2 * depth_, 0.0,
+                                           dst, n_),
+                      "dft_block_update");
+                check(c_, dft_block_update(c_, prec_, n_, 2 * depth_, 1, 1.0, h + s * fb, L_, coef_k_() + 2 * depth_,
+                                           2 * depth_, 0.0, static_cast<char*>(rbar_.get()) + s * fb, n_),
+                      "dft_block_update");
+            }
+            // (-1/2 Lap + q0^2/2)^-1 R = 2 (-Lap + q0^2)^-1 R, so K R = R - (q0^2 / 2) kin(R)
+            check(c_, dft_precond_kinetic_exact(c_, &g_, prec_, 0.5 * q0_ * q0_, rbar_.get(), n_, ch, kr_.get(), n_),
+                  "dft_precond_kinetic_exact");
+            for (int s = 0; s < ch; ++s) {
+                void* dst = s == 0 ? ra : rb;
+                check(c_, dft_axpby(c_, prec_, n_, beta_, static_cast<char*>(rbar_.get()) + s * fb, 1.0, dst), "dft_axpby");
+                check(c_, dft_axpby(c_, prec_, n_, -0.5 * beta_ * q0_ * q0_, static_cast<char*>(kr_.get()) + s * fb, 1.0,
+                                    dst),
+                      "dft_axpby");
+                check(c_, dft_clamp_min(c_, prec_, n_, 0.0, dst), "dft_clamp_min");
+            }
+            if (closed_) check(c_, dft_copy(c_, rb, ra, fb), "dft_copy");
+            head_ = (head_ + 1) % depth_;
+            return;
         }
         // rho_next = sum c_i (rho_i + beta R_i), one pass over the history block
         std::vector<double> C((size_t)2 * depth_, 0.0);
@@ -184,13 +226,16 @@ class PulayMixer {
         return true;
     }
 
+    double* coef_k_() { return coef_.as<double>(); }
+
     dft_ctx* c_;
+    dft_grid g_;
     int prec_;
     int64_t n_, L_ = 0;
     bool closed_;
     int depth_;
-    double beta_;
-    Buf hist_, coef_, dots_;
+    double beta_, q0_;
+    Buf hist_, coef_, dots_, rbar_, kr_;
     std::vector<double> A_;
     int count_ = 0, head_ = 0;
 };
@@ -439,6 +484,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     double prevE = std::numeric_limits<double>::infinity(), prevDD = prevE, mixing = kMixing0;
     if (cfg_.mixer != DFT_MIX_LINEAR && cfg_.mixer != DFT_MIX_PULAY) throw std::invalid_argument("unknown density mixer");
     if (cfg_.mix_depth < 0 || cfg_.mix_depth > 32) throw std::invalid_argument("mix_depth must be in 0..32");
+    if (!(cfg_.mix_kerker >= 0.0) || !std::isfinite(cfg_.mix_kerker)) throw std::invalid_argument("mix_kerker must be >= 0");
     std::unique_ptr<PulayMixer> pulay;  // rebuilt for each precision phase
     int pulay_prec = -1;
     bool refining = false;
@@ -572,8 +618,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         }
         if (cfg_.mixer == DFT_MIX_PULAY) {
             if (!pulay || pulay_prec != p) {
-                pulay.reset(new PulayMixer(c_, p, n, closed, cfg_.mix_depth > 0 ? cfg_.mix_depth : 6,
-                                           cfg_.mix_beta > 0.0 ? cfg_.mix_beta : 0.3));
+                pulay.reset(new PulayMixer(c_, g_, p, n, closed, cfg_.mix_depth > 0 ? cfg_.mix_depth : 6,
+                                           cfg_.mix_beta > 0.0 ? cfg_.mix_beta : 0.3, cfg_.mix_kerker));
                 pulay_prec = p;
             }
             pulay->mix(f.ra.get(), f.rb.get(), f.ora.get(), f.orb.get());

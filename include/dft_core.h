@@ -66,6 +66,7 @@ typedef struct {
                               and the energy reported and tested by the stop rule is the free energy E - TS */
     int ortho_gram;        /* mixed mode: Gram products of the FP32 orthonormalisation (block Gram-Schmidt, CholQR):
                               0 or DFT_GRAM_F32: FP32 products; DFT_GRAM_SPLIT_BF16: as the Rayleigh-Ritz products */
+    double mix_kerker;     /* DFT_MIX_PULAY: Kerker screening wave number q0 (1/bohr) of the update; 0: none */
 } dft_scf_config;
 
 typedef struct {

@@ -230,6 +230,12 @@ def test_scf_pulay_mixing_matches_linear(oracle):
     pul = dft_amd.scf_run(dgrid(og), h3, multiplicity=2, **common)
     print("doublet", lin["total"], pul["total"], lin["iterations"], pul["iterations"])
     assert lin["converged"] and pul["converged"] and abs(lin["total"] - pul["total"]) <= 1e-7
+    # Kerker-screened update (q0 = 1/bohr): the same fixed points, closed and open shell
+    kc = dft_amd.scf_run(dgrid(og), nuc, mix_kerker=1.0, **common)
+    ko = dft_amd.scf_run(dgrid(og), h3, multiplicity=2, mix_kerker=1.0, **common)
+    print("kerker", kc["total"], kc["iterations"], ko["total"], ko["iterations"])
+    assert kc["converged"] and abs(kc["total"] - ref["total"]) <= 1e-7
+    assert ko["converged"] and abs(ko["total"] - lin["total"]) <= 1e-7
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32_split"])
