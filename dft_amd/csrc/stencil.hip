@@ -348,20 +348,23 @@ int kinetic_cheb_t(dft_ctx* c, const dft_grid* g, double sigma, int sweeps, cons
 }
 
 // Kinetic energy with the reference's face rule (TotalEnergyMolecule.cpp:80-186):
-// only existing neighbours contribute (psi_nb - psi)/h^2.
+// only existing neighbours contribute (psi_nb - psi)/h^2.  One orbital per
+// grid row (blockIdx.y), so every workgroup streams one column with
+// coalesced centre and neighbour loads; per-workgroup FP64 partials.
 template <typename T>
 __global__ __launch_bounds__(256) void k_kinetic(int nx, int ny, int nz, double dx2, double dy2, double dz2,
-                                                 const T* __restrict__ psi, int64_t ld, int nb,
+                                                 const T* __restrict__ psi, int64_t ld,
                                                  const double* __restrict__ el, double* __restrict__ partials) {
     __shared__ double lds[4];
-    const int64_t n = (int64_t)nx * ny * nz, plane = (int64_t)nx * ny;
+    const int o = blockIdx.y;
+    const double w = el[o];
     double s = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * 256;
-    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += stride) {
-        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / plane);
-        for (int o = 0; o < nb; ++o) {
-            if (el[o] <= 0.0) continue;
-            const T* q = psi + (int64_t)o * ld;
+    if (w > 0.0) {
+        const int64_t n = (int64_t)nx * ny * nz, plane = (int64_t)nx * ny;
+        const T* q = psi + (int64_t)o * ld;
+        const int64_t stride = (int64_t)gridDim.x * 256;
+        for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += stride) {
+            const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / plane);
             const double c = (double)q[p];
             double lap = 0.0;
             if (i > 0) lap += ((double)q[p - 1] - c) / dx2;
@@ -370,11 +373,12 @@ __global__ __launch_bounds__(256) void k_kinetic(int nx, int ny, int nz, double 
             if (j + 1 < ny) lap += ((double)q[p + nx] - c) / dy2;
             if (k > 0) lap += ((double)q[p - plane] - c) / dz2;
             if (k + 1 < nz) lap += ((double)q[p + plane] - c) / dz2;
-            s += el[o] * (c * (-0.5 * lap));
+            s += c * (-0.5 * lap);
         }
+        s *= w;
     }
     s = block_sum(s, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    if (threadIdx.x == 0) partials[(int64_t)o * gridDim.x + blockIdx.x] = s;
 }
 
 template <typename T>
@@ -416,19 +420,23 @@ int dft_kinetic_energy_occ(dft_ctx* c, const dft_grid* g, int prec, const void* 
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, ld >= n, "dft_kinetic_energy: ld < grid size");
     if (nb == 0) { *t_host = 0.0; return DFT_OK; }
-    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4 * (int64_t)c->num_cus);
-    if (int rc = ensure_partials(c, (size_t)blocks + 1 + (size_t)nb)) return rc;
-    double* el = c->partials + blocks + 1;
+    DFT_REQUIRE(c, nb <= 65535, "dft_kinetic_energy: more than 65535 orbitals");
+    // ~8 workgroups per CU over all orbitals, at least one per orbital
+    const int bx = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * (int64_t)c->num_cus / nb));
+    const size_t parts = (size_t)bx * nb;
+    if (int rc = ensure_partials(c, parts + 1 + (size_t)nb)) return rc;
+    double* el = c->partials + parts + 1;
     // the copy is synchronised below (finish_sum) before the host array can go away
     DFT_HIP(c, hipMemcpyAsync(el, occ_host, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
     const double dx2 = g->dx * g->dx, dy2 = g->dy * g->dy, dz2 = g->dz * g->dz;
+    const dim3 grid((unsigned)bx, (unsigned)nb);
     if (prec == DFT_F64)
-        k_kinetic<double><<<blocks, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const double*)psi, ld, nb, el, c->partials);
+        k_kinetic<double><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const double*)psi, ld, el, c->partials);
     else
-        k_kinetic<float><<<blocks, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const float*)psi, ld, nb, el, c->partials);
+        k_kinetic<float><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, dx2, dy2, dz2, (const float*)psi, ld, el, c->partials);
     DFT_CHECK_LAUNCH(c);
     double s = 0.0;
-    if (int rc = finish_sum(c, blocks, &s)) return rc;
+    if (int rc = finish_sum(c, (int64_t)parts, &s)) return rc;
     *t_host = s * (g->dx * g->dy * g->dz);
     return DFT_OK;
 }
