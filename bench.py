@@ -424,7 +424,7 @@ def main():
                    "stop": "|dE| < 1e-5 Ha",
                    "preconditioner": {"diag": "diagonal (the reference's)", "kinetic": "kinetic smoothing",
                                       "kinetic-cheb": "kinetic smoothing, 8 Chebyshev-combined steps, sigma 1.5",
-                                      "kinetic-exact": "(-1/2 Lap + 2)^-1 by BF16 sine transforms",
+                                      "kinetic-exact": "(-1/2 Lap + s_j)^-1 by BF16 sine transforms, s_j = max(0.5, 0.35 |theta_j|)",
                                       "cnn": f"learned ({a.cnn_model}) with diagonal fallback"}[a.precond],
                    "fp64_preconditioner": "diagonal"},
         "fp64_seconds": t64, "speedup_vs_fp64": t64 / per if per > 0 else None,

@@ -318,6 +318,12 @@ class Device:
                                                   C.c_int64(g.size), _vp(tmp.ptr)))
         tmp.free()
 
+    def precond_kinetic_exact_shifted(self, g, r, t, nb, theta, sigma_min=0.5, scale=0.35):
+        """t[:, j] = (-1/2 Lap + max(sigma_min, -scale theta[j]))^-1 r[:, j] (theta: device FP64)."""
+        self._chk(self.L.dft_precond_kinetic_exact_shifted(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma_min),
+                                                           C.c_double(scale), _vp(theta.ptr), _vp(r.ptr),
+                                                           C.c_int64(g.size), C.c_int(nb), _vp(t.ptr), C.c_int64(g.size)))
+
     def precond_kinetic_exact(self, g, r, t, nb, sigma=1.5):
         self._chk(self.L.dft_precond_kinetic_exact(self.ctx, C.byref(g), C.c_int(r.prec), C.c_double(sigma), _vp(r.ptr),
                                                    C.c_int64(g.size), C.c_int(nb), _vp(t.ptr), C.c_int64(g.size)))
@@ -434,7 +440,8 @@ class ScfConfig(C.Structure):
                 ("davidson_max_iter", C.c_int), ("refine_iterations", C.c_int), ("precond", C.c_int),
                 ("cnn", C.c_void_p), ("verbose", C.c_int), ("precond_sweeps", C.c_int),
                 ("precond_shift", C.c_double), ("mixer", C.c_int), ("mix_depth", C.c_int), ("mix_beta", C.c_double),
-                ("smearing", C.c_double), ("ortho_gram", C.c_int), ("mix_kerker", C.c_double)]
+                ("smearing", C.c_double), ("ortho_gram", C.c_int), ("mix_kerker", C.c_double),
+                ("precond_shift_scale", C.c_double)]
 
 
 class ScfReport(C.Structure):
@@ -459,7 +466,8 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
             stop_rule=STOP_REFERENCE, energy_tol=1e-5,
             init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
             precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False, precond_sweeps=0, precond_shift=0.0,
-            mixer=MIX_LINEAR, mix_depth=0, mix_beta=0.0, smearing=0.0, ortho_gram=0, mix_kerker=0.0):
+            mixer=MIX_LINEAR, mix_depth=0, mix_beta=0.0, smearing=0.0, ortho_gram=0, mix_kerker=0.0,
+            precond_shift_scale=0.0):
     """Run the device-resident SCF; returns a dict of results.  smearing: Fermi-Dirac kT in Ha (0: whole
     orbitals, the reference's rule); with smearing the total is the free energy E - TS."""
     L = core_lib()
@@ -467,7 +475,8 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
                     poisson_fixed, poisson_batch, poisson_method, poisson_tol, stop_rule, energy_tol, init_guess, davidson_tol,
                     davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0,
-                    precond_sweeps, precond_shift, mixer, mix_depth, mix_beta, smearing, ortho_gram, mix_kerker)
+                    precond_sweeps, precond_shift, mixer, mix_depth, mix_beta, smearing, ortho_gram, mix_kerker,
+                    precond_shift_scale)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
     eig = np.zeros(ne + 2)

@@ -38,16 +38,19 @@ __global__ __launch_bounds__(256) void k_cols_to_bf16(int64_t n, const T* __rest
     if (p < n) a[(int64_t)o * n + p] = (__bf16)(float)r[(int64_t)o * ldr + p];
 }
 
-// B[i + nx (j + ny (k + nz o))] /= l_x(i) + l_y(j) + l_z(k) + sigma: one
+// B[i + nx (j + ny (k + nz o))] /= l_x(i) + l_y(j) + l_z(k) + sigma_o: one
 // workgroup per (z plane k, column o), the x and y eigenvalues in LDS, so the
-// pass is a plain stream of the block
+// pass is a plain stream of the block.  sigma_o = sigma, or with per-column
+// shifts max(sigma, -scale theta_o).
 __global__ __launch_bounds__(256) void k_kin_scale(int nx, int ny, int nz, float ix2, float iy2, float iz2,
-                                                   float sigma, __bf16* __restrict__ B) {
+                                                   float sigma, const double* __restrict__ theta, float scale,
+                                                   __bf16* __restrict__ B) {
     __shared__ float lx[1024], ly[1024];
     const int k = blockIdx.x, o = blockIdx.y;
     for (int i = threadIdx.x; i < nx; i += 256) lx[i] = (1.0f - cospif((float)(i + 1) / (float)(nx + 1))) * ix2;
     for (int j = threadIdx.x; j < ny; j += 256) ly[j] = (1.0f - cospif((float)(j + 1) / (float)(ny + 1))) * iy2;
-    const float lz = (1.0f - cospif((float)(k + 1) / (float)(nz + 1))) * iz2 + sigma;
+    const float so = theta ? fmaxf(sigma, -scale * (float)theta[o]) : sigma;
+    const float lz = (1.0f - cospif((float)(k + 1) / (float)(nz + 1))) * iz2 + so;
     __syncthreads();
     __bf16* plane = B + ((int64_t)o * nz + k) * ((int64_t)nx * ny);
     const int np = nx * ny;
@@ -92,7 +95,8 @@ int dst3_bf16(dft_ctx* c, rocblas_handle h, int nx, int ny, int nz, int nb, cons
 }
 
 template <typename T>
-int kinetic_exact_t(dft_ctx* c, const dft_grid* g, double sigma, const T* r, int64_t ldr, int nb, T* t, int64_t ldt) {
+int kinetic_exact_t(dft_ctx* c, const dft_grid* g, double sigma, const double* theta, double scale, const T* r,
+                    int64_t ldr, int nb, T* t, int64_t ldt) {
     const int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz;
     const int64_t n = (int64_t)nx * ny * nz;
     const bool f64 = sizeof(T) == 8;
@@ -137,7 +141,7 @@ int kinetic_exact_t(dft_ctx* c, const dft_grid* g, double sigma, const T* r, int
     if (int rc = dst3_bf16(c, h, nx, ny, nz, nb, Sx, Sy, Sz, A, B, B, rocblas_datatype_bf16_r, n)) return rc;
     k_kin_scale<<<dim3((unsigned)nz, (unsigned)nb), 256, 0, c->stream>>>(
         nx, ny, nz, (float)(1.0 / (g->dx * g->dx)), (float)(1.0 / (g->dy * g->dy)), (float)(1.0 / (g->dz * g->dz)),
-        (float)sigma, B);
+        (float)sigma, theta, (float)scale, B);
     DFT_CHECK_LAUNCH(c);
     if (!f64) return dst3_bf16(c, h, nx, ny, nz, nb, Sx, Sy, Sz, B, A, t, rocblas_datatype_f32_r, ldt);
     if (int rc = dst3_bf16(c, h, nx, ny, nz, nb, Sx, Sy, Sz, B, A, F, rocblas_datatype_f32_r, n)) return rc;
@@ -148,11 +152,10 @@ int kinetic_exact_t(dft_ctx* c, const dft_grid* g, double sigma, const T* r, int
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int dft_precond_kinetic_exact(dft_ctx* c, const dft_grid* g, int prec, double sigma, const void* r, int64_t ldr, int nb,
-                              void* t, int64_t ldt) {
-    if (!c) return DFT_EINVAL;
+int kinetic_exact(dft_ctx* c, const dft_grid* g, int prec, double sigma, const double* theta, double scale,
+                         const void* r, int64_t ldr, int nb, void* t, int64_t ldt) {
     DFT_REQUIRE(c, valid_grid(g) && r && t && nb >= 0 && nb <= 65535 && sigma > 0.0 && ldr >= grid_points(g) &&
                        ldt >= grid_points(g),
                 "dft_precond_kinetic_exact: bad arguments");
@@ -161,9 +164,28 @@ int dft_precond_kinetic_exact(dft_ctx* c, const dft_grid* g, int prec, double si
                        g->ny <= 1024 && g->nz <= 65535,
                 "dft_precond_kinetic_exact: grid or block too large for the transforms");
     if (!nb) return DFT_OK;
-    if (prec == DFT_F64) return kinetic_exact_t<double>(c, g, sigma, (const double*)r, ldr, nb, (double*)t, ldt);
-    if (prec == DFT_F32) return kinetic_exact_t<float>(c, g, sigma, (const float*)r, ldr, nb, (float*)t, ldt);
+    if (prec == DFT_F64)
+        return kinetic_exact_t<double>(c, g, sigma, theta, scale, (const double*)r, ldr, nb, (double*)t, ldt);
+    if (prec == DFT_F32)
+        return kinetic_exact_t<float>(c, g, sigma, theta, scale, (const float*)r, ldr, nb, (float*)t, ldt);
     return set_err(c, DFT_EINVAL, "dft_precond_kinetic_exact: bad precision");
+}
+
+}  // namespace
+
+extern "C" {
+
+int dft_precond_kinetic_exact(dft_ctx* c, const dft_grid* g, int prec, double sigma, const void* r, int64_t ldr, int nb,
+                              void* t, int64_t ldt) {
+    if (!c) return DFT_EINVAL;
+    return kinetic_exact(c, g, prec, sigma, nullptr, 0.0, r, ldr, nb, t, ldt);
+}
+
+int dft_precond_kinetic_exact_shifted(dft_ctx* c, const dft_grid* g, int prec, double sigma_min, double scale,
+                                      const double* theta, const void* r, int64_t ldr, int nb, void* t, int64_t ldt) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, theta && scale > 0.0, "dft_precond_kinetic_exact_shifted: bad arguments");
+    return kinetic_exact(c, g, prec, sigma_min, theta, scale, r, ldr, nb, t, ldt);
 }
 
 }  // extern "C"

@@ -249,10 +249,14 @@ int BlockDavidson::restart(int m, int mprev, int pmax) {
 
 int KineticPreconditioner::apply(dft_ctx* c, const dft_grid& g, int prec, const void* /*veff*/, const void* /*X*/,
                                  int64_t /*ldx*/, const void* R, int64_t ldr, int /*first*/, int ncols,
-                                 const double* /*theta_dev*/, void* T, int64_t ldt) {
+                                 const double* theta_dev, void* T, int64_t ldt) {
     if (ncols <= 0) return 0;
     if (mode_ == kExact) {
-        check(c, dft_precond_kinetic_exact(c, &g, prec, sigma_, R, ldr, ncols, T, ldt), "dft_precond_kinetic_exact");
+        if (scale_ > 0.0)
+            check(c, dft_precond_kinetic_exact_shifted(c, &g, prec, sigma_, scale_, theta_dev, R, ldr, ncols, T, ldt),
+                  "dft_precond_kinetic_exact_shifted");
+        else
+            check(c, dft_precond_kinetic_exact(c, &g, prec, sigma_, R, ldr, ncols, T, ldt), "dft_precond_kinetic_exact");
         return 0;
     }
     const int64_t n = g.nx * g.ny * g.nz;

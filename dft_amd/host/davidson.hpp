@@ -72,14 +72,16 @@ struct DiagonalPreconditioner : Preconditioner {
 class KineticPreconditioner : public Preconditioner {
    public:
     enum Mode { kJacobi = 0, kChebyshev = 1, kExact = 2 };
-    KineticPreconditioner(int sweeps = 4, double sigma = 1.0, double omega = 2.0 / 3.0, Mode mode = kJacobi)
-        : sweeps_(sweeps), sigma_(sigma), omega_(omega), mode_(mode) {}
+    // kExact with scale > 0: per-state shifts sigma_j = max(sigma, -scale theta_j)
+    KineticPreconditioner(int sweeps = 4, double sigma = 1.0, double omega = 2.0 / 3.0, Mode mode = kJacobi,
+                          double scale = 0.0)
+        : sweeps_(sweeps), sigma_(sigma), omega_(omega), scale_(scale), mode_(mode) {}
     int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* X, int64_t ldx, const void* R,
               int64_t ldr, int first, int ncols, const double* theta_dev, void* T, int64_t ldt) override;
 
    private:
     int sweeps_;
-    double sigma_, omega_;
+    double sigma_, omega_, scale_;
     Mode mode_;
     Buf kt_;  // scratch columns for the ping-pong steps
     int64_t kt_cols_ = 0;

@@ -44,7 +44,7 @@ double fermi_entropy(const double* occ, int n);
 std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g);
 // DFT_PRECOND_* -> preconditioner (throws invalid_argument for CNN without a model)
 std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g,
-                                                    int sweeps = 0, double shift = 0.0);
+                                                    int sweeps = 0, double shift = 0.0, double shift_scale = 0.0);
 
 class ScfDriver {
    public:

@@ -56,7 +56,8 @@ typedef struct {
     const void* cnn;       /* trained network (const dft_cnn_model*) for DFT_PRECOND_CNN */
     int verbose;           /* 1: one line per SCF step on stderr */
     int precond_sweeps;    /* DFT_PRECOND_KINETIC(_CHEB): smoothing steps per correction; 0: 4 (8 for _CHEB) */
-    double precond_shift;  /* DFT_PRECOND_KINETIC*: sigma in (-1/2 Lap + sigma)^-1; 0: 1.0 (_CHEB 1.5, _EXACT 2.0) */
+    double precond_shift;  /* DFT_PRECOND_KINETIC*: sigma in (-1/2 Lap + sigma)^-1; 0: 1.0 (_CHEB 1.5, _EXACT see
+                              precond_shift_scale) */
     int mixer;             /* DFT_MIX_LINEAR (the reference's lagged linear mixing) or DFT_MIX_PULAY */
     int mix_depth;         /* DFT_MIX_PULAY: history length; 0: 6 */
     double mix_beta;       /* DFT_MIX_PULAY: residual weight; 0: 0.3 */
@@ -67,6 +68,9 @@ typedef struct {
     int ortho_gram;        /* mixed mode: Gram products of the FP32 orthonormalisation (block Gram-Schmidt, CholQR):
                               0 or DFT_GRAM_F32: FP32 products; DFT_GRAM_SPLIT_BF16: as the Rayleigh-Ritz products */
     double mix_kerker;     /* DFT_MIX_PULAY: Kerker screening wave number q0 (1/bohr) of the update; 0: none */
+    double precond_shift_scale; /* DFT_PRECOND_KINETIC_EXACT: per-state shifts sigma_j = max(precond_shift,
+                                   -scale theta_j); 0: scale 0.35 with precond_shift 0.5 by default; < 0: one shift
+                                   (precond_shift, default 2.0) for every state */
 } dft_scf_config;
 
 typedef struct {

@@ -116,6 +116,11 @@ int dft_precond_kinetic_cheb(dft_ctx* ctx, const dft_grid* g, int prec, double s
  * No tmp argument: the context's workspace holds the BF16 blocks. */
 int dft_precond_kinetic_exact(dft_ctx* ctx, const dft_grid* g, int prec, double sigma, const void* r, int64_t ldr,
                               int nb, void* t, int64_t ldt);
+/* The same with a shift per column: sigma_j = max(sigma_min, -scale theta[j])
+ * (theta: nb device FP64 Ritz values), so each state's correction solves
+ * (-1/2 Lap + sigma_j) t = r with sigma_j following its own eigenvalue. */
+int dft_precond_kinetic_exact_shifted(dft_ctx* ctx, const dft_grid* g, int prec, double sigma_min, double scale,
+                                      const double* theta, const void* r, int64_t ldr, int nb, void* t, int64_t ldt);
 /* One damped-Jacobi step of the kinetic smoothing correction (no reference
  * counterpart; see dft_core.h DFT_PRECOND_KINETIC): first = 1: t = cf r;
  * otherwise t = t + cf (r - kt), per column. */

@@ -356,6 +356,28 @@ def test_xc_f32_extreme_gradients(dev, oracle):
     assert rel(got, ref_a) <= 1e-3
 
 
+def test_precond_kinetic_exact_shifted(dev):
+    """Per-column shifts sigma_j = max(sigma_min, -scale theta_j) give, column
+    by column, exactly the single-shift correction with that sigma (values
+    chosen so the shifts are exact in FP32)."""
+    import dft_amd
+    g = dft_amd.Grid.centered(24, 0.3)
+    n = g.size
+    rng = np.random.default_rng(21)
+    r = rng.standard_normal((3, n))
+    theta = np.array([-3.0, -0.5, -10.0])
+    sig = np.maximum(1.0, 0.5 * -theta)  # 1.5, 1.0, 5.0
+    for dt in (np.float32, np.float64):
+        dr = dev.array(r, dtype=dt)
+        out = dev.array(shape=(3, n), dtype=dt)
+        dev.precond_kinetic_exact_shifted(g, dr, out, 3, dev.array(theta), sigma_min=1.0, scale=0.5)
+        got = out.download()
+        for j in range(3):
+            one = dev.array(shape=(1, n), dtype=dt)
+            dev.precond_kinetic_exact(g, dev.array(r[j:j + 1], dtype=dt), one, 1, sigma=float(sig[j]))
+            assert np.array_equal(got[j], one.download()[0])
+
+
 def test_clamp_min(dev):
     x = np.array([1.0, -2.0, 0.0, np.nan, 3.5, -1e-30])
     for dt in (np.float64, np.float32):
