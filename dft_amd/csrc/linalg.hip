@@ -778,6 +778,13 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
     return DFT_OK;
 }
 
+int dft_block_update_in_place(int prec, int m, int k) {
+    if (m <= 0 || k <= 0) return 0;
+    if (prec == DFT_F64) return update_mfma_fits<double>(m, k) ? 1 : 0;
+    if (prec == DFT_F32) return update_mfma_fits<float>(m, k) || (k <= 64 && m <= 512) ? 1 : 0;
+    return 0;
+}
+
 int dft_syev(dft_ctx* c, int m, double* a, int64_t lda, double* w) {
     if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, a && w && m > 0 && lda >= m && m <= 4096, "dft_syev: bad arguments");

@@ -29,6 +29,7 @@ BlockDavidson::BlockDavidson(dft_ctx* c, const dft_grid& g, int prec, int nstate
     rn_ = Buf(c, sizeof(double) * (size_t)k_);
     Qprev_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
     Cb_ = Buf(c, sizeof(double) * (size_t)mmax_ * k_);
+    Cr_ = Buf(c, sizeof(double) * (size_t)mmax_ * 2 * k_);
     G_.zero();
 }
 
@@ -189,12 +190,15 @@ int BlockDavidson::append_block(int m, int nt, DavidsonStats& st) {
     return orthonormalise(T_.get(), nt, T, R_.get());
 }
 
-int BlockDavidson::restart(int m, int mprev, int pmax) {
+int BlockDavidson::restart_coefficients(int m, int mprev, int pmax, std::vector<double>& Q, std::vector<double>& Pc) {
     // Coefficients (in the current basis V, m columns) of the Ritz block X
     // (= Q[:, :k]) and of the previous Ritz block (Qprev padded with zeros).
     const int k = k_;
-    std::vector<double> Q((size_t)mmax_ * k), Qp((size_t)mmax_ * k, 0.0);
+    Q.assign((size_t)mmax_ * k, 0.0);
     check(c_, dft_download(c_, Q.data(), Q_.get(), sizeof(double) * (size_t)mmax_ * k), "dft_download");
+    Pc.assign((size_t)m * k, 0.0);
+    if (mprev <= 0 || pmax <= 0) return 0;
+    std::vector<double> Qp((size_t)mmax_ * k, 0.0);
     check(c_, dft_download(c_, Qp.data(), Qprev_.get(), sizeof(double) * (size_t)mmax_ * k), "dft_download");
     auto q = [&](int i, int j) -> double& { return Q[(size_t)j * mmax_ + i]; };
     auto p = [&](int i, int j) -> double& { return Qp[(size_t)j * mmax_ + i]; };
@@ -202,7 +206,6 @@ int BlockDavidson::restart(int m, int mprev, int pmax) {
         for (int i = mprev; i < m; ++i) p(i, j) = 0.0;
     // P = (I - Q Q^T) Qprev, twice, then modified Gram-Schmidt among its columns
     int kept = 0;
-    std::vector<double> Pc((size_t)m * k);
     for (int j = 0; j < k; ++j) {
         std::vector<double> v(m);
         for (int i = 0; i < m; ++i) v[i] = p(i, j);
@@ -226,6 +229,29 @@ int BlockDavidson::restart(int m, int mprev, int pmax) {
         for (int i = 0; i < m; ++i) Pc[(size_t)kept * m + i] = v[i] / nn;
         ++kept;
     }
+    return kept;
+}
+
+int BlockDavidson::restart_in_place(int m, int kept, const std::vector<double>& Q, const std::vector<double>& Pc) {
+    const int k = k_, kn = k + kept;
+    // [Q_k | P], m x (k + kept), column-major with ld m
+    std::vector<double> C((size_t)m * kn);
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i < m; ++i) C[(size_t)j * m + i] = Q[(size_t)j * mmax_ + i];
+    std::copy(Pc.begin(), Pc.begin() + (size_t)m * kept, C.begin() + (size_t)m * k);
+    check(c_, dft_upload(c_, Cr_.get(), C.data(), sizeof(double) * C.size()), "dft_upload");
+    // one pass each over V and W; every output point is written by the wave that has read all of its inputs
+    check(c_, dft_block_update(c_, prec_, n_, m, kn, 1.0, V_.get(), n_, Cr_.as<double>(), m, 0.0, V_.get(), n_),
+          "dft_block_update");
+    check(c_, dft_block_update(c_, prec_, n_, m, kn, 1.0, W_.get(), n_, Cr_.as<double>(), m, 0.0, W_.get(), n_),
+          "dft_block_update");
+    check(c_, dft_copy(c_, X_.get(), V_.get(), (size_t)n_ * es_ * k), "dft_copy");
+    gram(kn, kn, V_.get(), W_.get(), G_.as<double>(), mmax_);
+    return kn;
+}
+
+int BlockDavidson::restart(int m, int kept, const std::vector<double>& Pc) {
+    const int k = k_;
     const size_t colb = (size_t)n_ * es_;
     char* V = static_cast<char*>(V_.get());
     char* W = static_cast<char*>(W_.get());
@@ -381,13 +407,14 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             j = j1;
         }
         if (m + nt > mmax_) {  // thick restart on [X, P] before the new block is formed
-            ritz_block(true);
-            m = mprev > 0 ? restart(m, mprev, std::max(0, mmax_ - k - nt)) : 0;
-            if (m == 0) {
-                check(c_, dft_copy(c_, V, X_.get(), colb * k), "dft_copy");
-                check(c_, dft_copy(c_, W, HX_.get(), colb * k), "dft_copy");
-                m = k;
-                gram(k, k, V, W, G, mmax_);
+            std::vector<double> Qh, Pc;
+            const int kept = restart_coefficients(m, mprev, std::max(0, mmax_ - k - nt), Qh, Pc);
+            if (dft_block_update_in_place(prec_, m, k + kept)) {
+                m = restart_in_place(m, kept, Qh, Pc);
+                x_fresh = true;
+            } else {
+                ritz_block(true);
+                m = restart(m, kept, Pc);
             }
             ++st.restarts;
             mprev = 0;

@@ -137,7 +137,13 @@ class BlockDavidson {
     int orthonormalise(void* src, int cnt, void* dst, void* tmp);
     // Thick restart: V <- [X, P], W <- [HX, HP] with at most pmax columns of
     // P; returns the new basis size.
-    int restart(int m, int mprev, int pmax);
+    // Thick restart.  restart_coefficients: the current Ritz coefficients (host
+    // copy of Q, ld mmax) and P = (I - QQ^T) Qprev orthonormalised (m x kept);
+    // restart_in_place: [V | W][:, :k+kept] = [V | W] [Q_k | P] in one in-place
+    // pass each, X = V[:, :k]; restart: from X and HX formed by the caller.
+    int restart_coefficients(int m, int mprev, int pmax, std::vector<double>& Q, std::vector<double>& Pc);
+    int restart_in_place(int m, int kept, const std::vector<double>& Q, const std::vector<double>& Pc);
+    int restart(int m, int kept, const std::vector<double>& Pc);
     // The nt correction columns at V[:, m:m+nt] made orthonormal and
     // orthogonal to V[:, :m], in place: Pythagorean block Gram-Schmidt (one
     // Gram pass [V T]^T T and one update per pass, a second pass only when
@@ -152,7 +158,7 @@ class BlockDavidson {
     int64_t n_;
     size_t es_;
     Buf V_, W_, X_, HX_, R_, T_, PV_;
-    Buf G_, Q_, P_, S_, theta_, rn_, Qprev_, Cb_;
+    Buf G_, Q_, P_, S_, theta_, rn_, Qprev_, Cb_, Cr_;
     std::vector<double> theta_h_, rn_h_;
     int gram_mode_ = DFT_GRAM_F64;
     int ortho_mode_ = DFT_GRAM_F64;

@@ -203,6 +203,12 @@ int dft_gram(dft_ctx* ctx, int mode, int64_t n, int ma, int mb, double alpha, co
  * update of the last columns of a basis block). */
 int dft_block_update(dft_ctx* ctx, int prec, int64_t n, int m, int k, double alpha, const void* a, int64_t lda,
                      const double* c, int64_t ldc, double beta, void* y, int64_t ldy);
+/* 1 when dft_block_update with this precision and shape runs as one kernel
+ * pass that may overwrite A in place (Y = the first k columns of A) without
+ * a staging copy; 0 when an overlapping Y would be staged through scratch
+ * memory (n k values).  A helper for callers that choose between an
+ * in-place update and separate buffers; no reference counterpart. */
+int dft_block_update_in_place(int prec, int m, int k);
 /* Per-column scaling y[:,j] = x[:,j] * s[j] (s device FP64, nb values);
  * and residual r[:,j] = hx[:,j] - theta[j] x[:,j]. */
 int dft_scale_columns(dft_ctx* ctx, int prec, int64_t n, int nb, const void* x, int64_t ldx, const double* s,
