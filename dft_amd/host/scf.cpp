@@ -393,7 +393,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
 
     // Under the |dE| stop rule the orbitals need only be as accurate as the
     // SCF is settled: tol = clamp(min(0.1 dRho_prev, 0.3 sqrt(max(etol,
-    // 0.1 |dE_prev|))), floor, 1e-2) (smeared runs: below), the floor being the
+    // 0.1 |dE_prev|))), floor, 0.1) (smeared runs: below), the floor being the
     // configured tolerance (FP64) or the split-BF16 Rayleigh-Ritz floor (FP32
     // storage, ~3e-4).  The reference stop rule keeps the fixed tolerance of
     // EigenvalueSolverMolecule.cpp:19.
@@ -419,11 +419,11 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             // the density moves by ~r / 4kT, so r must stay below ~0.4 kT of dRho
             const double dd_tol = smear ? std::min(0.1, 0.4 * kT) * dd_prev : 0.1 * dd_prev;
             // Whole-orbital runs: while the energy still moves by |dE| the orbitals need
-            // only be converged to ~0.3 sqrt(0.1 |dE|) (capped at 1e-2): early steps,
+            // only be converged to ~0.3 sqrt(0.1 |dE|) (capped at 0.1): early steps,
             // whose density changes by O(1), take a few iterations each.  Smeared runs
             // keep the 1e-3 / 0.3 sqrt(etol) caps: a loose early solve there can settle
             // the fractional occupations on another stationary point.
-            const double cap = smear ? 1.0e-3 : 1.0e-2;
+            const double cap = smear ? 1.0e-3 : 1.0e-1;
             const double dE_ref = smear ? etol : std::max(etol, 0.1 * de_prev);
             o.tol = std::max(o.tol, std::min({cap, dd_tol, 0.3 * std::sqrt(dE_ref)}));
         }
