@@ -360,19 +360,21 @@ __global__ __launch_bounds__(256) void k_kinetic(int nx, int ny, int nz, double 
     const double w = el[o];
     double s = 0.0;
     if (w > 0.0) {
-        const int64_t n = (int64_t)nx * ny * nz, plane = (int64_t)nx * ny;
+        // the caller guarantees fewer than 2^31 points: 32-bit index arithmetic
+        const int n = nx * ny * nz, plane = nx * ny;
+        const double ix2 = 1.0 / dx2, iy2 = 1.0 / dy2, iz2 = 1.0 / dz2;
         const T* q = psi + (int64_t)o * ld;
-        const int64_t stride = (int64_t)gridDim.x * 256;
-        for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += stride) {
-            const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / plane);
+        const int stride = (int)gridDim.x * 256;
+        for (int p = (int)blockIdx.x * 256 + (int)threadIdx.x; p < n; p += stride) {
+            const int k = p / plane, r = p - k * plane, j = r / nx, i = r - j * nx;
             const double c = (double)q[p];
             double lap = 0.0;
-            if (i > 0) lap += ((double)q[p - 1] - c) / dx2;
-            if (i + 1 < nx) lap += ((double)q[p + 1] - c) / dx2;
-            if (j > 0) lap += ((double)q[p - nx] - c) / dy2;
-            if (j + 1 < ny) lap += ((double)q[p + nx] - c) / dy2;
-            if (k > 0) lap += ((double)q[p - plane] - c) / dz2;
-            if (k + 1 < nz) lap += ((double)q[p + plane] - c) / dz2;
+            if (i > 0) lap += ((double)q[p - 1] - c) * ix2;
+            if (i + 1 < nx) lap += ((double)q[p + 1] - c) * ix2;
+            if (j > 0) lap += ((double)q[p - nx] - c) * iy2;
+            if (j + 1 < ny) lap += ((double)q[p + nx] - c) * iy2;
+            if (k > 0) lap += ((double)q[p - plane] - c) * iz2;
+            if (k + 1 < nz) lap += ((double)q[p + plane] - c) * iz2;
             s += c * (-0.5 * lap);
         }
         s *= w;
@@ -420,7 +422,7 @@ int dft_kinetic_energy_occ(dft_ctx* c, const dft_grid* g, int prec, const void* 
     const int64_t n = grid_points(g);
     DFT_REQUIRE(c, ld >= n, "dft_kinetic_energy: ld < grid size");
     if (nb == 0) { *t_host = 0.0; return DFT_OK; }
-    DFT_REQUIRE(c, nb <= 65535, "dft_kinetic_energy: more than 65535 orbitals");
+    DFT_REQUIRE(c, nb <= 65535 && n < (1LL << 31), "dft_kinetic_energy: more than 65535 orbitals or 2^31 points");
     // ~8 workgroups per CU over all orbitals, at least one per orbital
     const int bx = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8 * (int64_t)c->num_cus / nb));
     const size_t parts = (size_t)bx * nb;
