@@ -365,16 +365,22 @@ __global__ __launch_bounds__(256) void k_kinetic(int nx, int ny, int nz, double 
         const double ix2 = 1.0 / dx2, iy2 = 1.0 / dy2, iz2 = 1.0 / dz2;
         const T* q = psi + (int64_t)o * ld;
         const int stride = (int)gridDim.x * 256;
+#pragma unroll 4
         for (int p = (int)blockIdx.x * 256 + (int)threadIdx.x; p < n; p += stride) {
             const int k = p / plane, r = p - k * plane, j = r / nx, i = r - j * nx;
+            // a missing neighbour is read as the point itself, so its term (psi - psi) / h^2
+            // is an exact zero: six unconditional loads the wave issues together
             const double c = (double)q[p];
+            const double xm = (double)q[p - (i > 0 ? 1 : 0)], xp = (double)q[p + (i + 1 < nx ? 1 : 0)];
+            const double ym = (double)q[p - (j > 0 ? nx : 0)], yp = (double)q[p + (j + 1 < ny ? nx : 0)];
+            const double zm = (double)q[p - (k > 0 ? plane : 0)], zp = (double)q[p + (k + 1 < nz ? plane : 0)];
             double lap = 0.0;
-            if (i > 0) lap += ((double)q[p - 1] - c) * ix2;
-            if (i + 1 < nx) lap += ((double)q[p + 1] - c) * ix2;
-            if (j > 0) lap += ((double)q[p - nx] - c) * iy2;
-            if (j + 1 < ny) lap += ((double)q[p + nx] - c) * iy2;
-            if (k > 0) lap += ((double)q[p - plane] - c) * iz2;
-            if (k + 1 < nz) lap += ((double)q[p + plane] - c) * iz2;
+            lap += (xm - c) * ix2;
+            lap += (xp - c) * ix2;
+            lap += (ym - c) * iy2;
+            lap += (yp - c) * iy2;
+            lap += (zm - c) * iz2;
+            lap += (zp - c) * iz2;
             s += c * (-0.5 * lap);
         }
         s *= w;
