@@ -423,8 +423,13 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             // whose density changes by O(1), take a few iterations each.  Smeared runs
             // keep the 1e-3 / 0.3 sqrt(etol) caps: a loose early solve there can settle
             // the fractional occupations on another stationary point.
-            const double cap = smear ? 1.0e-3 : 1.0e-1;
-            const double dE_ref = smear ? etol : std::max(etol, 0.1 * de_prev);
+            // The first (cold) solve keeps the tight caps too: starting from the seed
+            // vectors a loose solve can stop before a deep state (a 1s level the
+            // seeds barely overlap) enters the lowest Ritz values, and the warm
+            // solves that follow would not bring it back.
+            const bool loose = !smear && have_prev;
+            const double cap = loose ? 1.0e-1 : 1.0e-3;
+            const double dE_ref = loose ? std::max(etol, 0.1 * de_prev) : etol;
             o.tol = std::max(o.tol, std::min({cap, dd_tol, 0.3 * std::sqrt(dE_ref)}));
         }
         DavidsonStats st = D.solve(veff, have_prev, o, *precond);

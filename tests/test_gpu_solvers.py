@@ -295,3 +295,25 @@ def test_scf_smearing():
                          precond=dft_amd.PRECOND_KINETIC_EXACT, **ce)
     print("carbon mixed", mx["total"], mx["iterations"], mx["refine_iterations"])
     assert mx["converged"] and abs(mx["total"] - f64["total"]) <= 1e-6
+
+
+@pytest.mark.parametrize("init", ["bare", "hydrogenic"])
+def test_scf_mixed_neon_finds_the_fp64_state(init):
+    """A Ne atom (PBE96), whose 1s level the box-centred seed vectors barely
+    overlap: the mixed-precision SCF with the default Davidson settings
+    (exact kinetic correction, |dE|-following tolerances) must reach the same
+    state and energy as the FP64 SCF with the reference's diagonal
+    correction, from either starting guess."""
+    import dft_amd
+    g = dft_amd.Grid.centered(48, 0.25)
+    ne = [[10, 0.0, 0.0, 0.0]]
+    kw = dict(functional=dft_amd.PBE96, nstates=8, poisson_method=dft_amd.POISSON_DIRECT, stop_rule=dft_amd.STOP_DE,
+              energy_tol=1e-6, init_guess=dft_amd.INIT_BARE if init == "bare" else dft_amd.INIT_HYDROGENIC,
+              davidson_tol=1e-7, max_iterations=200)
+    ref = dft_amd.scf_run(g, ne, **kw)
+    mx = dft_amd.scf_run(g, ne, precision=dft_amd.SCF_MIXED, refine_iterations=50,
+                         precond=dft_amd.PRECOND_KINETIC_EXACT, **kw)
+    print(init, "fp64", ref["total"], ref["iterations"], "mixed", mx["total"], mx["iterations"], mx["eigenvalues"][:2])
+    assert ref["converged"] and mx["converged"]
+    assert abs(mx["total"] - ref["total"]) <= 1e-5
+    assert abs(mx["eigenvalues"][0] - ref["eigenvalues"][0]) <= 1e-3
