@@ -317,3 +317,28 @@ def test_scf_mixed_neon_finds_the_fp64_state(init):
     assert ref["converged"] and mx["converged"]
     assert abs(mx["total"] - ref["total"]) <= 1e-5
     assert abs(mx["eigenvalues"][0] - ref["eigenvalues"][0]) <= 1e-3
+
+
+@pytest.mark.parametrize("seed", [107, 108])
+def test_scf_smearing_converges_random_radicals(seed):
+    """Seeded random H/C/N/O molecules within 3 bohr (the benchmark's random
+    geometry rule) that never converge with whole-orbital occupations: with
+    Fermi-Dirac smearing (kT = 0.01) and Pulay mixing both precisions converge,
+    to the same free energy within 1e-5 Ha."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    import dft_amd
+    rng = np.random.default_rng(seed)
+    atoms = bench.random_cluster(seed, int(rng.integers(3, 7)), 3.0)
+    ne = int(sum(a[0] for a in atoms))
+    g = dft_amd.Grid.centered(96, 0.25)
+    kw = dict(functional=dft_amd.PZ81, nstates=ne // 2 + 10, poisson_method=dft_amd.POISSON_DIRECT,
+              stop_rule=dft_amd.STOP_DE, energy_tol=1e-5, init_guess=dft_amd.INIT_HYDROGENIC, davidson_tol=1e-7,
+              max_iterations=120, smearing=0.01, mixer=dft_amd.MIX_PULAY)
+    ref = dft_amd.scf_run(g, atoms, **kw)
+    mx = dft_amd.scf_run(g, atoms, precision=dft_amd.SCF_MIXED, refine_iterations=50,
+                         precond=dft_amd.PRECOND_KINETIC_EXACT, **kw)
+    print(seed, len(atoms), ne, ref["total"], ref["iterations"], mx["total"], mx["iterations"])
+    assert ref["converged"] and mx["converged"]
+    assert abs(mx["total"] - ref["total"]) <= 1e-5
