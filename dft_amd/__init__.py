@@ -498,6 +498,8 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
     na = (ne + multiplicity - 1) // 2
     nb = ne - na
     out["eigenvalues"] = eig[: na + nb].copy()
+    out["n_alpha"], out["n_beta"] = na, nb
+    out["config"], out["report"] = cfg, rep  # for scf_result_json
     if rho is not None:
         out["rho_alpha"], out["rho_beta"] = rho[: grid.size], rho[grid.size:]
     return out
@@ -677,6 +679,32 @@ class CnnTrainer:
             _core_err(rc)
         return xin, tgt
 
+    def add_patches(self, xin, tgt):
+        """Append training pairs (count x 2 x P^3 inputs, count x P^3 targets) after those held."""
+        P3 = self.patch ** 3
+        xin = np.ascontiguousarray(xin, dtype=np.float32).reshape(-1, 2, P3)
+        tgt = np.ascontiguousarray(tgt, dtype=np.float32).reshape(-1, P3)
+        if xin.shape[0] != tgt.shape[0]:
+            raise ValueError("inputs and targets hold different patch counts")
+        rc = self.L.dft_cnn_trainer_add_patches(self.h, C.c_int64(xin.shape[0]), xin.ctypes.data_as(C.POINTER(C.c_float)),
+                                                tgt.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            _core_err(rc)
+
+    def save_patches(self, path, first=0, count=None, target_scale=0, h=0.0):
+        """Write patches [first, first + count) as a patch shard (include/dft_io.h)."""
+        count = self.count() - first if count is None else count
+        xin, tgt = self.patches(first, count)
+        save_patch_shard(path, xin, tgt, target_scale=target_scale, h=h)
+
+    def load_patches(self, path):
+        """Append every patch of a shard; returns the shard's header."""
+        xin, tgt, hdr = load_patch_shard(path)
+        if hdr["patch"] != self.patch:
+            raise ValueError(f"shard patch edge {hdr['patch']} differs from the trainer's {self.patch}")
+        self.add_patches(xin, tgt)
+        return hdr
+
 
 class ScfFields(C.Structure):
     _fields_ = [(k, C.POINTER(C.c_double)) for k in ("vh", "vxc_alpha", "vxc_beta", "rho_alpha", "rho_beta", "vh_out")]
@@ -707,3 +735,142 @@ def scf_step(grid, nuclei, rho_alpha, rho_beta, functional=PZ81, multiplicity=1,
     out["dRho"] = rep.density_difference
     out["ks"] = rep.ks_residual
     return out
+
+
+# ---------------------------------------------------------------- on-disk formats (include/dft_io.h)
+class FieldHeader(C.Structure):
+    _fields_ = [("dtype", C.c_int), ("ncols", C.c_int64), ("grid", Grid), ("data_bytes", C.c_uint64),
+                ("crc", C.c_uint32), ("name", C.c_char * 64)]
+
+
+class PatchShardHeader(C.Structure):
+    _fields_ = [("channels", C.c_int), ("target_scale", C.c_int), ("patch", C.c_int64), ("count", C.c_int64),
+                ("h", C.c_double), ("crc_in", C.c_uint32), ("crc_tgt", C.c_uint32)]
+
+
+def _path(p):
+    return os.fsencode(os.fspath(p))
+
+
+def crc32(data, crc=0):
+    """CRC-32 (zlib's) of a bytes-like object or array, continuing from crc."""
+    L = core_lib()
+    L.dft_crc32.restype = C.c_uint32
+    buf = np.ascontiguousarray(np.frombuffer(memoryview(data).cast("B"), dtype=np.uint8))
+    return L.dft_crc32(C.c_uint32(crc), buf.ctypes.data_as(_vp), C.c_size_t(buf.size))
+
+
+def _field_dict(h):
+    return dict(dtype=np.float64 if h.dtype == F64 else np.float32, ncols=h.ncols,
+                grid=Grid.from_buffer_copy(h.grid), name=h.name.decode(), crc=h.crc, data_bytes=h.data_bytes)
+
+
+def save_field(path, grid, data, name=""):
+    """Write a grid field (N values) or an orbital block ((ncols, N): one column per row, the device layout).
+    data: a float64/float32 numpy array or a DeviceArray (downloaded first)."""
+    if isinstance(data, DeviceArray):
+        data = data.download()
+    a = np.ascontiguousarray(data)
+    prec = _prec_of(a.dtype)
+    n = grid.size
+    if a.size % n or a.size == 0:
+        raise ValueError(f"{a.size} values are not whole columns of the {n}-point grid")
+    rc = core_lib().dft_field_write(_path(path), C.byref(grid), C.c_int(prec), C.c_int64(a.size // n), name.encode(),
+                                    a.ctypes.data_as(_vp))
+    if rc != 0:
+        _core_err(rc)
+
+
+def field_info(path):
+    """Header of a field file as a dict (dtype, ncols, grid, name, crc, data_bytes)."""
+    h = FieldHeader()
+    rc = core_lib().dft_field_read_header(_path(path), C.byref(h))
+    if rc != 0:
+        _core_err(rc)
+    return _field_dict(h)
+
+
+def load_field(path):
+    """Read a field file (checksum verified); returns (grid, array, name): array is (N,) for one column,
+    else (ncols, N)."""
+    info = field_info(path)
+    n = info["grid"].size
+    shape = (n,) if info["ncols"] == 1 else (info["ncols"], n)
+    out = np.empty(shape, dtype=info["dtype"])
+    h = FieldHeader()
+    rc = core_lib().dft_field_read(_path(path), C.byref(h), out.ctypes.data_as(_vp), C.c_uint64(out.nbytes))
+    if rc != 0:
+        _core_err(rc)
+    info = _field_dict(h)
+    return info["grid"], out, info["name"]
+
+
+def save_patch_shard(path, xin, tgt, target_scale=0, h=0.0):
+    """Write training pairs: xin (count, 2, P^3) and tgt (count, P^3), FP32."""
+    xin = np.ascontiguousarray(xin, dtype=np.float32)
+    tgt = np.ascontiguousarray(tgt, dtype=np.float32)
+    if xin.ndim != 3 or tgt.ndim != 2:
+        raise ValueError("inputs must be (count, 2, P^3) and targets (count, P^3)")
+    count = tgt.shape[0]
+    P = int(round(tgt.shape[1] ** (1.0 / 3.0)))
+    if xin.shape != (count, 2, P ** 3) or tgt.shape != (count, P ** 3):
+        raise ValueError("inputs must be (count, 2, P^3) and targets (count, P^3)")
+    rc = core_lib().dft_patch_shard_write(_path(path), C.c_int64(P), C.c_int64(count), C.c_int(target_scale),
+                                          C.c_double(h), xin.ctypes.data_as(C.POINTER(C.c_float)),
+                                          tgt.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        _core_err(rc)
+
+
+def patch_shard_info(path):
+    h = PatchShardHeader()
+    rc = core_lib().dft_patch_shard_read_header(_path(path), C.byref(h))
+    if rc != 0:
+        _core_err(rc)
+    return {k: getattr(h, k) for k, _ in PatchShardHeader._fields_}
+
+
+def load_patch_shard(path, first=0, count=None):
+    """Read patches [first, first + count) of a shard; returns (xin, tgt, header)."""
+    hdr = patch_shard_info(path)
+    count = hdr["count"] - first if count is None else count
+    P3 = hdr["patch"] ** 3
+    xin = np.empty((count, 2, P3), dtype=np.float32)
+    tgt = np.empty((count, P3), dtype=np.float32)
+    rc = core_lib().dft_patch_shard_read(_path(path), C.c_int64(first), C.c_int64(count),
+                                         xin.ctypes.data_as(C.POINTER(C.c_float)), tgt.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        _core_err(rc)
+    return xin, tgt, hdr
+
+
+def scf_result_json(grid, nuclei, result):
+    """The result document (include/dft_io.h) of a scf_run result, as a string."""
+    L = core_lib()
+    nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
+    eig = np.ascontiguousarray(result["eigenvalues"], dtype=np.float64)
+    na, nb = result["n_alpha"], result["n_beta"]
+    if eig.size != na + nb:
+        raise ValueError("eigenvalue count differs from n_alpha + n_beta")
+    args = [C.byref(grid), C.c_int(nuc.shape[0]), nuc.ctypes.data_as(_dp), C.byref(result["config"]),
+            C.byref(result["report"]), eig.ctypes.data_as(_dp), C.c_int(na), C.c_int(nb)]
+    n = C.c_size_t()
+    rc = L.dft_scf_result_json(*args, None, C.c_size_t(0), C.byref(n))
+    if rc != 0:
+        _core_err(rc)
+    buf = C.create_string_buffer(n.value + 1)
+    rc = L.dft_scf_result_json(*args, buf, C.c_size_t(n.value + 1), C.byref(n))
+    if rc != 0:
+        _core_err(rc)
+    return buf.value.decode()
+
+
+def save_scf_result(path, grid, nuclei, result, fields=True):
+    """Write the result document to path; with fields, also the output spin densities (when the run
+    kept them, want_density=True) as field files path + '.rho_alpha.dftf' / '.rho_beta.dftf'."""
+    with open(path, "w") as f:
+        f.write(scf_result_json(grid, nuclei, result))
+    if fields:
+        for k in ("rho_alpha", "rho_beta"):
+            if result.get(k) is not None:
+                save_field(f"{os.fspath(path)}.{k}.dftf", grid, result[k], name=k)

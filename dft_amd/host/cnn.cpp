@@ -539,4 +539,16 @@ int dft_cnn_trainer_get_patches(dft_cnn_trainer* t, int64_t first, int64_t count
     });
 }
 
+int dft_cnn_trainer_add_patches(dft_cnn_trainer* t, int64_t count, const float* in, const float* tgt) {
+    return guarded_cnn([&] {
+        if (!t || count < 0 || (count > 0 && (!in || !tgt))) throw std::invalid_argument("dft_cnn_trainer_add_patches: bad arguments");
+        if (count > t->ps->cap - t->ps->count) throw std::invalid_argument("dft_cnn_trainer_add_patches: more patches than free capacity");
+        if (count == 0) return;
+        const int64_t P3 = t->ps->P3, first = t->ps->count;
+        dftc::check(t->ctx, dft_upload(t->ctx, t->ps->in.as<float>() + 2 * P3 * first, in, sizeof(float) * 2 * P3 * count), "dft_upload");
+        dftc::check(t->ctx, dft_upload(t->ctx, t->ps->tgt.as<float>() + P3 * first, tgt, sizeof(float) * P3 * count), "dft_upload");
+        t->ps->count += count;
+    });
+}
+
 }  // extern "C"

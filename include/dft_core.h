@@ -186,6 +186,9 @@ int dft_cnn_trainer_get_params(dft_cnn_trainer* t, float* params, int64_t count)
 int dft_cnn_trainer_set_params(dft_cnn_trainer* t, const float* params, int64_t count);
 /* Copy patches [first, first + count) to the host: in (count x 2 x P^3), tgt (count x P^3). */
 int dft_cnn_trainer_get_patches(dft_cnn_trainer* t, int64_t first, int64_t count, float* in, float* tgt);
+/* Append count patches from the host (same layout) after the patches held,
+ * e.g. from a patch shard (dft_io.h); -1 when they exceed the free capacity. */
+int dft_cnn_trainer_add_patches(dft_cnn_trainer* t, int64_t count, const float* in, const float* tgt);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|void\*|const char\*|void)\s+\**(dft_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(?:int|void\*|const char\*|void|uint32_t)\s+\**(dft_\w+)\s*\(", text)))
 
 
 def exported(lib):
@@ -40,6 +40,15 @@ def test_core_library_exports_header(libs):
     import dft_amd
     names = declared("dft_core.h")
     assert {"dft_scf_run", "dft_davidson_run", "dft_core_last_error"} <= set(names)
+    missing = sorted(set(names) - exported(dft_amd.CORE_SO))
+    assert not missing, missing
+
+
+def test_io_library_exports_header(libs):
+    import dft_amd
+    names = declared("dft_io.h")
+    assert {"dft_crc32", "dft_field_write", "dft_field_read", "dft_patch_shard_write", "dft_patch_shard_read",
+            "dft_scf_result_json"} <= set(names)
     missing = sorted(set(names) - exported(dft_amd.CORE_SO))
     assert not missing, missing
 
