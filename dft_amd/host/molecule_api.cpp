@@ -187,9 +187,12 @@ double calculateMolecularExchangeCorrelationEnergy(const XCFunctional& functiona
 }
 
 // ---- ElectronDensityMolecule.cpp:5-76 ------------------------------------------
-std::vector<double> calculateMolecularSpinDensity(const std::vector<MolecularOrbital>& orbitals, SpinChannel spin) {
+namespace {
+// calculateMolecularSpinDensity's checks and sum (ElectronDensityMolecule.cpp:5-76), leaving
+// the density on the device; n receives its length.
+dftc::Buf spin_density_device(const std::vector<MolecularOrbital>& orbitals, SpinChannel spin, size_t& n) {
     if (orbitals.empty()) throw std::invalid_argument("La lista de orbitales moleculares no puede estar vacia.");
-    std::size_t n = 0;
+    n = 0;
     bool init = false;
     std::vector<const MolecularOrbital*> occ;
     for (const MolecularOrbital& o : orbitals) {
@@ -205,17 +208,47 @@ std::vector<double> calculateMolecularSpinDensity(const std::vector<MolecularOrb
         for (const MolecularOrbital& o : orbitals)
             if (!o.psi.empty()) { n = o.psi.size(); init = true; break; }
     if (!init) throw std::invalid_argument("No existen funciones de onda moleculares validas.");
-    if (occ.empty()) return std::vector<double>(n, 0.0);
+    dftc::Buf out(ctx(), n * 8);
+    if (occ.empty()) {
+        out.zero();
+        return out;
+    }
     std::vector<double> block(occ.size() * n), el(occ.size());
     for (size_t i = 0; i < occ.size(); ++i) {
         std::copy(occ[i]->psi.begin(), occ[i]->psi.end(), block.begin() + i * n);
         el[i] = (double)occ[i]->electrons;
     }
     auto dp = up(block);
-    dftc::Buf out(ctx(), n * 8);
     dftc::check(ctx(), dft_density(ctx(), DFT_F64, dp.get(), (int64_t)n, (int)occ.size(), el.data(), (int64_t)n, out.get(), 0),
                 "dft_density");
-    return down(out, n);
+    return out;
+}
+}  // namespace
+
+std::vector<double> calculateMolecularSpinDensity(const std::vector<MolecularOrbital>& orbitals, SpinChannel spin) {
+    size_t n = 0;
+    dftc::Buf d = spin_density_device(orbitals, spin, n);
+    return down(d, n);
+}
+
+std::vector<double> calculateMolecularElectronDensity(const std::vector<MolecularOrbital>& orbitals) {
+    size_t na = 0, nb = 0;
+    dftc::Buf da = spin_density_device(orbitals, SpinChannel::Alpha, na);
+    dftc::Buf db = spin_density_device(orbitals, SpinChannel::Beta, nb);
+    if (na != nb) throw std::runtime_error("Las densidades alpha y beta deben tener el mismo tamano.");
+    // alpha + beta pointwise: 1 * beta + 1 * alpha rounds exactly as alpha + beta
+    dftc::check(ctx(), dft_axpby(ctx(), DFT_F64, (int64_t)na, 1.0, db.get(), 1.0, da.get()), "dft_axpby");
+    return down(da, na);
+}
+
+double integrateMolecularElectronDensity(const std::vector<double>& density, double dx, double dy, double dz) {
+    if (density.empty()) throw std::invalid_argument("La densidad molecular no puede estar vacia.");
+    if (dx <= 0.0 || dy <= 0.0 || dz <= 0.0)
+        throw std::invalid_argument("Los pasos de la malla cartesiana deben ser mayores que cero.");
+    auto d = up(density);
+    double s = 0.0;
+    dftc::check(ctx(), dft_reduce(ctx(), DFT_RED_SUM, DFT_F64, d.get(), nullptr, (int64_t)density.size(), &s), "dft_reduce");
+    return s * dx * dy * dz;
 }
 
 // ---- TotalEnergyMolecule.cpp:13-333 --------------------------------------------

@@ -181,6 +181,10 @@ double calculateMolecularExchangeCorrelationEnergy(const XCFunctional& functiona
                                                    const std::vector<double>& betaDensity);
 
 std::vector<double> calculateMolecularSpinDensity(const std::vector<MolecularOrbital>& orbitals, SpinChannel spin);
+// n = n_alpha + n_beta (ElectronDensityMolecule.cpp:78-114); runtime_error when the two differ in length
+std::vector<double> calculateMolecularElectronDensity(const std::vector<MolecularOrbital>& orbitals);
+// sum(n) dx dy dz (ElectronDensityMolecule.cpp:116-140)
+double integrateMolecularElectronDensity(const std::vector<double>& density, double dx, double dy, double dz);
 
 double calculateMolecularKineticEnergy(const CartesianGrid& grid, const std::vector<MolecularOrbital>& orbitals);
 double calculateMolecularExternalEnergy(const CartesianGrid& grid, const std::vector<double>& density,

@@ -490,6 +490,37 @@ int og_spin_density(long n, int norb, const double* psi, const int* electrons, d
     return 0;
 }
 
+int og_electron_density(long n, int norb, const double* psi, const int* electrons, const int* spins, double* out) {
+    /* ElectronDensityMolecule.cpp:78-114: the alpha and the beta spin
+     * density, each summed over its own channel's occupied orbitals in list
+     * order (:57-73), then added pointwise.  spins[o]: 0 alpha, 1 beta. */
+    if (norb <= 0) return fail(-1, "La lista de orbitales moleculares no puede estar vacia.");
+    double* beta = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!beta) return fail(-2, "out of memory");
+    memset(out, 0, sizeof(double) * (size_t)n);
+    memset(beta, 0, sizeof(double) * (size_t)n);
+    for (int o = 0; o < norb; ++o) {
+        if (electrons[o] <= 0) continue;
+        double* acc = spins[o] ? beta : out;
+        const double* p = psi + (size_t)o * (size_t)n;
+        for (long i = 0; i < n; ++i) acc[i] += (double)electrons[o] * p[i] * p[i];
+    }
+    for (long i = 0; i < n; ++i) out[i] = out[i] + beta[i];
+    free(beta);
+    return 0;
+}
+
+int og_integrate_density(long n, const double* rho, double dx, double dy, double dz, double* electrons) {
+    /* ElectronDensityMolecule.cpp:116-140: plain sum in index order, then
+     * ((sum * dx) * dy) * dz */
+    if (n <= 0) return fail(-1, "La densidad molecular no puede estar vacia.");
+    if (dx <= 0.0 || dy <= 0.0 || dz <= 0.0) return fail(-1, "Los pasos de la malla cartesiana deben ser mayores que cero.");
+    double s = 0.0;
+    for (long i = 0; i < n; ++i) s += rho[i];
+    *electrons = s * dx * dy * dz;
+    return 0;
+}
+
 double og_kinetic_energy(const og_grid* g, int norb, const double* psi, const int* electrons) {
     /* TotalEnergyMolecule.cpp:13-199: sum over existing neighbours of
      * (psi_nb - psi)/h^2 (no ghost term), -1/2 applied, dV-weighted. */

@@ -61,6 +61,10 @@ double og_xc_energy(const og_grid* g, int functional, const double* na, const do
 
 /* ElectronDensityMolecule.cpp:5-76 (orbitals stored column-major, N each). */
 int og_spin_density(long n, int norb, const double* psi, const int* electrons, double* out);
+/* ElectronDensityMolecule.cpp:78-114: n_alpha + n_beta; spins[o] 0 alpha, 1 beta. */
+int og_electron_density(long n, int norb, const double* psi, const int* electrons, const int* spins, double* out);
+/* ElectronDensityMolecule.cpp:116-140: sum(rho) dx dy dz. */
+int og_integrate_density(long n, const double* rho, double dx, double dy, double dz, double* electrons);
 
 /* TotalEnergyMolecule.cpp:13-199 (Neumann-like stencil) */
 double og_kinetic_energy(const og_grid* g, int norb, const double* psi, const int* electrons);

@@ -155,6 +155,23 @@ class Oracle:
         self._chk(self.L.og_spin_density(C.c_long(n), C.c_int(norb), _d(psi), el.ctypes.data_as(_ip), _d(out)))
         return out
 
+    def electron_density(self, psi, electrons, spins):
+        psi = _f64(psi)
+        norb, n = psi.shape
+        out = np.empty(n)
+        el = np.ascontiguousarray(electrons, dtype=np.int32)
+        sp = np.ascontiguousarray(spins, dtype=np.int32)
+        self._chk(self.L.og_electron_density(C.c_long(n), C.c_int(norb), _d(psi), el.ctypes.data_as(_ip),
+                                             sp.ctypes.data_as(_ip), _d(out)))
+        return out
+
+    def integrate_density(self, rho, dx, dy, dz):
+        r = _f64(rho)
+        e = C.c_double()
+        self._chk(self.L.og_integrate_density(C.c_long(r.size), _d(r), C.c_double(dx), C.c_double(dy), C.c_double(dz),
+                                              C.byref(e)))
+        return e.value
+
     def kinetic_energy(self, g, psi, electrons):
         psi = _f64(psi).reshape(-1, g.size)
         el = np.ascontiguousarray(electrons, dtype=np.int32)
@@ -287,6 +304,23 @@ class Reference:
         el = np.ascontiguousarray(electrons, dtype=np.int32)
         self._chk(self.L.rf_spin_density(C.c_long(n), C.c_int(norb), _d(psi), el.ctypes.data_as(_ip), _d(out)))
         return out
+
+    def electron_density(self, psi, electrons, spins):
+        psi = _f64(psi)
+        norb, n = psi.shape
+        out = np.empty(n)
+        el = np.ascontiguousarray(electrons, dtype=np.int32)
+        sp = np.ascontiguousarray(spins, dtype=np.int32)
+        self._chk(self.L.rf_electron_density(C.c_long(n), C.c_int(norb), _d(psi), el.ctypes.data_as(_ip),
+                                             sp.ctypes.data_as(_ip), _d(out)))
+        return out
+
+    def integrate_density(self, rho, dx, dy, dz):
+        r = _f64(rho)
+        e = C.c_double()
+        self._chk(self.L.rf_integrate_density(C.c_long(r.size), _d(r), C.c_double(dx), C.c_double(dy), C.c_double(dz),
+                                              C.byref(e)))
+        return e.value
 
     def kinetic_energy(self, g, psi, electrons):
         psi = _f64(psi).reshape(-1, g.size)

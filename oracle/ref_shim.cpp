@@ -143,6 +143,25 @@ int rf_spin_density(long npts, int norb, const double* psi, const int* electrons
     });
 }
 
+// calculateMolecularElectronDensity (spins[o]: 0 alpha, 1 beta)
+int rf_electron_density(long npts, int norb, const double* psi, const int* electrons, const int* spins, double* out) {
+    return guarded([&] {
+        std::vector<MolecularOrbital> orbs(norb);
+        for (int o = 0; o < norb; ++o) {
+            orbs[o].spin = spins[o] ? SpinChannel::Beta : SpinChannel::Alpha;
+            orbs[o].electrons = electrons[o];
+            orbs[o].psi = vec(psi + (size_t)o * npts, npts);
+        }
+        auto r = calculateMolecularElectronDensity(orbs);
+        std::memcpy(out, r.data(), r.size() * sizeof(double));
+    });
+}
+
+// integrateMolecularElectronDensity
+int rf_integrate_density(long npts, const double* rho, double dx, double dy, double dz, double* e) {
+    return guarded([&] { *e = integrateMolecularElectronDensity(vec(rho, npts), dx, dy, dz); });
+}
+
 // calculateMolecularKineticEnergy / calculateMolecularExternalEnergy
 int rf_kinetic(const long* n, const double* b, int norb, const double* psi, const int* electrons, double* e) {
     return guarded([&] {

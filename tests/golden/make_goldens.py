@@ -113,6 +113,16 @@ def main():
         out[f"scf_{name}_eig"] = r["eigenvalues"]
         print(name, r["total"], r["iterations"])
 
+    # total density over both spin channels (mixed list order, an empty orbital) and its integral;
+    # drawn last so the fixtures above keep their random inputs
+    g = Grid(7, 8, 6, -1.9, 2.1, -2.0, 1.6, -1.4, 1.8)
+    psi = rng.standard_normal((5, g.size))
+    el = np.array([1, 1, 0, 2, 1], dtype=np.int32)
+    spins = np.array([0, 1, 0, 1, 1], dtype=np.int32)
+    dens = R.electron_density(psi, el, spins)
+    out.update(edens_grid=np.array(g.n + g.bounds, dtype=float), edens_psi=psi, edens_el=el, edens_spins=spins,
+               edens_density=dens, edens_integral=np.array(R.integrate_density(dens, g.dx, g.dy, g.dz)))
+
     path = os.path.join(HERE, "reference_goldens.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

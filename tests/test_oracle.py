@@ -70,6 +70,15 @@ def test_density_energies_golden(oracle, goldens):
     assert oracle.nuclear_repulsion(nuc) == float(goldens["misc_enn"])
 
 
+def test_electron_density_golden(oracle, goldens):
+    """calculateMolecularElectronDensity and integrateMolecularElectronDensity (ElectronDensityMolecule.cpp:78-140)
+    on the reference's own outputs, bitwise."""
+    g = grid_from(goldens["edens_grid"])
+    d = oracle.electron_density(goldens["edens_psi"], goldens["edens_el"], goldens["edens_spins"])
+    np.testing.assert_array_equal(d, goldens["edens_density"])
+    assert oracle.integrate_density(d, g.dx, g.dy, g.dz) == float(goldens["edens_integral"])
+
+
 def test_davidson_golden(oracle, goldens):
     g = grid_from(goldens["dav_grid"])
     eig, psi, _ = oracle.davidson(g, goldens["dav_v"], 4)
@@ -259,6 +268,33 @@ def test_density_closed_form_and_charge(oracle):
     assert abs(n.sum() * dv - 4.0) < 1e-12
 
 
+def test_electron_density_channels_and_charge(oracle):
+    """n = n_alpha + n_beta with each channel summed over its own orbitals; unoccupied orbitals (electrons <= 0)
+    add nothing; normalised orbitals integrate to the electron count; a constant density integrates to
+    c N dV.  A swapped channel or a dropped orbital changes the sums checked here."""
+    g = Grid.centered(9, 0.45)
+    rng = np.random.default_rng(9)
+    psi = rng.standard_normal((6, g.size))
+    dv = g.dx * g.dy * g.dz
+    psi /= np.sqrt((psi ** 2).sum(1, keepdims=True) * dv)
+    el = [1, 2, 1, 0, -1, 1]
+    spins = [1, 0, 0, 1, 0, 1]
+    d = oracle.electron_density(psi, el, spins)
+    na = oracle.spin_density(psi[[1, 2]], [2, 1])
+    nb = oracle.spin_density(psi[[0, 5]], [1, 1])
+    np.testing.assert_array_equal(d, na + nb)
+    np.testing.assert_allclose(d, 2 * psi[1] ** 2 + psi[2] ** 2 + psi[0] ** 2 + psi[5] ** 2, rtol=1e-14)
+    assert abs(oracle.integrate_density(d, g.dx, g.dy, g.dz) - 5.0) < 1e-12
+    c = 0.3
+    assert abs(oracle.integrate_density(np.full(g.size, c), g.dx, g.dy, g.dz) - c * g.size * dv) < 1e-13 * c * g.size * dv
+    with pytest.raises(Exception):
+        oracle.integrate_density(np.zeros(0), g.dx, g.dy, g.dz)
+    with pytest.raises(Exception):
+        oracle.integrate_density(d, g.dx, 0.0, g.dz)
+    with pytest.raises(Exception):
+        oracle.electron_density(np.zeros((0, g.size)), [], [])
+
+
 def test_kinetic_neumann_stencil_by_matrix(oracle):
     g = Grid(3, 3, 3, -1.0, 1.0, -0.5, 0.7, -0.9, 0.9)
     rng = np.random.default_rng(8)
@@ -329,3 +365,15 @@ def test_against_live_reference_random_grid(oracle):
     for fid in (1, 2):
         for s in (0, 1):
             np.testing.assert_array_equal(oracle.xc_potential(g, fid, na, nb, s), R.xc_potential(g, fid, na, nb, s))
+
+
+@needs_ref
+def test_electron_density_against_live_reference(oracle):
+    R = Reference()
+    rng = np.random.default_rng(13)
+    g = Grid(6, 5, 7, -1.0, 1.5, -1.2, 1.1, -0.9, 1.3)
+    psi = rng.standard_normal((4, g.size))
+    el, spins = [2, 1, 1, 0], [0, 1, 1, 0]
+    d = R.electron_density(psi, el, spins)
+    np.testing.assert_array_equal(oracle.electron_density(psi, el, spins), d)
+    assert oracle.integrate_density(d, g.dx, g.dy, g.dz) == R.integrate_density(d, g.dx, g.dy, g.dz)
