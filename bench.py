@@ -24,6 +24,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from dft_amd.geometry import benzene, random_cluster  # noqa: E402  (workload geometries)
+
 METRIC = "SCF time to |dE|<1e-5 Ha: mixed+learned vs plain FP64 on 1 GPU; H·psi GB/s"
 HBM_PEAK_GBS = 8000.0
 PROFILES = os.path.join(ROOT, "profiles")
@@ -43,45 +45,6 @@ def profiled_traffic(key):
                 if line.startswith("traffic_bytes_per_launch ") and line.split()[1].rstrip(":") == key:
                     return float(line.split(":")[1])
     return None
-
-
-def benzene():
-    # planar ring in z=0: C at radius 2.64 bohr (C-C = 2.64), H at 2.64 + 2.05
-    out = []
-    for k in range(6):
-        a = np.pi / 3 * k
-        out.append([6, 2.64 * np.cos(a), 2.64 * np.sin(a), 0.0])
-    for k in range(6):
-        a = np.pi / 3 * k
-        out.append([1, 4.69 * np.cos(a), 4.69 * np.sin(a), 0.0])
-    return out
-
-
-def random_cluster(seed, natoms, radius, zs=(1, 6, 7, 8), min_dist=2.0, ne_range=None):
-    """Seeded random molecule: Z uniform in zs, positions uniform in a ball,
-    pairwise distance >= min_dist; an H is added when the electron count is
-    odd; ne_range rejects clusters outside [lo, hi] electrons."""
-    rng = np.random.default_rng(seed)
-    while True:
-        pos = []
-        while len(pos) < natoms:
-            p = rng.uniform(-radius, radius, 3)
-            if np.linalg.norm(p) > radius:
-                continue
-            if all(np.linalg.norm(p - q) >= min_dist for q in pos):
-                pos.append(p)
-        Z = rng.choice(zs, natoms)
-        atoms = [[float(z), *p] for z, p in zip(Z, pos)]
-        ne = int(sum(Z))
-        if ne % 2:
-            for _ in range(1000):
-                p = rng.uniform(-radius, radius, 3)
-                if np.linalg.norm(p) <= radius and all(np.linalg.norm(p - np.array(a[1:])) >= min_dist for a in atoms):
-                    atoms.append([1.0, *p])
-                    ne += 1
-                    break
-        if ne % 2 == 0 and (ne_range is None or ne_range[0] <= ne <= ne_range[1]):
-            return atoms
 
 
 WORKLOADS = {
