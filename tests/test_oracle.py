@@ -377,3 +377,23 @@ def test_electron_density_against_live_reference(oracle):
     d = R.electron_density(psi, el, spins)
     np.testing.assert_array_equal(oracle.electron_density(psi, el, spins), d)
     assert oracle.integrate_density(d, g.dx, g.dy, g.dz) == R.integrate_density(d, g.dx, g.dy, g.dz)
+
+
+def test_full_size_closed_forms_on_small_grids(oracle):
+    """The closed forms of the full-size GPU checks hold for the oracle: sine
+    modes are H psi eigenvectors (a constant potential shifts the eigenvalue),
+    and the boundary sine mode is the converged Jacobi solution."""
+    from fullsize_forms import boundary_sine_mode, dirichlet_sine_modes
+    # spacings exact in binary: xmin + (n - 1) dx lands on xmax, so every boundary point takes the
+    # reference's zero branch (a face point that misses by rounding takes Q / |r| instead)
+    g = Grid(13, 11, 9, -1.5, 1.5, -1.25, 1.25, -1.0, 1.0)
+    h = (g.dx, g.dy, g.dz)
+    psi, lam = dirichlet_sine_modes(g.n, h, [(1, 1, 1), (2, 5, 3), (13, 11, 9)])
+    c = -0.37
+    for o in range(3):
+        got = oracle.hpsi(g, np.full(g.size, c), psi[o])
+        np.testing.assert_allclose(got, (lam[o] + c) * psi[o], rtol=0, atol=1e-12 * np.max(np.abs(got)))
+    rho, v = boundary_sine_mode(g.n, h)
+    ref, _, md = oracle.poisson(g, rho, v0=np.zeros(g.size), max_sweeps=400000, tol=1e-15)
+    assert md < 1e-15
+    np.testing.assert_allclose(ref, v, rtol=0, atol=1e-11 * np.max(np.abs(v)))
