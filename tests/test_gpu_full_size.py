@@ -153,3 +153,52 @@ def test_xc_benzene_grid_against_oracle(dev, oracle, fid):
     # every term n eps_xc dV is negative, so the oracle's sum in index order is within (N - 1) 2^-53 of
     # the exact sum, relative (4.5e-10 at 160^3); the device's tree sum is far closer
     assert abs(e - ref_e) <= (og.size - 1) * 2.0 ** -53 * abs(ref_e)
+
+
+def test_davidson_products_benzene_shape_exact(dev):
+    """The Davidson's dense products at the timed workload's shape: N = 160^3
+    rows, a basis of 87 columns, a block of 29.  Operands in {-1, 0, 1} (and
+    small integer coefficients) make every product and every partial sum an
+    integer below 2^24, exact in FP32, BF16 and FP64 alike, so each mode must
+    return the integer result bit for bit: the Gram in FP64, FP32 and
+    split-BF16 (V^T V and V^T W), the block update Y = V C (and Y += V C), and
+    the Ritz residual R = W Q - V Q diag(theta), whose squared column norms
+    agree at the FP64 bar."""
+    import dft_amd
+    n, m, k = 160 ** 3, 87, 29
+    print(f"[full size] Davidson products {n} x {m} x {k}", flush=True)
+    rng = np.random.default_rng(87)
+    V8 = rng.integers(-1, 2, size=(m, n), dtype=np.int8)
+    W8 = rng.integers(-1, 2, size=(m, n), dtype=np.int8)
+    Cm = rng.integers(-2, 3, size=(k, m)).astype(np.float64)  # column-major m x k
+    th = rng.integers(-2, 3, size=k).astype(np.float64)
+    V32, W32 = V8.astype(np.float32), W8.astype(np.float32)
+    ref_vv = (V32 @ V32.T).astype(np.float64)  # integers below 2^24: exact in FP32 arithmetic
+    ref_vw = (V32 @ W32[:k].T).astype(np.float64)
+    ref_y = (Cm.astype(np.float32) @ V32).astype(np.float64)
+    ref_r = (Cm.astype(np.float32) @ W32 - th.astype(np.float32)[:, None] * (Cm.astype(np.float32) @ V32)).astype(np.float64)
+    assert np.max(np.abs(ref_vv)) < 2 ** 24 and np.max(np.abs(ref_r)) < 2 ** 24
+    dC, dth = dev.array(Cm), dev.array(th)
+    for dt, modes in ((np.float64, (dft_amd.GRAM_F64,)), (np.float32, (dft_amd.GRAM_F32, dft_amd.GRAM_SPLIT_BF16))):
+        dV, dW = dev.array(V8.astype(dt), dtype=dt), dev.array(W8.astype(dt), dtype=dt)
+        for mode in modes:
+            out = dev.zeros((m, m))
+            dev.gram(mode, dV, dV, out, n, m, m)
+            assert np.array_equal(out.download().T, ref_vv), ("V^T V", mode)
+            out = dev.zeros((k, m))
+            dev.gram(mode, dV, dW, out, n, m, k)
+            assert np.array_equal(out.download().T, ref_vw), ("V^T W", mode)
+        y = dev.array(np.full((k, n), np.nan), dtype=dt)  # beta = 0 never reads Y
+        dev.block_update(dV, dC, y, n, m, k, alpha=1.0, beta=0.0)
+        assert np.array_equal(y.download().astype(np.float64), ref_y), ("V C", dt)
+        dev.block_update(dV, dC, y, n, m, k, alpha=1.0, beta=1.0)
+        assert np.array_equal(y.download().astype(np.float64), 2.0 * ref_y), ("V C + Y", dt)
+        r = dev.array(np.full((k, n), np.nan), dtype=dt)
+        rn = dev.zeros((k,))
+        dev.ritz_residual(dV, dW, dC, dth, r, rn, n, m, k)
+        got = r.download().astype(np.float64)
+        assert np.array_equal(got, ref_r), ("ritz residual", dt)
+        want = np.sum(ref_r ** 2, axis=1)
+        assert np.max(np.abs(rn.download() - want)) <= 1e-12 * np.max(want), ("residual norms", dt)
+        print(f"[full size] Davidson products {np.dtype(dt).name}: exact", flush=True)
+        del dV, dW, y, r
