@@ -447,6 +447,20 @@ int dft_scf_result_json(const dft_grid* g, int natoms, const double* nuclei, con
             jenum(s, cfg->precond, {"diagonal", "cnn", "kinetic", "kinetic_chebyshev", "kinetic_exact"});
         s += ',', jkey(s, "mixer"), jenum(s, cfg->mixer, {"linear", "pulay"});
         s += ',', jkey(s, "smearing"), jnum(s, cfg->smearing);
+        // the remaining knobs of dft_scf_config, so that the document re-runs as it ran
+        s += ',', jkey(s, "maxIterations"), jint(s, cfg->max_iterations);
+        s += ',', jkey(s, "maxSubspace"), jint(s, cfg->max_subspace);
+        s += ',', jkey(s, "poissonBatch"), jint(s, cfg->poisson_batch);
+        s += ',', jkey(s, "poissonTolerance"), jnum(s, cfg->poisson_tol);
+        s += ',', jkey(s, "davidsonMaxIterations"), jint(s, cfg->davidson_max_iter);
+        s += ',', jkey(s, "refineIterations"), jint(s, cfg->refine_iterations);
+        s += ',', jkey(s, "precondSweeps"), jint(s, cfg->precond_sweeps);
+        s += ',', jkey(s, "precondShift"), jnum(s, cfg->precond_shift);
+        s += ',', jkey(s, "precondShiftScale"), jnum(s, cfg->precond_shift_scale);
+        s += ',', jkey(s, "mixDepth"), jint(s, cfg->mix_depth);
+        s += ',', jkey(s, "mixBeta"), jnum(s, cfg->mix_beta);
+        s += ',', jkey(s, "mixKerker"), jnum(s, cfg->mix_kerker);
+        s += ',', jkey(s, "orthoGram"), jenum(s, cfg->ortho_gram, {"default", "f32", "split_bf16"});
         s += "},\"scf\":{\"energy\":{";
         // EnergyComponents (DFTData.h:47-54)
         static const char* const ek[6] = {"kinetic", "external", "hartree", "exchangeCorrelation", "nuclearRepulsion", "total"};
