@@ -521,12 +521,20 @@ int og_integrate_density(long n, const double* rho, double dx, double dy, double
     return 0;
 }
 
+/* TotalEnergyMolecule.cpp:13-199 over one list of orbitals, continuing the
+ * accumulator e: the SCF's list is the alpha orbitals followed by the beta
+ * ones (MolecularSCFIteration.cpp:146-161), summed with one accumulator. */
+static double kinetic_accumulate(const og_grid* g, int norb, const double* psi, const int* electrons, double e);
+
 double og_kinetic_energy(const og_grid* g, int norb, const double* psi, const int* electrons) {
+    return kinetic_accumulate(g, norb, psi, electrons, 0.0);
+}
+
+static double kinetic_accumulate(const og_grid* g, int norb, const double* psi, const int* electrons, double e) {
     /* TotalEnergyMolecule.cpp:13-199: sum over existing neighbours of
      * (psi_nb - psi)/h^2 (no ghost term), -1/2 applied, dV-weighted. */
     const double dx2 = g->dx * g->dx, dy2 = g->dy * g->dy, dz2 = g->dz * g->dz, dv = volume_element(g);
     const long n = gsize(g), sy = g->nx, sz = g->nx * g->ny;
-    double e = 0.0;
     for (int o = 0; o < norb; ++o) {
         if (electrons[o] <= 0) continue;
         const double* p = psi + (size_t)o * (size_t)n;
@@ -1031,7 +1039,7 @@ int og_scf(const og_grid* g, int natoms, const double* nuc, const og_scf_config*
         } else og_poisson(g, orho, ovh, 10000, 1.0e-8, 1, &sw, &md);
         /* :195-205 -> TotalEnergyMolecule.cpp:237-333 */
         double E[6];
-        E[0] = og_kinetic_energy(g, na, pa, occa) + og_kinetic_energy(g, nb, pb, occb);
+        E[0] = kinetic_accumulate(g, nb, pb, occb, kinetic_accumulate(g, na, pa, occa, 0.0));
         E[1] = og_external_energy(g, orho, vnuc);
         E[2] = og_hartree_energy(g, orho, ovh);
         E[3] = og_xc_energy(g, cfg->functional, ora, orb);
@@ -1143,7 +1151,7 @@ int og_scf_step(const og_grid* g, int natoms, const double* nuc, int functional,
     og_spin_density(n, nb, pb, occ, orb);
     for (long i = 0; i < n; ++i) orho[i] = ora[i] + orb[i];
     og_poisson(g, orho, ovh, 10000, 1.0e-8, 1, &sw, &md);
-    E[0] = og_kinetic_energy(g, na, pa, occ) + og_kinetic_energy(g, nb, pb, occ);
+    E[0] = kinetic_accumulate(g, nb, pb, occ, kinetic_accumulate(g, na, pa, occ, 0.0));
     E[1] = og_external_energy(g, orho, vnuc);
     E[2] = og_hartree_energy(g, orho, ovh);
     E[3] = og_xc_energy(g, functional, ora, orb);

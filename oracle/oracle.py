@@ -354,6 +354,22 @@ class Reference:
                                      _d(psi), _d(eig)))
         return eig, psi
 
+    def scf_step(self, g, nuclei, functional, multiplicity, ra, rb):
+        """executeMolecularSCFIteration on given densities, the same outputs as Oracle.scf_step."""
+        nuc = _f64(nuclei).reshape(-1, 4)
+        n = g.size
+        out = {k: np.empty(n) for k in ("vh", "vxa", "vxb", "ora", "orb", "ovh")}
+        ne = int(round(nuc[:, 0].sum()))
+        eig = np.empty(ne + 2)
+        E = np.empty(6)
+        dd, ks = C.c_double(), C.c_double()
+        self._chk(self.L.rf_scf_step(*self._g(g), C.c_int(nuc.shape[0]), _d(nuc), C.c_int(functional),
+                                     C.c_int(multiplicity), _d(_f64(ra, n)), _d(_f64(rb, n)), _d(out["vh"]),
+                                     _d(out["vxa"]), _d(out["vxb"]), _d(out["ora"]), _d(out["orb"]), _d(out["ovh"]),
+                                     _d(eig), _d(E), C.byref(dd), C.byref(ks)))
+        out.update(eigenvalues=eig[:ne], energy=E, dRho=dd.value, ks=ks.value)
+        return out
+
     def scf(self, g, nuclei, functional=1, charge=0, multiplicity=1):
         nuc = _f64(nuclei).reshape(-1, 4)
         out = np.zeros(11)

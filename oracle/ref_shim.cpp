@@ -11,6 +11,7 @@
 #include "ExchangeCorrelationMolecule.h"
 #include "HartreePotentialMolecule.h"
 #include "KohnShamHamiltonianMolecule.h"
+#include "MolecularSCFIteration.h"
 #include "Molecule.h"
 #include "NuclearPotential.h"
 #include "NuclearRepulsion.h"
@@ -215,6 +216,47 @@ int rf_davidson(const long* n, const double* b, const double* v, int norb, const
             }
         } catch (...) {
             std::cerr.rdbuf(old);
+            throw;
+        }
+    });
+}
+
+// executeMolecularSCFIteration on given spin densities, orbitals started from
+// the seed vectors (no previous orbitals).  Outputs as og_scf_step: vh, vxa,
+// vxb, ora, orb, ovh (N each), eig (alpha then beta), E[6], dd (effective
+// density difference), ks (max KS residual).
+int rf_scf_step(const long* n, const double* b, int natoms, const double* nuc, int id, int mult, const double* ra,
+                const double* rb, double* vh, double* vxa, double* vxb, double* ora, double* orb, double* ovh,
+                double* eig, double* E, double* dd, double* ks) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        Molecule m = make_molecule(natoms, nuc, 0);
+        const int ne = m.getElectronCount(), sd = mult - 1;
+        const int na = (ne + sd) / 2, nbeta = (ne - sd) / 2;
+        const size_t N = g.getSize();
+        const std::vector<double> vn = NuclearPotential::calculate(m, g);
+        std::ostringstream sink1, sink2;
+        std::streambuf* o1 = std::cout.rdbuf(sink1.rdbuf());
+        std::streambuf* o2 = std::cerr.rdbuf(sink2.rdbuf());
+        try {
+            const MolecularSCFIterationResult r = executeMolecularSCFIteration(
+                g, m, functional(id), vn, vec(ra, N), vec(rb, N), na, nbeta, mult == 1, {}, {});
+            std::cout.rdbuf(o1);
+            std::cerr.rdbuf(o2);
+            const std::vector<double>* fields[6] = {&r.hartreePotential, &r.alphaXCPotential, &r.betaXCPotential,
+                                                    &r.outputAlphaDensity, &r.outputBetaDensity, &r.outputHartreePotential};
+            double* outs[6] = {vh, vxa, vxb, ora, orb, ovh};
+            for (int f = 0; f < 6; ++f) std::memcpy(outs[f], fields[f]->data(), N * sizeof(double));
+            for (int o = 0; o < na; ++o) eig[o] = r.alphaOrbitals[o].eigenvalue;
+            for (int o = 0; o < nbeta; ++o) eig[na + o] = r.betaOrbitals[o].eigenvalue;
+            const double e[6] = {r.energy.kinetic, r.energy.external, r.energy.hartree, r.energy.exchangeCorrelation,
+                                 r.energy.nuclearRepulsion, r.energy.total};
+            std::memcpy(E, e, sizeof e);
+            *dd = r.effectiveDensityDifference;
+            *ks = r.residual;
+        } catch (...) {
+            std::cout.rdbuf(o1);
+            std::cerr.rdbuf(o2);
             throw;
         }
     });

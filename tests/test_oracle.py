@@ -397,3 +397,35 @@ def test_full_size_closed_forms_on_small_grids(oracle):
     ref, _, md = oracle.poisson(g, rho, v0=np.zeros(g.size), max_sweeps=400000, tol=1e-15)
     assert md < 1e-15
     np.testing.assert_allclose(ref, v, rtol=0, atol=1e-11 * np.max(np.abs(v)))
+
+
+@needs_ref
+@pytest.mark.parametrize("case", ["h2o_pz81", "n_pbe_quartet"])
+def test_scf_step_against_live_reference(oracle, case):
+    """The oracle's SCF step (the checker of the GPU step) against the
+    reference's executeMolecularSCFIteration (MolecularSCFIteration.cpp:12-252)
+    on the same input densities: fields, eigenvalues, energies, dRho and the
+    KS residual, bit for bit."""
+    R = Reference()
+    if case == "h2o_pz81":
+        g = Grid(9, 9, 9, -8.1, 7.9, -8.1, 7.9, -8.1, 7.9)
+        nuc, fid, mult = [[8, 0, 0, 0], [1, 1.430, 0, 1.108], [1, -1.430, 0, 1.108]], 1, 1
+        rng = np.random.default_rng(3)
+        ra = np.abs(rng.standard_normal(g.size)) * 0.05
+        rb = ra.copy()
+    else:
+        g = Grid.centered(14, 0.6)
+        nuc, fid, mult = [[7, 0.05, -0.02, 0.03]], 2, 4
+        xs, ys, zs = g.coords()
+        Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+        r = np.sqrt((X - 0.05) ** 2 + (Y + 0.02) ** 2 + (Z - 0.03) ** 2).ravel()
+        dv = g.dx * g.dy * g.dz
+        ra = np.exp(-1.6 * r) + 0.3 * np.exp(-0.7 * r)
+        ra *= 5.0 / (ra.sum() * dv)
+        rb = np.exp(-2.2 * r)
+        rb *= 2.0 / (rb.sum() * dv)
+    mine = oracle.scf_step(g, nuc, fid, mult, ra, rb)
+    ref = R.scf_step(g, nuc, fid, mult, ra, rb)
+    for k in ("vh", "vxa", "vxb", "ora", "orb", "ovh", "eigenvalues", "energy"):
+        np.testing.assert_array_equal(mine[k], ref[k], err_msg=k)
+    assert mine["dRho"] == ref["dRho"] and mine["ks"] == ref["ks"]
