@@ -64,3 +64,22 @@ def test_poisson_direct_thin_grids(dev, oracle, shape):
     v = dev.array(np.full(og.size, 9.0))  # every point is written
     dev.poisson_direct(g, dev.array(rho), v)
     assert rel(v.download(), ref) <= 1e-12, shape
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 2, 2), (2, 3, 9), (5, 4, 3)])
+@pytest.mark.parametrize("fid", [1, 2])
+def test_xc_thin_grids(dev, oracle, shape, fid):
+    """One-sided derivatives at every face and the radius-2 divergence of the
+    GGA flux on grids two and three points thick, open shell."""
+    og, g = _grids(shape)
+    print(f"[thin] XC {fid} {shape}", flush=True)
+    rng = np.random.default_rng(og.size + fid)
+    na = 0.2 + rng.random(og.size)
+    nb = 0.1 + 0.5 * rng.random(og.size)
+    ref_a = oracle.xc_potential(og, fid, na, nb, 0)
+    ref_b = oracle.xc_potential(og, fid, na, nb, 1)
+    ref_e = oracle.xc_energy(og, fid, na, nb)
+    va, vb = dev.array(shape=(og.size,)), dev.array(shape=(og.size,))
+    e = dev.xc(g, fid, dev.array(na), dev.array(nb), va, vb)
+    assert rel(va.download(), ref_a) <= 1e-12 and rel(vb.download(), ref_b) <= 1e-12, shape
+    assert abs(e - ref_e) <= 1e-12 * abs(ref_e)
