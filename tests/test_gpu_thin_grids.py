@@ -83,3 +83,17 @@ def test_xc_thin_grids(dev, oracle, shape, fid):
     e = dev.xc(g, fid, dev.array(na), dev.array(nb), va, vb)
     assert rel(va.download(), ref_a) <= 1e-12 and rel(vb.download(), ref_b) <= 1e-12, shape
     assert abs(e - ref_e) <= 1e-12 * abs(ref_e)
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (3, 3, 3), (3, 5, 4), (40, 3, 3), (2, 9, 9), (70, 3, 4)])
+def test_poisson_reference_rule_thin_grids(dev, oracle, shape):
+    """The reference's Jacobi (zero start, max|dV| < 1e-8, cap 10000) on thin
+    grids: the same potential and the same sweep count as the oracle."""
+    og, g = _grids(shape)
+    print(f"[thin] Jacobi {shape}", flush=True)
+    rho = np.abs(np.random.default_rng(3).standard_normal(og.size))
+    ref, rsw, _ = oracle.poisson(og, rho)
+    v = dev.array(np.full(og.size, 9.0))
+    sw, _ = dev.poisson(g, dev.array(rho), v)
+    assert sw == rsw, (shape, sw, rsw)
+    assert rel(v.download(), ref) <= 1e-12, shape
