@@ -74,3 +74,104 @@ def test_single_shard_trains_on_everything(tmp_path):
     _shard(cp.shard_path(tmp_path, 1), 6, 4)
     plan = cp.shard_plan(tmp_path)
     assert (plan["train_count"], plan["val_count"]) == (6, 0)
+
+
+class _FakeTrainer:
+    """Stands in for CnnTrainer (a device patch set): records calls, holds patches in host memory."""
+    made = []
+
+    def __init__(self, patch=32, capacity=1, seed=0, channels=16):
+        self.patch, self.capacity, self.seed = patch, capacity, seed
+        self.xin = np.zeros((0, 2, patch ** 3), np.float32)
+        self.tgt = np.zeros((0, patch ** 3), np.float32)
+        self.calls = []
+        _FakeTrainer.made.append(self)
+
+    def harvest(self, grid, atoms, **kw):
+        n = min(self.capacity, 3 + len(atoms))
+        rng = np.random.default_rng(kw["seed"])
+        self.add_patches(rng.standard_normal((n, 2, self.patch ** 3)), rng.standard_normal((n, self.patch ** 3)))
+        self.calls.append(("harvest", kw["seed"], kw["nstates"]))
+        return n
+
+    def add_patches(self, xin, tgt):
+        self.xin = np.concatenate([self.xin, np.asarray(xin, np.float32).reshape(-1, 2, self.patch ** 3)])
+        self.tgt = np.concatenate([self.tgt, np.asarray(tgt, np.float32).reshape(-1, self.patch ** 3)])
+        assert len(self.tgt) <= self.capacity
+
+    def count(self):
+        return len(self.tgt)
+
+    def save_patches(self, path, h=0.0):
+        dft_amd.save_patch_shard(path, self.xin, self.tgt, h=h)
+
+    def load_patches(self, path):
+        xin, tgt, hdr = dft_amd.load_patch_shard(path)
+        self.add_patches(xin, tgt)
+        return hdr
+
+    def train(self, steps, batch=8, lr=1e-3, train_count=0):
+        self.calls.append(("train", steps, train_count))
+        return np.full(steps, 0.5)
+
+    def eval(self, first, count):
+        self.calls.append(("eval", first, count))
+        return 0.25
+
+    def model(self):
+        return dft_amd.CnnModel(np.zeros(dft_amd.cnn_param_count(), np.float32))
+
+    def close(self):
+        pass
+
+
+def test_harvest_resumes_and_train_holds_out_the_last_molecule(tmp_path, monkeypatch):
+    monkeypatch.setattr(cp, "CnnTrainer", _FakeTrainer)
+    monkeypatch.setattr(cp, "Grid", type("G", (), {"centered": staticmethod(lambda n, h: (n, h))}))
+    _FakeTrainer.made.clear()
+    m = cp.harvest(tmp_path, [3, 1], n=16, h=0.5, patch=4, target_patches=20)
+    assert [r["seed"] for r in m["molecules"]] == [3, 1] and m["per_molecule"] == 10
+    assert all(r["patches"] == min(10, 3 + len(training_molecule(r["seed"]))) for r in m["molecules"])
+    # nstates: the occupied count plus 4
+    assert [c[2] for t in _FakeTrainer.made for c in t.calls] == \
+        [int(round(np.array(training_molecule(s))[:, 0].sum())) // 2 + 4 for s in (3, 1)]
+    # a second call harvests only the new seed
+    _FakeTrainer.made.clear()
+    m = cp.harvest(tmp_path, [1, 3, 2], n=16, h=0.5, patch=4, target_patches=30)
+    assert [c[1] for t in _FakeTrainer.made for c in t.calls] == [2]
+    assert sorted(r["seed"] for r in m["molecules"]) == [1, 2, 3]
+    assert cp.patch_shard_info(cp.shard_path(tmp_path, 1))["h"] == 0.5
+    # training: every shard loaded, the last one (seed 3) held out
+    plan = cp.shard_plan(tmp_path)
+    _FakeTrainer.made.clear()
+    _, log = cp.train(tmp_path, steps=1200, batch=2, chunk=500)
+    t = _FakeTrainer.made[0]
+    assert t.count() == plan["total"]
+    assert [c for c in t.calls if c[0] == "train"] == [("train", 500, plan["train_count"])] * 2 + \
+        [("train", 200, plan["train_count"])]
+    assert ("eval", plan["train_count"], plan["val_count"]) in t.calls
+    assert [r["step"] for r in log["curve"]] == [500, 1000, 1200]
+    assert (tmp_path / "model.npz").exists() and (tmp_path / "train_log.json").exists()
+
+
+def test_evaluate_compares_diagonal_and_learned(tmp_path, monkeypatch):
+    dft_amd.CnnModel(np.zeros(dft_amd.cnn_param_count(), np.float32)).save(tmp_path / "model.npz")
+    runs = []
+
+    def fake_scf(grid, atoms, precond=None, cnn=None, **kw):
+        learned = precond == dft_amd.PRECOND_CNN
+        assert (cnn is not None) == learned
+        runs.append(precond)
+        return dict(converged=True, happly_columns=300 if learned else 400, davidson_iterations=10,
+                    seconds_total=2.0 if learned else 3.0, cnn_applied=7 if learned else 0, cnn_fallbacks=1,
+                    total=-1.0 + (1e-9 if learned else 0.0))
+
+    monkeypatch.setattr(cp, "scf_run", fake_scf)
+    monkeypatch.setattr(cp, "scf_result_json", lambda g, a, r: "{}")
+    monkeypatch.setattr(cp, "Grid", type("G", (), {"centered": staticmethod(lambda n, h: (n, h))}))
+    s = cp.evaluate(tmp_path, [1001, 1002], n=16, h=0.5)
+    assert runs == [dft_amd.PRECOND_DIAG, dft_amd.PRECOND_CNN] * 2
+    assert s["mean_happly_gain"] == pytest.approx(4 / 3)
+    assert s["molecules"][0]["happly"] == [400, 300] and s["molecules"][0]["time_gain"] == pytest.approx(1.5)
+    assert s["max_abs_energy_difference"] == pytest.approx(1e-9)
+    assert (tmp_path / "eval.json").exists() and (tmp_path / "eval_seed01001_learned.json").exists()
