@@ -157,6 +157,8 @@ uint64_t file_size(std::FILE* f) {
 void field_header_from_bytes(const unsigned char* h, dft_field_header* out, const char* path) {
     if (std::memcmp(h, kFieldMagic, 8) != 0) throw std::invalid_argument(std::string("not a field file: ") + path);
     if (get<uint32_t>(h, 8) != kFieldHeader) throw std::invalid_argument(std::string("field file: unknown header size: ") + path);
+    if (get<uint32_t>(h, kFieldHeader - 4) != crc32_update(0, h, kFieldHeader - 4))
+        throw std::runtime_error(std::string("field file: header checksum mismatch: ") + path);
     dft_field_header r{};
     r.dtype = (int)get<uint32_t>(h, 12);
     r.grid.nx = get<int64_t>(h, 16);
@@ -209,6 +211,8 @@ PatchHeader read_patch_header(std::FILE* f, const char* path) {
         throw std::invalid_argument(std::string("patch shard: no header: ") + path);
     if (std::memcmp(h, kPatchMagic, 8) != 0) throw std::invalid_argument(std::string("not a patch shard: ") + path);
     if (get<uint32_t>(h, 8) != kPatchHeader) throw std::invalid_argument(std::string("patch shard: unknown header size: ") + path);
+    if (get<uint32_t>(h, kPatchHeader - 4) != crc32_update(0, h, kPatchHeader - 4))
+        throw std::runtime_error(std::string("patch shard: header checksum mismatch: ") + path);
     PatchHeader r;
     r.pub.channels = (int)get<uint32_t>(h, 12);
     r.pub.patch = get<int64_t>(h, 16);
@@ -299,6 +303,7 @@ int dft_field_write(const char* path, const dft_grid* g, int dtype, int64_t ncol
         put<uint64_t>(h, 120, bytes);
         put<uint32_t>(h, 128, crc32_update(0, static_cast<const unsigned char*>(data), (size_t)bytes));
         std::memcpy(h + 132, nm, std::strlen(nm));
+        put<uint32_t>(h, kFieldHeader - 4, crc32_update(0, h, kFieldHeader - 4));
         write_atomically(path, h, kFieldHeader, {{data, bytes}}, "dft_field_write");
     });
 }
@@ -351,6 +356,7 @@ int dft_patch_shard_write(const char* path, int64_t patch, int64_t count, int ta
         put<uint32_t>(hd, 52, count ? crc32_update(0, reinterpret_cast<const unsigned char*>(tgt), (size_t)tb) : 0u);
         put<int32_t>(hd, 56, (int32_t)target_scale);
         put<double>(hd, 64, h);
+        put<uint32_t>(hd, kPatchHeader - 4, crc32_update(0, hd, kPatchHeader - 4));
         write_atomically(path, hd, kPatchHeader, {{in, ib}, {tgt, tb}}, "dft_patch_shard_write");
     });
 }

@@ -29,6 +29,7 @@
  *     128 uint32   CRC-32 of the data (zlib's polynomial, reflected 0xEDB88320)
  *     132 char[64] name, NUL-padded (e.g. "rho_alpha", "psi_alpha")
  *     196 zeros
+ *     252 uint32   CRC-32 of header bytes 0..251
  *
  *   patch shard, 128-byte header
  *     0   char[8]  "DFTPAT01"
@@ -44,12 +45,14 @@
  *     60  uint32   0
  *     64  double   grid spacing h of the harvest (0 if unknown)
  *     72  zeros
+ *     124 uint32   CRC-32 of header bytes 0..123
  *
  * Status codes as elsewhere: 0 ok; -1 (DFT_EINVAL) for a bad argument, a file
  * that cannot be opened or written, a wrong magic or version, sizes that do
  * not agree with the header, or a destination too small; -2 (DFT_ENUMERIC)
- * for a short read or a checksum mismatch.  dft_core_last_error() holds the
- * message.  Functions are reentrant apart from that message.
+ * for a short read or a checksum mismatch (of the header or the data).
+ * dft_core_last_error() holds the message.  Functions are reentrant apart
+ * from that message.
  */
 #ifndef DFT_IO_H
 #define DFT_IO_H
@@ -77,7 +80,7 @@ uint32_t dft_crc32(uint32_t crc, const void* data, size_t n);
 
 /* Write ncols columns of N = nx ny nz values (host memory, device layout). */
 int dft_field_write(const char* path, const dft_grid* g, int dtype, int64_t ncols, const char* name, const void* data);
-/* Read the header only (the data's checksum is not verified). */
+/* Read the header only (its own checksum is verified, the data's is not). */
 int dft_field_read_header(const char* path, dft_field_header* hdr);
 /* Read header and data into data[0 .. capacity_bytes); verifies sizes and the checksum. */
 int dft_field_read(const char* path, dft_field_header* hdr, void* data, uint64_t capacity_bytes);
@@ -95,8 +98,9 @@ typedef struct {
 int dft_patch_shard_write(const char* path, int64_t patch, int64_t count, int target_scale, double h, const float* in,
                           const float* tgt);
 int dft_patch_shard_read_header(const char* path, dft_patch_shard_header* hdr);
-/* Read patches [first, first + count) of a shard (checksums verified when the
- * whole shard is read, i.e. first = 0 and count = the shard's count). */
+/* Read patches [first, first + count) of a shard.  The header's checksum is
+ * always verified, the data's when the whole shard is read (first = 0 and
+ * count = the shard's count). */
 int dft_patch_shard_read(const char* path, int64_t first, int64_t count, float* in, float* tgt);
 
 /* The result document (JSON, UTF-8, no trailing newline) of a run of

@@ -48,7 +48,8 @@ def _parse_field_header(raw):
     nbytes, = struct.unpack_from("<Q", raw, 120)
     crc, = struct.unpack_from("<I", raw, 128)
     name = raw[132:196].split(b"\0", 1)[0].decode()
-    assert raw[196:256] == b"\0" * 60
+    assert raw[196:252] == b"\0" * 56
+    assert struct.unpack_from("<I", raw, 252)[0] == zlib.crc32(raw[:252])  # the header's own checksum
     return dict(hb=hb, dtype=dtype, n=(nx, ny, nz), ncols=ncols, geo=geo, nbytes=nbytes, crc=crc, name=name)
 
 
@@ -100,8 +101,16 @@ def test_field_rejects_corruption(tmp_path, grid):
     with pytest.raises(dft_amd.DftError, match="not a field file") as e:
         dft_amd.field_info(tmp_path / "magic.dftf")
     assert e.value.code == -1
+    # a changed header byte fails the header's checksum, wherever it is (here: the grid's xmax)
     bad = bytearray(raw)
-    struct.pack_into("<q", bad, 40, 2)  # ncols 2 with one column's bytes
+    bad[57] ^= 0x01
+    (tmp_path / "hdr.dftf").write_bytes(bytes(bad))
+    with pytest.raises(dft_amd.DftError, match="header checksum") as e:
+        dft_amd.field_info(tmp_path / "hdr.dftf")
+    assert e.value.code == -2
+    bad = bytearray(raw)
+    struct.pack_into("<q", bad, 40, 2)  # ncols 2 with one column's bytes, header checksum redone
+    struct.pack_into("<I", bad, 252, zlib.crc32(bytes(bad[:252])))
     (tmp_path / "shape.dftf").write_bytes(bytes(bad))
     with pytest.raises(dft_amd.DftError, match="disagrees") as e:
         dft_amd.field_info(tmp_path / "shape.dftf")
@@ -155,6 +164,7 @@ def test_patch_shard_round_trip_and_layout(tmp_path):
     ts, = struct.unpack_from("<i", raw, 56)
     h, = struct.unpack_from("<d", raw, 64)
     assert (hb, ch, p, n, ib, tb, ts, h) == (PATCH_HDR, 2, P, count, xin.nbytes, tgt.nbytes, 1, 0.2)
+    assert raw[72:124] == b"\0" * 52 and struct.unpack_from("<I", raw, 124)[0] == zlib.crc32(raw[:124])
     assert raw[PATCH_HDR:PATCH_HDR + ib] == xin.tobytes() and raw[PATCH_HDR + ib:] == tgt.tobytes()
     assert (ci, ct) == (zlib.crc32(xin.tobytes()), zlib.crc32(tgt.tobytes()))
     a, b, hdr = dft_amd.load_patch_shard(path)
