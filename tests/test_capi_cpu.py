@@ -53,6 +53,19 @@ def test_io_library_exports_header(libs):
     assert not missing, missing
 
 
+@pytest.mark.parametrize("header", ["dft_hip.h", "dft_core.h", "dft_io.h"])
+def test_c_abi_headers_are_plain_c(header):
+    """The C-ABI headers compile as strict C99 (no C++ in them), so C and FFI callers can include them."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), "-x", "c", "-"], input=f'#include "{header}"\n',
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+
+
 def test_reference_cpp_interface_exported(libs):
     """The drop-in C++ names of the reference (mangled) are defined in libdft_core.so."""
     import subprocess
