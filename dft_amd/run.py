@@ -12,6 +12,7 @@ The reference builds its molecules in main.cpp and prints its results; it
 reads no run description.
 
   python -m dft_amd.run run.json --out result.json [--fields]
+  python -m dft_amd.run result.json --out rerun.json --check 1e-10   # repeat a result and compare
 """
 import argparse
 import json
@@ -96,16 +97,46 @@ def run(doc, want_density=False, verbose=False):
     return grid, nuclei, scf_run(grid, nuclei, want_density=want_density, verbose=verbose, **kw)
 
 
+ENERGY_KEYS = ("kinetic", "external", "hartree", "exchangeCorrelation", "nuclearRepulsion", "total")
+
+
+def compare_energies(recorded, result, tol):
+    """Largest |difference| (Ha) between a result document's energies and a new run's, and whether it is within
+    tol; the step counts must agree too."""
+    old = recorded["scf"]["energy"]
+    # a term recorded as null (non-finite) is never reproduced
+    diffs = [abs((float("nan") if old[k] is None else float(old[k])) - float(result["energy"][i]))
+             for i, k in enumerate(ENERGY_KEYS)]
+    diff = max(diffs) if all(d == d for d in diffs) else float("inf")
+    same_steps = int(recorded["scf"]["iterations"]) == int(result["iterations"])
+    return diff, diff <= tol and same_steps
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m dft_amd.run", description=__doc__.split("\n\n")[0])
     ap.add_argument("description", help="run description or result document (JSON)")
     ap.add_argument("--out", required=True, help="result document to write")
     ap.add_argument("--fields", action="store_true", help="also write the output spin densities as field files")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--check", type=float, metavar="TOL", default=None,
+                    help="the description is a result document: fail unless every energy term is reproduced "
+                         "within TOL Ha in the same number of SCF steps")
     a = ap.parse_args(argv)
-    grid, nuclei, r = run(a.description, want_density=a.fields, verbose=a.verbose)
+    recorded = None
+    if a.check is not None:
+        with open(a.description) as f:
+            recorded = json.load(f)
+        if "scf" not in recorded:
+            ap.error("--check needs a result document (with its \"scf\" section)")
+    grid, nuclei, r = run(recorded if recorded is not None else a.description, want_density=a.fields,
+                          verbose=a.verbose)
     save_scf_result(a.out, grid, nuclei, r, fields=a.fields)
     print(f"E = {r['total']:.10f} Ha, {r['iterations']} SCF steps, converged {r['converged']}", file=sys.stderr)
+    if recorded is not None:
+        diff, ok = compare_energies(recorded, r, a.check)
+        print(f"check: largest energy difference {diff:.3e} Ha, steps {r['iterations']} vs "
+              f"{recorded['scf']['iterations']}: {'reproduced' if ok else 'NOT reproduced'}", file=sys.stderr)
+        return 0 if ok else 3
     return 0 if r["converged"] else 1
 
 

@@ -78,3 +78,33 @@ def test_description_errors():
         R.parse_run({**base, "config": {"functional": "B3LYP"}})
     with pytest.raises(ValueError, match="disagrees"):
         R.parse_run({"grid": {**base["grid"], "spacing": [0.5, 0.5, 0.4]}, "nuclei": []})
+
+
+def test_check_mode_compares_a_rerun(tmp_path, monkeypatch):
+    """--check re-runs a result document and passes only when every energy term and the step count repeat."""
+    grid = dft_amd.Grid.make(7, 7, 7, -8.1, 7.9, -8.1, 7.9, -8.1, 7.9)
+    cfg, _ = _config()
+    rep = dft_amd.ScfReport()
+    energies = [1.5, -4.0, 2.0, -0.75, 0.7142857142857143, -0.5357142857142857]
+    for i, e in enumerate(energies):
+        rep.energy[i] = e
+    rep.iterations = 12
+    nuc = np.array([[1.0, -0.7, 0.0, 0.0], [1.0, 0.7, 0.0, 0.0]])
+    res = dict(config=cfg, report=rep, eigenvalues=np.array([-0.3, -0.3]), n_alpha=1, n_beta=1)
+    path = tmp_path / "result.json"
+    path.write_text(dft_amd.scf_result_json(grid, nuc, res))
+
+    def fake_run(doc, want_density=False, verbose=False):
+        g, n, kw = R.parse_run(doc)
+        assert kw["smearing"] == cfg.smearing  # the knobs of the document drive the re-run
+        shift = fake_run.shift
+        out = dict(energy=[e + shift for e in energies], total=energies[5] + shift, iterations=12, converged=True,
+                   config=cfg, report=rep, eigenvalues=np.array([-0.3, -0.3]), n_alpha=1, n_beta=1)
+        return g, n, out
+
+    monkeypatch.setattr(R, "run", fake_run)
+    fake_run.shift = 0.0
+    assert R.main([str(path), "--out", str(tmp_path / "rerun.json"), "--check", "1e-12"]) == 0
+    fake_run.shift = 1e-9
+    assert R.main([str(path), "--out", str(tmp_path / "rerun.json"), "--check", "1e-10"]) == 3
+    assert R.main([str(path), "--out", str(tmp_path / "rerun.json"), "--check", "1e-8"]) == 0
