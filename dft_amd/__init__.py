@@ -5,6 +5,13 @@ include/dft_hip.h) and libdft_core.so (the C++ layer keeping the reference's
 solver interfaces, include/dft_molecule.h and include/dft_core.h).  This
 module loads them through ctypes for tests and benchmarks; it raises if the
 native libraries are missing -- there is no Python or CPU fallback.
+
+Also here: the file formats of include/dft_io.h (save_field / load_field,
+save_patch_shard / load_patch_shard, scf_result_json / save_scf_result);
+dft_amd.run (an SCF from a run description or a result document);
+dft_amd.cnn_pipeline (harvest, training and evaluation of the learned
+preconditioner on the seeded molecule set); dft_amd.geometry (the
+workloads' molecules).
 """
 import ctypes as C
 import os
