@@ -434,6 +434,8 @@ INIT_BARE, INIT_HYDROGENIC = 0, 1
 PRECOND_DIAG, PRECOND_CNN, PRECOND_KINETIC, PRECOND_KINETIC_CHEB, PRECOND_KINETIC_EXACT = 0, 1, 2, 3, 4
 POISSON_JACOBI, POISSON_CHEBYSHEV, POISSON_DIRECT = 0, 1, 2
 MIX_LINEAR, MIX_PULAY = 0, 1
+XC_FLUX_REFERENCE, XC_FLUX_CONSISTENT = 0, 1
+XC_FLAG_FLUX_CONSISTENT = 0x100  # OR-ed into the functional id of Device.xc
 
 
 class ScfConfig(C.Structure):
@@ -448,7 +450,7 @@ class ScfConfig(C.Structure):
                 ("cnn", C.c_void_p), ("verbose", C.c_int), ("precond_sweeps", C.c_int),
                 ("precond_shift", C.c_double), ("mixer", C.c_int), ("mix_depth", C.c_int), ("mix_beta", C.c_double),
                 ("smearing", C.c_double), ("ortho_gram", C.c_int), ("mix_kerker", C.c_double),
-                ("precond_shift_scale", C.c_double)]
+                ("precond_shift_scale", C.c_double), ("xc_flux", C.c_int)]
 
 
 class ScfReport(C.Structure):
@@ -474,16 +476,17 @@ def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=S
             init_guess=INIT_BARE, davidson_tol=1e-7, davidson_max_iter=0, refine_iterations=0,
             precond=PRECOND_DIAG, cnn=None, verbose=False, want_density=False, precond_sweeps=0, precond_shift=0.0,
             mixer=MIX_LINEAR, mix_depth=0, mix_beta=0.0, smearing=0.0, ortho_gram=0, mix_kerker=0.0,
-            precond_shift_scale=0.0):
+            precond_shift_scale=0.0, xc_flux=XC_FLUX_REFERENCE):
     """Run the device-resident SCF; returns a dict of results.  smearing: Fermi-Dirac kT in Ha (0: whole
-    orbitals, the reference's rule); with smearing the total is the free energy E - TS."""
+    orbitals, the reference's rule); with smearing the total is the free energy E - TS.  xc_flux:
+    XC_FLUX_REFERENCE (the reference's GGA potential) or XC_FLUX_CONSISTENT (the derivative of E_xc)."""
     L = core_lib()
     nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 4)
     cfg = ScfConfig(functional, charge, multiplicity, precision, nstates, max_subspace, max_iterations,
                     poisson_fixed, poisson_batch, poisson_method, poisson_tol, stop_rule, energy_tol, init_guess, davidson_tol,
                     davidson_max_iter, refine_iterations, precond, _model_ptr(cnn), 1 if verbose else 0,
                     precond_sweeps, precond_shift, mixer, mix_depth, mix_beta, smearing, ortho_gram, mix_kerker,
-                    precond_shift_scale)
+                    precond_shift_scale, xc_flux)
     rep = ScfReport()
     ne = int(round(nuc[:, 0].sum())) - charge
     eig = np.zeros(ne + 2)

@@ -40,7 +40,7 @@ __device__ inline T gradmag(const T* __restrict__ f, int64_t p, int i, int j, in
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_xc_point(GridDev g, int functional, const T* __restrict__ na,
+__global__ __launch_bounds__(256) void k_xc_point(GridDev g, int functional, int consistent, const T* __restrict__ na,
                                                   const T* __restrict__ nb, T* __restrict__ va, T* __restrict__ vb,
                                                   T* __restrict__ ca, T* __restrict__ cb, double dv,
                                                   double* __restrict__ partials) {
@@ -50,11 +50,13 @@ __global__ __launch_bounds__(256) void k_xc_point(GridDev g, int functional, con
     for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < n; p += (int64_t)gridDim.x * 256) {
         const T a = na[p], b = nb[p];
         const T tot = a + b;
-        T oa = T(0), ob = T(0), oca = T(0), ocb = T(0);
+        T oa = T(0), ob = T(0), oca = T(0), ocb = T(0), ga_keep = T(1), gb_keep = T(1);
         if (!(tot <= T(xc::kRhoFloor))) {
             const int i = (int)(p % g.nx), j = (int)((p / g.nx) % g.ny), k = (int)(p / plane);
             const T ga = functional == DFT_XC_PBE96 ? gradmag(na, p, i, j, k, g) : T(0);
             const T gb = functional == DFT_XC_PBE96 ? gradmag(nb, p, i, j, k, g) : T(0);
+            ga_keep = ga;
+            gb_keep = gb;
             // FP64 evaluation for small densities and for extreme relative gradients
             // (the GGA's t^6 terms overflow FP32 beyond |grad n| / n ~ 1e6)
             if (sizeof(T) == 4 && ((double)tot < xc::kF32Floor || (double)ga + (double)gb > 1.0e4 * (double)tot)) {
@@ -69,7 +71,14 @@ __global__ __launch_bounds__(256) void k_xc_point(GridDev g, int functional, con
         }
         va[p] = oa;
         if (vb) vb[p] = ob;
-        if (ca) { ca[p] = oca; cb[p] = ocb; }
+        if (ca) {
+            if (consistent) {  // flux c grad n / |grad n| (0 where the gradient vanishes)
+                oca = ga_keep > T(0) ? oca / ga_keep : T(0);
+                ocb = gb_keep > T(0) ? ocb / gb_keep : T(0);
+            }
+            ca[p] = oca;
+            cb[p] = ocb;
+        }
     }
     e = block_sum(e, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = e;
@@ -134,6 +143,8 @@ template <typename T>
 int xc_t(dft_ctx* c, const dft_grid* gg, int functional, const T* na, const T* nb, T* va, T* vb, double* exc) {
     const int64_t n = grid_points(gg);
     GridDev g{(int)gg->nx, (int)gg->ny, (int)gg->nz, gg->dx, gg->dy, gg->dz};
+    const int consistent = (functional & DFT_XC_FLUX_CONSISTENT) ? 1 : 0;
+    functional &= ~DFT_XC_FLUX_CONSISTENT;
     const bool gga = functional == DFT_XC_PBE96;
     T* ca = nullptr;
     T* cb = nullptr;
@@ -145,7 +156,7 @@ int xc_t(dft_ctx* c, const dft_grid* gg, int functional, const T* na, const T* n
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8 * (int64_t)c->num_cus);
     if (int rc = ensure_partials(c, (size_t)blocks + 1)) return rc;
     const double dv = gg->dx * gg->dy * gg->dz;
-    k_xc_point<T><<<blocks, 256, 0, c->stream>>>(g, functional, na, nb, va, vb, ca, cb, dv, c->partials);
+    k_xc_point<T><<<blocks, 256, 0, c->stream>>>(g, functional, consistent, na, nb, va, vb, ca, cb, dv, c->partials);
     DFT_CHECK_LAUNCH(c);
     if (gga) {
         k_xc_div<T><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(g, na, nb, ca, cb, va, vb);
@@ -163,7 +174,8 @@ int dft_xc(dft_ctx* c, const dft_grid* g, int prec, int functional, const void* 
            double* exc_host) {
     if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, valid_grid(g) && na && nb && va, "dft_xc: bad arguments");
-    DFT_REQUIRE(c, functional == DFT_XC_PZ81 || functional == DFT_XC_PBE96, "dft_xc: unknown functional");
+    const int base = functional & ~DFT_XC_FLUX_CONSISTENT;
+    DFT_REQUIRE(c, base == DFT_XC_PZ81 || base == DFT_XC_PBE96, "dft_xc: unknown functional");
     if (prec == DFT_F64)
         return xc_t<double>(c, g, functional, (const double*)na, (const double*)nb, (double*)va, (double*)vb, exc_host);
     if (prec == DFT_F32)

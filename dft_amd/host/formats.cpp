@@ -467,6 +467,7 @@ int dft_scf_result_json(const dft_grid* g, int natoms, const double* nuclei, con
         s += ',', jkey(s, "mixBeta"), jnum(s, cfg->mix_beta);
         s += ',', jkey(s, "mixKerker"), jnum(s, cfg->mix_kerker);
         s += ',', jkey(s, "orthoGram"), jenum(s, cfg->ortho_gram, {"default", "f32", "split_bf16"});
+        s += ',', jkey(s, "xcFlux"), jenum(s, cfg->xc_flux, {"reference", "consistent"});
         s += "},\"scf\":{\"energy\":{";
         // EnergyComponents (DFTData.h:47-54)
         static const char* const ek[6] = {"kinetic", "external", "hartree", "exchangeCorrelation", "nuclearRepulsion", "total"};

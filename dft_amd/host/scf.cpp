@@ -301,6 +301,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     if (natoms == 0) throw std::invalid_argument("La molecula debe contener al menos un nucleo.");
     if (cfg_.multiplicity < 1) throw std::invalid_argument("La multiplicidad electronica debe ser al menos 1.");
     if (cfg_.functional != DFT_XC_PZ81 && cfg_.functional != DFT_XC_PBE96) throw std::invalid_argument("unknown functional");
+    if (cfg_.xc_flux != DFT_XC_FLUX_REFERENCE && cfg_.xc_flux != DFT_XC_FLUX_CONSISTENT_SCF)
+        throw std::invalid_argument("xc_flux must be DFT_XC_FLUX_REFERENCE or DFT_XC_FLUX_CONSISTENT_SCF");
     int zsum = 0;
     for (int a = 0; a < natoms; ++a) zsum += (int)nuc_[4 * a];
     const int ne = zsum - cfg_.charge;
@@ -525,8 +527,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         poisson(f, f.rho.get(), f.vh.get());
         tick(t_pois, t);
         // :68-90
-        check(c_, dft_xc(c_, &g_, p, cfg_.functional, f.ra.get(), f.rb.get(), f.vxa.get(), closed ? nullptr : f.vxb.get(),
-                         nullptr), "dft_xc");
+        check(c_, dft_xc(c_, &g_, p, cfg_.functional | (cfg_.xc_flux ? DFT_XC_FLUX_CONSISTENT : 0), f.ra.get(),
+                         f.rb.get(), f.vxa.get(), closed ? nullptr : f.vxb.get(), nullptr), "dft_xc");
         tick(t_xc, t);
         // a non-finite point anywhere in V_eff would poison every Davidson product: name the field
         {

@@ -35,6 +35,7 @@ _NAMED = {
                                    "kinetic_chebyshev": PRECOND_KINETIC_CHEB, "kinetic_exact": PRECOND_KINETIC_EXACT}),
     "mixer": ("mixer", {"linear": MIX_LINEAR, "pulay": MIX_PULAY}),
     "orthoGram": ("ortho_gram", {"default": 0, "f32": 1, "split_bf16": 2}),
+    "xcFlux": ("xc_flux", {"reference": 0, "consistent": 1}),
 }
 _INTS = {"multiplicity": "multiplicity", "nstates": "nstates", "poissonFixedSweeps": "poisson_fixed",
          "maxIterations": "max_iterations", "maxSubspace": "max_subspace", "poissonBatch": "poisson_batch",

@@ -71,7 +71,12 @@ typedef struct {
     double precond_shift_scale; /* DFT_PRECOND_KINETIC_EXACT: per-state shifts sigma_j = max(precond_shift,
                                    -scale theta_j); 0: scale 0.35 with precond_shift 0.5 by default; < 0: one shift
                                    (precond_shift, default 2.0) for every state */
+    int xc_flux;           /* DFT_XC_FLUX_REFERENCE (0): the GGA potential's flux c_sigma grad n_sigma as
+                              ExchangeCorrelationMolecule.cpp:264-274 builds it; DFT_XC_FLUX_CONSISTENT_SCF (1):
+                              c_sigma grad n_sigma / |grad n_sigma|, the derivative of the E_xc the SCF reports */
 } dft_scf_config;
+
+enum { DFT_XC_FLUX_REFERENCE = 0, DFT_XC_FLUX_CONSISTENT_SCF = 1 };
 
 typedef struct {
     int iterations;            /* SCF steps taken (all phases) */

@@ -38,6 +38,12 @@ enum { DFT_F64 = 0, DFT_F32 = 1 };
 enum { DFT_GRAM_F64 = 0, DFT_GRAM_F32 = 1, DFT_GRAM_SPLIT_BF16 = 2 };
 /* XCFunctional subclasses (XCFunctional.h:15-21, PZ81.h, PBE96.h). */
 enum { DFT_XC_PZ81 = 1, DFT_XC_PBE96 = 2 };
+/* Flag OR-ed into a functional id for dft_xc: the GGA flux is
+ * c_sigma grad n_sigma / |grad n_sigma| (the functional derivative of E_xc)
+ * instead of the reference's c_sigma grad n_sigma
+ * (ExchangeCorrelationMolecule.cpp:264-274 with c_sigma = d f / d|grad n_sigma|,
+ * PBE96.cpp:928-957).  No counterpart in the reference; PZ81 is unaffected. */
+enum { DFT_XC_FLUX_CONSISTENT = 0x100 };
 /* Reductions (dft_reduce). */
 enum { DFT_RED_SUM = 0, DFT_RED_DOT = 1, DFT_RED_MAXABS = 2, DFT_RED_DIFF2 = 3 };
 
@@ -159,7 +165,8 @@ int dft_hartree_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* rh
 
 /* ---- (c) exchange-correlation -------------------------------------------
  * Replaces calculateMolecularSpinExchangeCorrelationPotential
- * (ExchangeCorrelationMolecule.cpp:576-747) for both spins in one call:
+ * (ExchangeCorrelationMolecule.cpp:576-747) for both spins in one call
+ * (functional may carry DFT_XC_FLUX_CONSISTENT):
  * va/vb receive v_alpha/v_beta (vb may be NULL to compute alpha only, the
  * reference's closed-shell call).  exc_host (may be NULL) receives the
  * energy of calculateMolecularExchangeCorrelationEnergy (:749-842). */

@@ -12,9 +12,15 @@
  *   - solveMolecularOrbitals solves the block of states at once (block
  *     Davidson with MFMA Gram products) instead of one orbital at a time; it
  *     returns the same eigenpairs to the residual tolerance 1e-7.
- *   - XCFunctional subclasses are dispatched to the device by their
- *     deviceFunctional() id; a user subclass without an id is rejected with
+ *   - XC functionals are dispatched to the device by their dynamic type
+ *     (PZ81, PBE96); any other XCFunctional subclass is rejected with
  *     std::invalid_argument.
+ * The types mirror the reference's DFTData.h, DFTConstants.h, XCInput.h and
+ * XCFunctional.h member for member.  This header stands in for the
+ * reference's headers; it is not meant to be included beside them.  Inside
+ * the reference's own tree the hot-path functions come from
+ * dft_amd/host/ks_path.cpp compiled against the reference's headers
+ * (-DDFT_REFERENCE_HEADERS), beneath its unchanged SCF driver.
  */
 #pragma once
 
@@ -30,11 +36,15 @@ namespace DFTConstants {
 constexpr double PI = 3.141592653589793238462643383279502884;
 constexpr double EPS = 1.0e-14;
 constexpr double RHO_FLOOR = 1.0e-20;
+constexpr std::size_t GRID_POINTS = 2000;
+constexpr double RMAX = 30.0;
 constexpr double MIXING = 0.25;
 constexpr int MAX_SCF_ITERATIONS = 250;
 constexpr double DENSITY_TOL = 1.0e-9;
 constexpr double ENERGY_TOL = 1.0e-11;
 constexpr double KS_RESIDUAL_TOL = 1.0e-7;
+constexpr double H_EXACT = -0.5;
+constexpr double ANGULAR_MOMENTUM_EPS = 1.0e-14;
 constexpr double EXCHANGE_COEFFICIENT = 0.7385587663820223;
 }  // namespace DFTConstants
 
@@ -88,14 +98,42 @@ class Molecule {
     int charge_;
 };
 
-// ---- DFTData.h:7-91 (molecular part) -----------------------------------------
+// ---- DFTData.h:7-91 ------------------------------------------------------------
 enum class SpinChannel { Alpha, Beta };
+
+struct ElectronicState {
+    int n = 0;
+    int l = 0;
+    int alphaElectrons = 0;
+    int betaElectrons = 0;
+};
+
+struct AtomicConfiguration {
+    int Z = 0;
+    std::string symbol;
+    std::vector<ElectronicState> states;
+};
+
+struct AtomicOrbital {
+    int n = 0;
+    int l = 0;
+    SpinChannel spin = SpinChannel::Alpha;
+    int electrons = 0;
+    double eigenvalue = 0.0;
+    std::vector<double> u;
+};
 
 struct MolecularOrbital {
     SpinChannel spin = SpinChannel::Alpha;
     int electrons = 0;
     double eigenvalue = 0.0;
     std::vector<double> psi;
+};
+
+struct TridiagonalMatrix {
+    std::vector<double> lower;
+    std::vector<double> diagonal;
+    std::vector<double> upper;
 };
 
 struct EnergyComponents {
@@ -108,15 +146,27 @@ struct EnergyComponents {
 };
 
 struct SCFResult {
+    std::vector<AtomicOrbital> orbitals;  // radial solver only; the molecular path leaves it empty
     std::vector<MolecularOrbital> molecularOrbitals;
-    std::vector<double> alphaDensity, betaDensity, density;
-    std::vector<double> alphaEffectivePotential, betaEffectivePotential, effectivePotential;
+    std::vector<double> alphaDensity;
+    std::vector<double> betaDensity;
+    std::vector<double> density;
+    std::vector<double> alphaEffectivePotential;
+    std::vector<double> betaEffectivePotential;
+    std::vector<double> effectivePotential;
     EnergyComponents energy;
     double densityDifference = 0.0;
     double energyDifference = 0.0;
     double maxKSResidual = 0.0;
     int iterations = 0;
     bool converged = false;
+};
+
+struct AtomicResult {
+    int Z = 0;
+    std::string symbol;
+    int electrons = 0;
+    SCFResult scf;
 };
 
 struct MolecularResult {
@@ -145,20 +195,16 @@ class XCFunctional {
    public:
     virtual ~XCFunctional() = default;
     virtual XCResult evaluate(const XCInput& input) const = 0;
-    // Device functional id (1 PZ81, 2 PBE96); 0 = not available on the device.
-    virtual int deviceFunctional() const { return 0; }
 };
 
 class PZ81 final : public XCFunctional {
    public:
     XCResult evaluate(const XCInput& input) const override;
-    int deviceFunctional() const override { return 1; }
 };
 
 class PBE96 final : public XCFunctional {
    public:
     XCResult evaluate(const XCInput& input) const override;
-    int deviceFunctional() const override { return 2; }
 };
 
 // ---- grid kernels (KohnShamHamiltonianMolecule.h, HartreePotentialMolecule.h,

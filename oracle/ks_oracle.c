@@ -431,6 +431,17 @@ static double divergence(const og_grid* g, const double* coef, const double* f, 
 
 int og_xc_potential(const og_grid* g, int functional, const double* na, const double* nb, int spin, double* v) {
     /* ExchangeCorrelationMolecule.cpp:576-747 */
+    return og_xc_potential_flux(g, functional, na, nb, spin, 0, v);
+}
+
+int og_xc_potential_flux(const og_grid* g, int functional, const double* na, const double* nb, int spin,
+                         int consistent, double* v) {
+    /* ExchangeCorrelationMolecule.cpp:576-747.  consistent = 0 is the
+     * reference: the flux is c_sigma * (finite-difference dn_sigma/dx_a),
+     * with c_sigma = d f / d|grad n_sigma| (PBE96.cpp:928-957).
+     * consistent = 1 divides c_sigma by |grad n_sigma| first (0 where the
+     * gradient vanishes), so the flux is c_sigma grad n_sigma / |grad n_sigma|
+     * = d f / d(grad n_sigma), the gradient term of d E_xc / d n_sigma. */
     if (spin != 0 && spin != 1) return fail(-1, "El canal de spin debe ser 0 (alpha) o 1 (beta).");
     if (functional != 1 && functional != 2) return fail(-1, "unknown functional");
     const long n = gsize(g);
@@ -446,6 +457,10 @@ int og_xc_potential(const og_grid* g, int functional, const double* na, const do
                 og_xc_point(functional, in, out);
                 dd[c] = spin == 0 ? out[1] : out[2];
                 cf[c] = spin == 0 ? out[3] : out[4];
+                if (consistent) {
+                    const double gm = in[2 + spin];
+                    cf[c] = gm > 0.0 ? cf[c] / gm : 0.0;
+                }
             }
     const double* f = spin == 0 ? na : nb;
     for (long k = 0; k < g->nz; ++k)

@@ -56,6 +56,11 @@ int og_xc_point(int functional, const double in[4], double out[5]);
 /* ExchangeCorrelationMolecule.cpp:576-747 */
 int og_xc_potential(const og_grid* g, int functional, const double* na,
                     const double* nb, int spin, double* v);
+/* As og_xc_potential; consistent = 1 replaces the flux c_sigma grad n_sigma
+ * by c_sigma grad n_sigma / |grad n_sigma|, the functional derivative's flux
+ * (build option; no counterpart in the reference). */
+int og_xc_potential_flux(const og_grid* g, int functional, const double* na,
+                         const double* nb, int spin, int consistent, double* v);
 /* ExchangeCorrelationMolecule.cpp:749-842 */
 double og_xc_energy(const og_grid* g, int functional, const double* na, const double* nb);
 

@@ -138,10 +138,11 @@ class Oracle:
             self._chk(self.L.og_xc_point(C.c_int(fid), _d(inp[p]), _d(out[p])))
         return out
 
-    def xc_potential(self, g, fid, na, nb, spin):
+    def xc_potential(self, g, fid, na, nb, spin, consistent=False):
         out = np.empty(g.size)
-        self._chk(self.L.og_xc_potential(C.byref(self._g(g)), C.c_int(fid), _d(_f64(na, g.size)),
-                                         _d(_f64(nb, g.size)), C.c_int(spin), _d(out)))
+        self._chk(self.L.og_xc_potential_flux(C.byref(self._g(g)), C.c_int(fid), _d(_f64(na, g.size)),
+                                              _d(_f64(nb, g.size)), C.c_int(spin), C.c_int(int(consistent)),
+                                              _d(out)))
         return out
 
     def xc_energy(self, g, fid, na, nb):

@@ -429,3 +429,34 @@ def test_scf_step_against_live_reference(oracle, case):
     for k in ("vh", "vxa", "vxb", "ora", "orb", "ovh", "eigenvalues", "energy"):
         np.testing.assert_array_equal(mine[k], ref[k], err_msg=k)
     assert mine["dRho"] == ref["dRho"] and mine["ks"] == ref["ks"]
+
+
+@pytest.mark.parametrize("fid", [1, 2])
+def test_xc_potential_flux_against_energy_derivative(oracle, fid):
+    """sum v_alpha dn dV against the central difference of E_xc along dn (a smooth bump inside the box).
+
+    The consistent flux (c grad n / |grad n|) gives the derivative of the discrete E_xc; the reference's
+    flux (c grad n, ExchangeCorrelationMolecule.cpp:264-274 with c = d f / d|grad n|, PBE96.cpp:928-957)
+    misses it by several per cent for PBE96.  PZ81 has no flux, so both agree there."""
+    g = Grid(26, 24, 22, -3.1, 3.15, -2.9, 2.85, -2.6, 2.65)
+    xs, ys, zs = g.coords()
+    z, y, x = np.meshgrid(zs, ys, xs, indexing="ij")
+    r = np.sqrt(x ** 2 + y ** 2 + z ** 2).ravel()
+    na = (4.0 / np.pi) * np.exp(-4.0 * r) + 0.02 * np.exp(-0.5 * r ** 2)
+    nb = 0.8 * na
+    dn = 0.05 * np.exp(-(((x - 0.4) ** 2 + (y + 0.3) ** 2 + (z - 0.2) ** 2) / 0.5).ravel())
+    dV = g.dx * g.dy * g.dz
+    e = 1e-4
+    fd = (oracle.xc_energy(g, fid, na + e * dn, nb) - oracle.xc_energy(g, fid, na - e * dn, nb)) / (2 * e)
+    lin_ref = np.sum(oracle.xc_potential(g, fid, na, nb, 0) * dn) * dV
+    lin_con = np.sum(oracle.xc_potential(g, fid, na, nb, 0, consistent=True) * dn) * dV
+    # 1e-6: the central difference's own O(e^2) error and the one-sided faces (the bump's far tail)
+    assert abs(lin_con - fd) < 1e-6 * abs(fd)
+    if fid == 1:
+        assert lin_ref == lin_con
+    else:
+        assert abs(lin_ref - fd) > 0.02 * abs(fd)
+        # beta channel, perturbing n_beta
+        fdb = (oracle.xc_energy(g, fid, na, nb + e * dn) - oracle.xc_energy(g, fid, na, nb - e * dn)) / (2 * e)
+        lin_b = np.sum(oracle.xc_potential(g, fid, na, nb, 1, consistent=True) * dn) * dV
+        assert abs(lin_b - fdb) < 1e-6 * abs(fdb)
