@@ -55,9 +55,10 @@ WORKLOADS = {
                n=96, h=0.2, functional=2, nstates=8, poisson_fixed=200, init=0),
     "c3": dict(desc="benzene, 160^3, h=0.2, LSDA/PZ81, Davidson block 29, Poisson at the Jacobi fixed point",
                atoms=benzene(), n=160, h=0.2, functional=1, nstates=29, poisson_fixed=400, poisson="direct", init=1),
-    "c5": dict(desc="40-atom random cluster (seed 5, radius 7 bohr, 150-170 electrons), 256^3, h=0.18, PBE96, block 96, "
-                    "Poisson at the Jacobi fixed point",
-               atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, poisson="direct", init=1),
+    "c5": dict(desc="40-atom random cluster (seed 5, radius 7 bohr, 150-170 electrons), 256^3, h=0.18, PBE96 with the "
+                    "consistent GGA flux, block 96, Poisson at the Jacobi fixed point, Fermi-Dirac kT 0.01 Ha, Pulay depth 10",
+               atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, poisson="direct", init=1,
+               scf=dict(smearing=0.01, mixer=1, mix_depth=10, xc_flux=1)),
 }
 
 
@@ -83,8 +84,8 @@ def run_scf(dft_amd, grid, atoms, w, precision, max_iterations=200, verbose=Fals
     return dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=precision, nstates=w["nstates"],
                            max_iterations=max_iterations, poisson_batch=64, **poisson_args(dft_amd, w),
                            stop_rule=dft_amd.STOP_DE, energy_tol=1e-5, init_guess=w["init"], davidson_tol=1e-7,
-                           refine_iterations=30 if precision == dft_amd.SCF_MIXED else 0, verbose=verbose,
-                           precond=precond, cnn=cnn)
+                           refine_iterations=40 if precision == dft_amd.SCF_MIXED else 0, verbose=verbose,
+                           precond=precond, cnn=cnn, **w.get("scf", {}))
 
 
 def time_hpsi(dft_amd, grid, nb, prec, reps=20):
@@ -282,6 +283,8 @@ def main():
                     help="only the timed mixed SCF (profiling): no FP64 runs, no CPU baseline")
     ap.add_argument("--profile-scf", type=int, default=0,
                     help="only run this many SCF steps of each precision (profiling)")
+    ap.add_argument("--profile-precision", default="both", choices=["both", "fp64", "mixed"],
+                    help="with --profile-scf: which precisions to run")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="with --hpsi-only: also run 5 device copies of this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -311,13 +314,18 @@ def main():
         print(json.dumps({"hpsi_f32_ms": ms, "hpsi_f32_GBps": by / ms / 1e6, "hpsi_f64_ms": ms64,
                           "hpsi_f64_GBps": by64 / ms64 / 1e6}))
         return
+    pc = {"diag": dft_amd.PRECOND_DIAG, "kinetic": dft_amd.PRECOND_KINETIC, "kinetic-cheb": dft_amd.PRECOND_KINETIC_CHEB,
+          "kinetic-exact": dft_amd.PRECOND_KINETIC_EXACT, "cnn": dft_amd.PRECOND_CNN}[a.precond]
+    model = dft_amd.CnnModel.load(a.cnn_model) if a.precond == "cnn" else None
     if a.profile_scf:
-        for prec in (dft_amd.SCF_FP64, dft_amd.SCF_MIXED):
+        precs = {"both": (dft_amd.SCF_FP64, dft_amd.SCF_MIXED), "fp64": (dft_amd.SCF_FP64,),
+                 "mixed": (dft_amd.SCF_MIXED,)}[a.profile_precision]
+        for prec in precs:
             t = time.perf_counter()
             r = dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=prec, nstates=w["nstates"],
                                 max_iterations=a.profile_scf, poisson_batch=64, **poisson_args(dft_amd, w),
                                 stop_rule=dft_amd.STOP_DE, energy_tol=1e-5, init_guess=w["init"], davidson_tol=1e-7,
-                                refine_iterations=0)
+                                refine_iterations=0, precond=pc, cnn=model, verbose=a.verbose, **w.get("scf", {}))
             print(json.dumps({"precision": prec, "steps": r["iterations"], "seconds": time.perf_counter() - t,
                               "davidson_iterations": r["davidson_iterations"], "happly": r["happly_columns"],
                               "phase": {k: r[f"seconds_{k}"] for k in ("poisson", "xc", "davidson", "other")}}),
@@ -329,14 +337,12 @@ def main():
         return
     ne = int(round(atoms[:, 0].sum()))
     nocc = ne // 2
-    pc = {"diag": dft_amd.PRECOND_DIAG, "kinetic": dft_amd.PRECOND_KINETIC, "kinetic-cheb": dft_amd.PRECOND_KINETIC_CHEB,
-          "kinetic-exact": dft_amd.PRECOND_KINETIC_EXACT, "cnn": dft_amd.PRECOND_CNN}[a.precond]
-    model = dft_amd.CnnModel.load(a.cnn_model) if a.precond == "cnn" else None
     print(f"[bench] workload {a.workload}: {w['desc']}; {len(atoms)} atoms, {ne} electrons", file=sys.stderr, flush=True)
     for i in range(a.warmup):  # loads the code objects (rocBLAS kernels included) the timed run uses
         dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=dft_amd.SCF_MIXED, nstates=w["nstates"],
                         max_iterations=2, poisson_batch=64, **poisson_args(dft_amd, w), stop_rule=dft_amd.STOP_DE,
-                        energy_tol=1e-5, init_guess=w["init"], refine_iterations=1, precond=pc, cnn=model)
+                        energy_tol=1e-5, init_guess=w["init"], refine_iterations=1, precond=pc, cnn=model,
+                        **w.get("scf", {}))
         print(f"[bench] warmup {i + 1} done", file=sys.stderr, flush=True)
     if a.mixed_only:
         t = time.perf_counter()

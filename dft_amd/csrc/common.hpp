@@ -36,6 +36,10 @@ struct dft_ctx {
     int dst_shape[3] = {0, 0, 0};
     // rocBLAS handle for the dense FP64 subspace solvers (created on first use)
     void* blas = nullptr;
+    // three-piece bf16 split of a coefficient block for the bf16 MFMA update
+    // (linalg.hip), grown on demand
+    void* coef3 = nullptr;
+    size_t coef3_cap = 0;
     int* dev_info = nullptr;
 };
 
@@ -77,6 +81,9 @@ int rocblas_for(dft_ctx* c, void** handle);
 int ensure_scratch(dft_ctx* c, size_t bytes);
 // Sum of n FP64 device values already written to ctx->partials -> host.
 int finish_sum(dft_ctx* c, int64_t n, double* out_host);
+// The same for n values at `partials` (device); writes partials[n].
+int finish_sum_at(dft_ctx* c, double* partials, int64_t n, double* out_host);
+
 
 inline size_t prec_bytes(int prec) { return prec == DFT_F64 ? 8 : 4; }
 inline int64_t grid_points(const dft_grid* g) { return g->nx * g->ny * g->nz; }

@@ -58,16 +58,19 @@ __global__ __launch_bounds__(256) void k_final_sum(const double* __restrict__ p,
     }
 }
 
-static int finish(dft_ctx* c, int64_t n, double* out_host, int op_max) {
-    k_final_sum<<<1, 256, 0, c->stream>>>(c->partials, n, c->partials + n, op_max);
+static int finish(dft_ctx* c, double* partials, int64_t n, double* out_host, int op_max) {
+    k_final_sum<<<1, 256, 0, c->stream>>>(partials, n, partials + n, op_max);
     DFT_CHECK_LAUNCH(c);
-    DFT_HIP(c, hipMemcpyAsync(c->host_scalar, c->partials + n, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DFT_HIP(c, hipMemcpyAsync(c->host_scalar, partials + n, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     DFT_HIP(c, hipStreamSynchronize(c->stream));
     *out_host = c->host_scalar[0];
     return 0;
 }
 
-int finish_sum(dft_ctx* c, int64_t n, double* out_host) { return finish(c, n, out_host, 0); }
+int finish_sum(dft_ctx* c, int64_t n, double* out_host) { return finish(c, c->partials, n, out_host, 0); }
+int finish_sum_at(dft_ctx* c, double* partials, int64_t n, double* out_host) {
+    return finish(c, partials, n, out_host, 0);
+}
 
 }  // namespace dft
 
@@ -164,7 +167,14 @@ int dft_ctx_create(int device, dft_ctx** out) {
         return DFT_EHIP;
     }
     c->sweep_max_cap = 4096;
-    if (ensure_partials(c, 8192) != 0) { delete c; return DFT_EHIP; }
+    if (ensure_partials(c, 8192) != 0) {
+        (void)hipFree(c->partials);
+        (void)hipFree(c->sweep_max);
+        (void)hipHostFree(c->host_scalar);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return DFT_EHIP;
+    }
     *out = c;
     return DFT_OK;
 }
@@ -175,6 +185,7 @@ int dft_ctx_destroy(dft_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->partials) (void)hipFree(c->partials);
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->coef3) (void)hipFree(c->coef3);
     if (c->sweep_max) (void)hipFree(c->sweep_max);
     if (c->dst_mats) (void)hipFree(c->dst_mats);
     if (c->host_scalar) (void)hipHostFree(c->host_scalar);
@@ -301,7 +312,7 @@ int dft_reduce(dft_ctx* c, int op, int prec, const void* a, const void* b, int64
     }
 #undef LAUNCH
     DFT_CHECK_LAUNCH(c);
-    return finish(c, blocks, out, op == DFT_RED_MAXABS);
+    return finish(c, c->partials, blocks, out, op == DFT_RED_MAXABS);
 }
 
 int dft_axpby(dft_ctx* c, int prec, int64_t n, double alpha, const void* x, double beta, void* y) {
