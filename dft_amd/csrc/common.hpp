@@ -36,6 +36,9 @@ struct dft_ctx {
     int dst_shape[3] = {0, 0, 0};
     // rocBLAS handle for the dense FP64 subspace solvers (created on first use)
     void* blas = nullptr;
+    // device address, on this context's device, of the fixed energy-partials
+    // array of the allocation-free XC entry point (xc.hip), set at creation
+    double* xc_scratch = nullptr;
     // three-piece bf16 split of a coefficient block for the bf16 MFMA update
     // (linalg.hip), grown on demand
     void* coef3 = nullptr;
@@ -83,7 +86,9 @@ int ensure_scratch(dft_ctx* c, size_t bytes);
 int finish_sum(dft_ctx* c, int64_t n, double* out_host);
 // The same for n values at `partials` (device); writes partials[n].
 int finish_sum_at(dft_ctx* c, double* partials, int64_t n, double* out_host);
-
+// Look up the allocation-free XC entry point's device array on c's device
+// (the device must be current); sets c->xc_scratch (xc.hip).
+int xc_scratch_attach(dft_ctx* c);
 
 inline size_t prec_bytes(int prec) { return prec == DFT_F64 ? 8 : 4; }
 inline int64_t grid_points(const dft_grid* g) { return g->nx * g->ny * g->nz; }

@@ -167,7 +167,7 @@ int dft_ctx_create(int device, dft_ctx** out) {
         return DFT_EHIP;
     }
     c->sweep_max_cap = 4096;
-    if (ensure_partials(c, 8192) != 0) {
+    if (ensure_partials(c, 8192) != 0 || dft::xc_scratch_attach(c) != 0) {
         (void)hipFree(c->partials);
         (void)hipFree(c->sweep_max);
         (void)hipHostFree(c->host_scalar);

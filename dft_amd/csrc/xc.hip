@@ -28,12 +28,23 @@
 #include "xc_functionals.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 using namespace dft;
+
+// The allocation-free entry point's scratch: the workgroup energy partials
+// of one XC call and their sum, in a fixed device array (one copy per GPU,
+// held until the process ends).  Host code uses only the address that
+// hipGetSymbolAddress returns for it on each device (dft_ctx::xc_scratch).
+constexpr int64_t kXcScratchDoubles = 16384;
+static __device__ __align__(256) double scratch[kXcScratchDoubles];
 
 namespace {
 
 constexpr double kGgaF32Floor = 1.0e-10;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
 
 struct GridDev {
     int nx, ny, nz;
@@ -362,6 +373,22 @@ int xc_args(dft_ctx* c, const dft_grid* g, int prec, int functional, const void*
 
 }  // namespace
 
+namespace dft {
+int xc_scratch_attach(dft_ctx* c) {
+    if (c->device < 0 || c->device >= kMaxDevices) return set_err(c, DFT_EINVAL, "xc scratch: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(c, DFT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[c->device] = static_cast<double*>(p);
+    }
+    c->xc_scratch = g_scratch_ptr[c->device];
+    return DFT_OK;
+}
+}  // namespace dft
+
 extern "C" {
 
 int dft_xc(dft_ctx* c, const dft_grid* g, int prec, int functional, const void* na, const void* nb, void* va, void* vb,
@@ -377,6 +404,26 @@ int dft_xc(dft_ctx* c, const dft_grid* g, int prec, int functional, const void* 
                                                        (float*)va, (float*)vb, c->partials, &parts);
     if (rc) return rc;
     if (exc_host) return finish_sum(c, parts, exc_host);
+    return DFT_OK;
+}
+
+int dft_xc_noalloc(dft_ctx* c, const dft_grid* g, int prec, int functional, const void* na, const void* nb, void* va,
+                   void* vb, double* exc_host) {
+    if (!c) return DFT_EINVAL;
+    if (int rc = xc_args(c, g, prec, functional, na, nb, va)) return rc;
+    DFT_REQUIRE(c, c->xc_scratch, "dft_xc_noalloc: the device scratch array was not attached");
+    const int64_t need = xc_partials_needed(g, prec, functional, c->num_cus);
+    if (need + 1 > kXcScratchDoubles)
+        return set_err(c, DFT_EINVAL,
+                       "dft_xc_noalloc: this grid needs " + std::to_string(need + 1) +
+                           " energy partials, beyond the fixed scratch of " + std::to_string(kXcScratchDoubles));
+    int64_t parts = 0;
+    const int rc = prec == DFT_F64 ? xc_launch<double>(c, g, functional, (const double*)na, (const double*)nb,
+                                                        (double*)va, (double*)vb, c->xc_scratch, &parts)
+                                    : xc_launch<float>(c, g, functional, (const float*)na, (const float*)nb,
+                                                       (float*)va, (float*)vb, c->xc_scratch, &parts);
+    if (rc) return rc;
+    if (exc_host) return finish_sum_at(c, c->xc_scratch, parts, exc_host);
     return DFT_OK;
 }
 

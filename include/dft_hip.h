@@ -172,6 +172,17 @@ int dft_hartree_energy(dft_ctx* ctx, const dft_grid* g, int prec, const void* rh
  * energy of calculateMolecularExchangeCorrelationEnergy (:749-842). */
 int dft_xc(dft_ctx* ctx, const dft_grid* g, int prec, int functional, const void* na, const void* nb, void* va,
            void* vb, double* exc_host);
+/* dft_xc without any allocation: the same kernels and results, with the
+ * energy partials in a fixed device array of 16384 doubles (one per GPU,
+ * held until the process ends) instead of context memory.  No counterpart in
+ * the reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Grids whose
+ * launch needs more than 16383 workgroup partials are rejected with
+ * DFT_EINVAL.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (several contexts or threads) race; each call overwrites
+ * every partial it reads, so nothing needs clearing between calls. */
+int dft_xc_noalloc(dft_ctx* ctx, const dft_grid* g, int prec, int functional, const void* na, const void* nb,
+                   void* va, void* vb, double* exc_host);
 /* Pointwise XCFunctional::evaluate for npts inputs {na, nb, |grad na|,
  * |grad nb|} (FP64, device) -> {eps, v_a, v_b, c_a, c_b}. */
 int dft_xc_points(dft_ctx* ctx, int functional, const double* in_dev, int64_t npts, double* out_dev);
