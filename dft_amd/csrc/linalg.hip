@@ -257,6 +257,224 @@ int update_mfma(dft_ctx* c, int64_t n, int m1, int m, int k, T alpha, const T* A
     return DFT_OK;
 }
 
+
+// ---- FP32 block update on the bf16 matrix cores (three-piece split) -------
+// Y = [A | A2] (alpha C) + beta Y for FP32 blocks: N grid points, m inner
+// rows (m1 from A, the rest from A2), k <= 96 output columns.  Every FP32
+// operand value x is split as x1 + x2 + x3 with x1 = bf16(x), x2 =
+// bf16(x - x1), x3 = bf16(x - x1 - x2) (24 significand bits: exact for
+// normal values), and a product keeps the six terms of order <= 2^-18,
+// a3b1 + a1b3 + a2b2 + a2b1 + a1b2 + a1b1 (smallest first; the dropped
+// terms are ~2^-27 relative), each a v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation: FP32-level products at the bf16 MFMA rate, so the pass is
+// bound by its HBM stream (N (m + k) floats) rather than by FP32 MFMA.
+// Layout: MFMA operand A = (alpha C)^T, row j = output column, K = inner
+// row; operand B = the grid columns, K = inner row, column = grid point.
+// Lane l of a wave: points p0 + 4 (l % 16) .. + 3 (one 16-byte load per
+// inner row), K slice 8 (l / 16) .. + 7 of each 32-row step; result
+// register r of tile jt = output column jt 16 + 4 (l / 16) + r at those
+// four points (one 16-byte store).  The split coefficients are prepared once
+// per call in operand order ([32-row step][jt][piece][lane][8]) and staged
+// through LDS 64 inner rows at a time for the 8 waves (512 points) of a
+// workgroup.  Y may alias columns of A or A2: a wave reads every inner row
+// at its points before it writes there.  NORM: part[j * gridDim.x + b] =
+// the workgroup's sum of squares of stored column j.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kS3Waves = 8, kS3Chunk = 64;
+
+__global__ __launch_bounds__(256) void k_split_coef(int m, int k, int njt, int nks, double alpha,
+                                                    const double* __restrict__ cm, int64_t ldc,
+                                                    bf16x8* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;  // one (step, jt, lane) triple
+    const int64_t total = (int64_t)nks * njt * 64;
+    if (q >= total) return;
+    const int lane = (int)(q % 64), jt = (int)((q / 64) % njt), ks = (int)(q / (64 * njt));
+    const int j = jt * 16 + (lane & 15), ib = ks * 32 + 8 * (lane >> 4);
+    bf16x8 p1, p2, p3;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = ib + e;
+        const double x = (i < m && j < k) ? alpha * cm[(int64_t)j * ldc + i] : 0.0;
+        const __bf16 h1 = (__bf16)(float)x;
+        const double r1 = x - (double)(float)h1;
+        const __bf16 h2 = (__bf16)(float)r1;
+        const double r2 = r1 - (double)(float)h2;
+        p1[e] = h1;
+        p2[e] = h2;
+        p3[e] = (__bf16)(float)r2;
+    }
+    bf16x8* o = out + (((int64_t)ks * njt + jt) * 3) * 64 + lane;
+    o[0] = p1;
+    o[64] = p2;
+    o[128] = p3;
+}
+
+template <int NJT, bool NORM>
+__global__ __launch_bounds__(512) void k_update_split3(int64_t n, int m1, int m, int k, const float* A, int64_t lda,
+                                                       const float* A2, int64_t lda2,
+                                                       const bf16x8* __restrict__ cs, float beta, float* Y,
+                                                       int64_t ldy, double* __restrict__ part) {
+    constexpr int KSC = kS3Chunk / 32;
+    __shared__ bf16x8 lds[KSC * NJT * 3 * 64];
+    __shared__ double red[NORM ? kS3Waves : 1][NORM ? NJT * 16 : 1];  // per-wave FP64 column sums of squares
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, qd = lane >> 4;
+    const int nks = (m + 31) / 32;
+    const int64_t ntile = (n + 63) / 64;
+    if constexpr (NORM)
+        for (int q = threadIdx.x; q < kS3Waves * NJT * 16; q += 512) (&red[0][0])[q] = 0.0;
+    for (int64_t t0 = (int64_t)blockIdx.x * kS3Waves; t0 < ntile; t0 += (int64_t)gridDim.x * kS3Waves) {
+        const int64_t p = (t0 + w) * 64 + 4 * r16;
+        const bool pin = p < n;  // n % 4 == 0: all four points or none
+        f32x4 acc[4][NJT];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int jt = 0; jt < NJT; ++jt) acc[c][jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks0 = 0; ks0 < nks; ks0 += KSC) {
+            __syncthreads();  // the previous chunk's operands are consumed
+            const int nsteps = min(KSC, nks - ks0);
+            const bf16x8* src = cs + (int64_t)ks0 * NJT * 3 * 64;
+            for (int q = threadIdx.x; q < nsteps * NJT * 3 * 64; q += 512) lds[q] = src[q];
+            __syncthreads();
+            for (int kk = 0; kk < nsteps; ++kk) {
+                const int ib = (ks0 + kk) * 32 + 8 * qd;
+                f32x4 x[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = ib + e;
+                    if (pin && i < m) {
+                        const float* s = i < m1 ? A + (int64_t)i * lda : A2 + (int64_t)(i - m1) * lda2;
+                        x[e] = *reinterpret_cast<const f32x4*>(s + p);
+                    } else {
+                        x[e] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+                bf16x8 b1[4], b2[4], b3[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float v = x[e][c];
+                        const __bf16 h1 = (__bf16)v;
+                        const float r1 = v - (float)h1;
+                        const __bf16 h2 = (__bf16)r1;
+                        b1[c][e] = h1;
+                        b2[c][e] = h2;
+                        b3[c][e] = (__bf16)(r1 - (float)h2);
+                    }
+#pragma unroll
+                for (int jt = 0; jt < NJT; ++jt) {
+                    const bf16x8* blk = lds + ((kk * NJT + jt) * 3) * 64;
+                    const bf16x8 a1 = blk[lane], a2 = blk[64 + lane], a3 = blk[128 + lane];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        f32x4 s = acc[c][jt];
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1[c], s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3[c], s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2[c], s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1[c], s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2[c], s, 0, 0, 0);
+                        acc[c][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[c], s, 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // epilogue; lanes past the grid keep to the shuffles of the norm sums
+#pragma unroll
+        for (int jt = 0; jt < NJT; ++jt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = jt * 16 + 4 * qd + r;
+                f32x4 o = f32x4{acc[0][jt][r], acc[1][jt][r], acc[2][jt][r], acc[3][jt][r]};
+                const bool st = pin && j < k;
+                if (st) {
+                    f32x4* y = reinterpret_cast<f32x4*>(Y + (int64_t)j * ldy + p);
+                    if (beta != 0.f) o += beta * *y;
+                    *y = o;
+                }
+                if constexpr (NORM) {
+                    double v = st ? ((double)o[0] * o[0] + (double)o[1] * o[1]) + ((double)o[2] * o[2] + (double)o[3] * o[3])
+                                  : 0.0;
+#pragma unroll
+                    for (int s = 1; s < 16; s <<= 1) v += __shfl_xor(v, s, 16);
+                    if (r16 == 0) red[w][j] += v;
+                }
+            }
+    }
+    if constexpr (NORM) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < k; j += 512) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < kS3Waves; ++q) s += red[q][j];
+            part[(int64_t)j * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+// true when the bf16 split update takes the shape (FP32, k <= 96, 16-byte
+// aligned columns)
+bool split3_fits(int64_t n, int k, int64_t lda, int64_t lda2, int64_t ldy, const void* a, const void* a2,
+                 const void* y) {
+    return k >= 1 && k <= 96 && n % 4 == 0 && lda % 4 == 0 && lda2 % 4 == 0 && ldy % 4 == 0 &&
+           reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(a2) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(y) % 16 == 0;
+}
+
+// Y = alpha [A | A2] C + beta Y on the bf16 matrix cores (C: FP64, m x k,
+// column-major, ldc); norm2 (device, k doubles) as update_mfma.
+int update_split3(dft_ctx* c, int64_t n, int m1, int m, int k, double alpha, const float* A, int64_t lda,
+                  const float* A2, int64_t lda2, const double* Cm, int64_t ldc, float beta, float* Y, int64_t ldy,
+                  double* norm2 = nullptr) {
+    if (m1 >= m) A2 = A, lda2 = lda;
+    const int njt = (k + 15) / 16, nks = (m + 31) / 32;
+    const size_t need = (size_t)nks * njt * 3 * 64 * sizeof(bf16x8);
+    if (c->coef3_cap < need) {
+        if (c->coef3) {
+            DFT_HIP(c, hipStreamSynchronize(c->stream));
+            DFT_HIP(c, hipFree(c->coef3));
+            c->coef3 = nullptr;
+            c->coef3_cap = 0;
+        }
+        DFT_HIP(c, hipMalloc(&c->coef3, need));
+        c->coef3_cap = need;
+    }
+    bf16x8* cs = static_cast<bf16x8*>(c->coef3);
+    const int64_t pq = (int64_t)nks * njt * 64;
+    k_split_coef<<<(unsigned)((pq + 255) / 256), 256, 0, c->stream>>>(m, k, njt, nks, alpha, Cm, ldc, cs);
+    DFT_CHECK_LAUNCH(c);
+    const int64_t ntile = (n + 63) / 64;
+    const unsigned blocks = (unsigned)std::min<int64_t>((ntile + kS3Waves - 1) / kS3Waves, 2LL * c->num_cus);
+    double* part = nullptr;
+    if (norm2) {
+        if (int rc = ensure_partials(c, (size_t)blocks * k + 8)) return rc;
+        part = c->partials;
+    }
+#define DFT_S3(NJ)                                                                                                  \
+    if (norm2)                                                                                                      \
+        k_update_split3<NJ, true><<<blocks, 512, 0, c->stream>>>(n, m1, m, k, A, lda, A2, lda2, cs, beta, Y, ldy, part); \
+    else                                                                                                            \
+        k_update_split3<NJ, false><<<blocks, 512, 0, c->stream>>>(n, m1, m, k, A, lda, A2, lda2, cs, beta, Y, ldy, part)
+    switch (njt) {
+        case 1: DFT_S3(1); break;
+        case 2: DFT_S3(2); break;
+        case 3: DFT_S3(3); break;
+        case 4: DFT_S3(4); break;
+        case 5: DFT_S3(5); break;
+        default: DFT_S3(6); break;
+    }
+#undef DFT_S3
+    DFT_CHECK_LAUNCH(c);
+    if (norm2) {
+        k_sum_parts<<<k, 64, 0, c->stream>>>(part, (int)blocks, norm2);
+        DFT_CHECK_LAUNCH(c);
+    }
+    return DFT_OK;
+}
+
 // coefficients of the fused Davidson residual, 2m x k column-major:
 // rows i < m: Q[i][j]; rows m + i: -theta[j] Q[i][j]
 template <typename T>
@@ -715,6 +933,9 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
                        (prec == DFT_F64 || prec == DFT_F32),
                 "dft_block_update: bad arguments");
     if (k == 0) return DFT_OK;
+    // FP32 beyond the FP32 MFMA kernel's LDS budget: the bf16 split kernel (in place, any m)
+    if (prec == DFT_F32 && !update_mfma_fits<float>(m, k) && split3_fits(n, k, lda, lda, ldy, a, a, y))
+        return update_split3(c, n, m, m, k, alpha, (const float*)a, lda, nullptr, 0, cm, ldc, (float)beta, (float*)y, ldy);
     rocblas_handle h;
     if (int rc = blas(c, &h)) return rc;
     const size_t es = prec == DFT_F64 ? sizeof(double) : sizeof(float);
@@ -781,7 +1002,8 @@ int dft_block_update(dft_ctx* c, int prec, int64_t n, int m, int k, double alpha
 int dft_block_update_in_place(int prec, int m, int k) {
     if (m <= 0 || k <= 0) return 0;
     if (prec == DFT_F64) return update_mfma_fits<double>(m, k) ? 1 : 0;
-    if (prec == DFT_F32) return update_mfma_fits<float>(m, k) || (k <= 64 && m <= 512) ? 1 : 0;
+    // FP32: the FP32 MFMA kernel, the bf16 split kernel (k <= 96) or the VALU kernel
+    if (prec == DFT_F32) return update_mfma_fits<float>(m, k) || k <= 96 ? 1 : 0;
     return 0;
 }
 
@@ -890,6 +1112,15 @@ int dft_ritz_residual(dft_ctx* c, int prec, int64_t n, int m, int k, const void*
         DFT_CHECK_LAUNCH(c);
         return update_mfma<float>(c, n, m, 2 * m, k, 1.0f, (const float*)w, ldw, (const float*)v, ldv, cf, 2 * m, 0.0f,
                                   (float*)r, ldr, rn2);
+    }
+    // FP32 beyond the LDS budget: one bf16 split pass over [W | V] with its norms
+    if (prec == DFT_F32 && split3_fits(n, k, ldw, ldv, ldr, w, v, r)) {
+        if (int rc = ensure_scratch(c, sizeof(double) * (size_t)2 * m * k)) return rc;
+        double* cf = (double*)c->scratch;
+        k_ritz_coef<double><<<gc, 256, 0, c->stream>>>(m, k, q, ldq, theta, cf);
+        DFT_CHECK_LAUNCH(c);
+        return update_split3(c, n, m, 2 * m, k, 1.0, (const float*)w, ldw, (const float*)v, ldv, cf, 2 * m, 0.0f,
+                             (float*)r, ldr, rn2);
     }
     // 2m x k beyond the LDS budget: chunks of the [W | V] rows, the norms from the last chunk's epilogue
     const int mc = prec == DFT_F64 ? update_chunk<double>(2 * m, k) : update_chunk<float>(2 * m, k);

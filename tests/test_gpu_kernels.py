@@ -440,9 +440,10 @@ def test_gram_modes(dev, n, ma, mb):
                                    (40000, 300, 40), (40001, 270, 17)])
 def test_block_update(dev, n, m, k):
     """Y = alpha A C + beta Y against numpy, FP64 and FP32: one MFMA pass; C
-    beyond the LDS budget with k <= 48 (m = 300, 270) as chunks of the inner
-    dimension accumulated into Y; wider blocks (m = 192, 288, 131 at k = 96,
-    70) on the library GEMM."""
+    beyond the LDS budget: FP32 on the bf16 three-piece split kernel when the
+    columns are 16-byte aligned (m = 192, 288, 300), else (and FP64) with
+    k <= 48 as chunks of the inner dimension accumulated into Y (m = 270),
+    wider blocks (m = 131 at k = 70, FP64 at k = 96) on the library GEMM."""
     rng = np.random.default_rng(m)
     A = rng.standard_normal((m, n))
     Cm = rng.standard_normal((k, m))  # column-major m x k
@@ -465,9 +466,11 @@ def test_block_update(dev, n, m, k):
 def test_ritz_residual(dev, n, m, k):
     """R = W Q - V Q diag(theta) and its column norms^2, against numpy: one
     fused pass (vector and scalar grid sizes); with the 2m x k coefficients
-    beyond the LDS budget and k <= 48 (m = 150, 201), chunks of the [W | V]
-    rows, one of them straddling W and V, the norms from the last; wider
-    blocks (m = 288, 97 at k = 96, 70) as two library passes and the norms."""
+    beyond the LDS budget, FP32 as one bf16 split pass over [W | V] with its
+    norms when the columns are 16-byte aligned (m = 150, 288), else with
+    k <= 48 as chunks of the [W | V] rows, one of them straddling W and V,
+    the norms from the last (m = 201; FP64 m = 150), and wider blocks
+    (m = 97 at k = 70; FP64 m = 288) as two library passes and the norms."""
     rng = np.random.default_rng(n + m)
     V = rng.standard_normal((m, n))
     W = rng.standard_normal((m, n))
@@ -504,13 +507,15 @@ def test_gram_trailing_columns(dev):
         assert rel(out.download().T, ref) <= tol
 
 
-@pytest.mark.parametrize("n,m,k,path", [(70000, 40, 13, "mfma"), (70001, 29, 8, "mfma"), (30000, 300, 40, "valu"),
-                                        (30000, 300, 80, "library"), (30000, 200, 40, "library")])
+@pytest.mark.parametrize("n,m,k,path", [(70000, 40, 13, "mfma"), (70001, 29, 8, "mfma"), (30000, 300, 40, "split"),
+                                        (30000, 300, 80, "split"), (30000, 200, 40, "split"), (30000, 288, 96, "split"),
+                                        (30001, 300, 40, "valu"), (30001, 300, 80, "library")])
 def test_block_update_in_place(dev, n, m, k, path):
     """Y = the last k columns of A, overwritten with A C (the Davidson's
-    in-place block orthonormalisation), on each kernel path: the MFMA and
-    single-launch VALU kernels in place, the library GEMM through a staging
-    copy.  (FP32 VALU needs k <= 64 with C^T beyond the MFMA LDS budget.)"""
+    in-place block orthonormalisation), on each kernel path: the MFMA, bf16
+    split and single-launch VALU kernels in place, the library GEMM through a
+    staging copy.  (FP32: the split kernel needs 16-byte aligned columns, the
+    VALU kernel k <= 64 with C^T beyond the MFMA LDS budget.)"""
     from types import SimpleNamespace
     rng = np.random.default_rng(n + k)
     A = rng.standard_normal((m, n))
