@@ -388,6 +388,18 @@ class Device:
         self._chk(self.L.dft_cnn_prepare(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
                                          C.c_int64(g.size), C.c_int(ncols), _vp(theta.ptr), _vp(scale.ptr), _vp(xin.ptr)))
 
+    def cnn_prepare_box(self, g, veff, r, ncols, theta, scale, box, xin):
+        b = np.ascontiguousarray(box, dtype=np.int64)
+        self._chk(self.L.dft_cnn_prepare_box(self.ctx, C.byref(g), C.c_int(r.prec), _vp(veff.ptr), _vp(r.ptr),
+                                             C.c_int64(g.size), C.c_int(ncols), _vp(theta.ptr), _vp(scale.ptr),
+                                             b.ctypes.data_as(C.POINTER(C.c_int64)), _vp(xin.ptr)))
+
+    def cnn_add_box(self, g, ncols, out, scale, box, taper, t):
+        b = np.ascontiguousarray(box, dtype=np.int64)
+        self._chk(self.L.dft_cnn_add_box(self.ctx, C.byref(g), C.c_int(t.prec), C.c_int(ncols), _vp(out.ptr),
+                                         _vp(scale.ptr), b.ctypes.data_as(C.POINTER(C.c_int64)), C.c_int(taper),
+                                         _vp(t.ptr), C.c_int64(g.size)))
+
     def cnn_finish(self, g, ncols, out, scale, t):
         self._chk(self.L.dft_cnn_finish(self.ctx, C.byref(g), C.c_int(t.prec), C.c_int(ncols), _vp(out.ptr),
                                         _vp(scale.ptr), _vp(t.ptr), C.c_int64(g.size)))
@@ -562,26 +574,33 @@ def cnn_param_count(channels=CNN_CHANNELS):
 
 
 class _CnnModelC(C.Structure):
-    _fields_ = [("channels", C.c_int), ("count", C.c_int64), ("params", C.POINTER(C.c_float))]
+    _fields_ = [("channels", C.c_int), ("count", C.c_int64), ("params", C.POINTER(C.c_float)), ("base", C.c_int)]
+
+
+CNN_BASE_NONE, CNN_BASE_KINETIC = 0, 1
 
 
 class CnnModel:
-    """Trained network parameters (FP32 master copy, kernel layout); save/load as .npz."""
+    """Trained network parameters (FP32 master copy, kernel layout); save/load as .npz.  base:
+    CNN_BASE_NONE (the output is the correction) or CNN_BASE_KINETIC (the output is added to the
+    exact kinetic correction over the molecule's box; trained on harvests with target_scale 2)."""
 
-    def __init__(self, params, channels=CNN_CHANNELS):
+    def __init__(self, params, channels=CNN_CHANNELS, base=CNN_BASE_NONE):
         self.params = np.ascontiguousarray(params, dtype=np.float32)
         if self.params.size != cnn_param_count(channels):
             raise ValueError("parameter count does not match the network")
-        self.channels = channels
-        self._c = _CnnModelC(channels, self.params.size, self.params.ctypes.data_as(C.POINTER(C.c_float)))
+        if base not in (CNN_BASE_NONE, CNN_BASE_KINETIC):
+            raise ValueError("base must be CNN_BASE_NONE or CNN_BASE_KINETIC")
+        self.channels, self.base = channels, base
+        self._c = _CnnModelC(channels, self.params.size, self.params.ctypes.data_as(C.POINTER(C.c_float)), base)
 
     def save(self, path):
-        np.savez(path, params=self.params, channels=np.int64(self.channels))
+        np.savez(path, params=self.params, channels=np.int64(self.channels), base=np.int64(self.base))
 
     @classmethod
     def load(cls, path):
         with np.load(path, allow_pickle=False) as z:
-            return cls(z["params"], int(z["channels"]))
+            return cls(z["params"], int(z["channels"]), int(z["base"]) if "base" in z.files else CNN_BASE_NONE)
 
 
 def _model_ptr(m):
@@ -675,8 +694,8 @@ class CnnTrainer:
         if rc != 0:
             _core_err(rc)
 
-    def model(self):
-        return CnnModel(self.params(), self.channels)
+    def model(self, base=CNN_BASE_NONE):
+        return CnnModel(self.params(), self.channels, base)
 
     def patches(self, first, count):
         P3 = self.patch ** 3

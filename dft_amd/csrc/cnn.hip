@@ -287,6 +287,57 @@ __global__ __launch_bounds__(256) void k_cnn_finish(int64_t N, int ncols, const 
     Tm[(int64_t)b * ldt + v] = (T)((double)out[(int64_t)b * N + v] * scale[b] * h2);
 }
 
+// The network's inputs on a sub-box of the grid: box = {x0, y0, z0, bx, by,
+// bz}; in[b][0][v] = r_b[g(v)] / s_b, in[b][1][v] = (V[g(v)] - theta_b) h2,
+// v over the box (x fastest), g(v) the grid index.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cnn_prep_box(int nx, int ny, int x0, int y0, int z0, int bx, int by, int bz,
+                                                      int ncols, const T* __restrict__ V, const T* __restrict__ R,
+                                                      int64_t ldr, const double* __restrict__ theta,
+                                                      const double* __restrict__ scale, double h2,
+                                                      float* __restrict__ in) {
+    const int64_t nb = (int64_t)bx * by * bz;
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (v >= nb || b >= ncols) return;
+    const int lx = (int)(v % bx), ly = (int)((v / bx) % by), lz = (int)(v / ((int64_t)bx * by));
+    const int64_t g = ((int64_t)(z0 + lz) * ny + (y0 + ly)) * nx + (x0 + lx);
+    const double s = scale[b];
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    in[((int64_t)b * 2) * nb + v] = (float)((double)R[(int64_t)b * ldr + g] * inv);
+    in[((int64_t)b * 2 + 1) * nb + v] = (float)(((double)V[g] - theta[b]) * h2);
+}
+
+// Linear ramp over `taper` points at a box face that lies inside the grid
+// (1 at and beyond taper + 1 points from the face), 1 at a face on the grid
+// boundary, where the network's zero padding is the orbitals' own boundary.
+__device__ inline double box_ramp(int l, int b, int lo_inside, int hi_inside, int taper) {
+    double w = 1.0;
+    if (taper > 0) {
+        if (lo_inside) w = fmin(w, (double)(l + 1) / (double)(taper + 1));
+        if (hi_inside) w = fmin(w, (double)(b - l) / (double)(taper + 1));
+    }
+    return w;
+}
+
+// T_b[g(v)] += out[b][v] s_b h2 w(v) over the box (w: the face ramps).
+template <typename T>
+__global__ __launch_bounds__(256) void k_cnn_add_box(int nx, int ny, int nz, int x0, int y0, int z0, int bx, int by,
+                                                     int bz, int taper, int ncols, const float* __restrict__ out,
+                                                     const double* __restrict__ scale, double h2, T* __restrict__ Tm,
+                                                     int64_t ldt) {
+    const int64_t nb = (int64_t)bx * by * bz;
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (v >= nb || b >= ncols) return;
+    const int lx = (int)(v % bx), ly = (int)((v / bx) % by), lz = (int)(v / ((int64_t)bx * by));
+    const int64_t g = ((int64_t)(z0 + lz) * ny + (y0 + ly)) * nx + (x0 + lx);
+    const double w = box_ramp(lx, bx, x0 > 0, x0 + bx < nx, taper) * box_ramp(ly, by, y0 > 0, y0 + by < ny, taper) *
+                     box_ramp(lz, bz, z0 > 0, z0 + bz < nz, taper);
+    T* t = Tm + (int64_t)b * ldt + g;
+    *t = (T)((double)*t + (double)out[(int64_t)b * nb + v] * scale[b] * h2 * w);
+}
+
 // max |X_b| per column: partials[b * nblk + blk]
 template <typename T>
 __global__ __launch_bounds__(256) void k_colmax(int64_t n, const T* __restrict__ X, int64_t ldx, int nblk,
@@ -503,6 +554,53 @@ int dft_cnn_prepare(dft_ctx* c, const dft_grid* g, int prec, const void* veff, c
                                                        scale_dev, h2, in);
     else
         return set_err(c, DFT_EINVAL, "dft_cnn_prepare: bad precision");
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+static int box_ok(const dft_grid* g, const int64_t* box) {
+    return box && box[0] >= 0 && box[1] >= 0 && box[2] >= 0 && box[3] >= 1 && box[4] >= 1 && box[5] >= 1 &&
+           box[0] + box[3] <= g->nx && box[1] + box[4] <= g->ny && box[2] + box[5] <= g->nz;
+}
+
+int dft_cnn_prepare_box(dft_ctx* c, const dft_grid* g, int prec, const void* veff, const void* R, int64_t ldr,
+                        int ncols, const double* theta_dev, const double* scale_dev, const int64_t* box, float* in) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, valid_grid(g) && veff && R && theta_dev && scale_dev && in && ncols > 0 && ncols <= 65535 &&
+                       ldr >= grid_points(g) && box_ok(g, box),
+                "dft_cnn_prepare_box: bad arguments");
+    const int64_t nb = box[3] * box[4] * box[5];
+    const double h2 = 1.0 / (1.0 / (g->dx * g->dx) + 1.0 / (g->dy * g->dy) + 1.0 / (g->dz * g->dz));
+    const dim3 grid((unsigned)((nb + 255) / 256), ncols);
+#define DFT_PREP_BOX(E)                                                                                              \
+    k_cnn_prep_box<E><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)box[0], (int)box[1], (int)box[2],  \
+                                                   (int)box[3], (int)box[4], (int)box[5], ncols, (const E*)veff,   \
+                                                   (const E*)R, ldr, theta_dev, scale_dev, h2, in)
+    if (prec == DFT_F64) DFT_PREP_BOX(double);
+    else if (prec == DFT_F32) DFT_PREP_BOX(float);
+    else return set_err(c, DFT_EINVAL, "dft_cnn_prepare_box: bad precision");
+#undef DFT_PREP_BOX
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+int dft_cnn_add_box(dft_ctx* c, const dft_grid* g, int prec, int ncols, const float* out, const double* scale_dev,
+                    const int64_t* box, int taper, void* T, int64_t ldt) {
+    if (!c) return DFT_EINVAL;
+    DFT_REQUIRE(c, valid_grid(g) && out && scale_dev && T && ncols > 0 && ncols <= 65535 && ldt >= grid_points(g) &&
+                       taper >= 0 && box_ok(g, box),
+                "dft_cnn_add_box: bad arguments");
+    const int64_t nb = box[3] * box[4] * box[5];
+    const double h2 = 1.0 / (1.0 / (g->dx * g->dx) + 1.0 / (g->dy * g->dy) + 1.0 / (g->dz * g->dz));
+    const dim3 grid((unsigned)((nb + 255) / 256), ncols);
+#define DFT_ADD_BOX(E)                                                                                               \
+    k_cnn_add_box<E><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, (int)box[0], (int)box[1],    \
+                                                  (int)box[2], (int)box[3], (int)box[4], (int)box[5], taper, ncols, \
+                                                  out, scale_dev, h2, (E*)T, ldt)
+    if (prec == DFT_F64) DFT_ADD_BOX(double);
+    else if (prec == DFT_F32) DFT_ADD_BOX(float);
+    else return set_err(c, DFT_EINVAL, "dft_cnn_add_box: bad precision");
+#undef DFT_ADD_BOX
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }

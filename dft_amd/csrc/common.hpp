@@ -26,10 +26,6 @@ struct dft_ctx {
     void* scratch = nullptr;
     size_t scratch_cap = 0;
     int num_cus = 256;
-    // set when the runtime refused a cooperative launch (the device's
-    // cooperative queue can be held by another process); Jacobi then runs
-    // one launch per sweep
-    bool coop_refused = false;
     // BF16 sine-transform matrices of the exact kinetic correction, for the
     // grid shape dst_shape (built on first use)
     void* dst_mats = nullptr;

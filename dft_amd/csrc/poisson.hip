@@ -10,7 +10,6 @@
 // host finds the exact sweep at which the reference would have stopped.
 #include "common.hpp"
 
-#include <hip/hip_cooperative_groups.h>
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -151,29 +150,6 @@ __global__ __launch_bounds__(256) void k_jacobi(int nx, int ny, int nz, int zc, 
     block_max_to((double)md, red, slot);
 }
 
-// Several sweeps per launch: a cooperative grid of resident workgroups walks
-// the tiles of each sweep, then meets at a grid-wide barrier before the next
-// sweep reads what this one wrote.  Sweep s writes its max|dV| to slot[s].
-template <typename T, int TY>
-__global__ __launch_bounds__(256) void k_jacobi_multi(int nx, int ny, int nz, int zc, int gx, int gy, int gz, JCoef<T> co,
-                                                      const T* __restrict__ rho, T* a, T* b, int nsweeps,
-                                                      unsigned long long* __restrict__ slot) {
-    __shared__ double red[4];
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-    const int ntiles = gx * gy * gz;
-    for (int s = 0; s < nsweeps; ++s) {
-        const T* src = (s & 1) ? b : a;
-        T* dst = (s & 1) ? a : b;
-        T md = T(0);
-        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-            const T m = jacobi_tile<T, TY>(nx, ny, nz, zc, co, rho, src, dst, t % gx, (t / gx) % gy, t / (gx * gy));
-            md = md < m ? m : md;
-        }
-        block_max_to((double)md, red, slot + s);
-        grid.sync();
-    }
-}
-
 // Boundary values (HartreePotentialMolecule.cpp:37-95, 171-202): 0 on a
 // face, else Q/|r| measured from the origin.  Written to both buffers.
 template <typename T>
@@ -220,28 +196,14 @@ struct Jacobi {
     int zc;
     const T* rho;
 
-    int coop_blocks = 0;  // resident workgroups for the cooperative kernel (0: unavailable)
-
     int sweep(const T* in, T* out, unsigned long long* slot) {
         k_jacobi<T, 4><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, zc, co, rho, in, out, slot, T(1));
         DFT_CHECK_LAUNCH(c);
         return DFT_OK;
     }
 
-    // nsweeps sweeps starting from buffer a (even sweeps a -> b, odd b -> a)
+    // nsweeps sweeps starting from buffer a (even sweeps a -> b, odd b -> a), one launch each
     int sweeps(T* a, T* b, int nsweeps, unsigned long long* slot) {
-        if (coop_blocks > 0 && nsweeps > 1 && !c->coop_refused) {
-            int nx = (int)g->nx, ny = (int)g->ny, nz = (int)g->nz, gx = (int)grid.x, gy = (int)grid.y, gz = (int)grid.z;
-            int zcv = zc;
-            void* args[] = {&nx, &ny, &nz, &zcv, &gx, &gy, &gz, &co, &rho, &a, &b, &nsweeps, &slot};
-            const hipError_t e = hipLaunchCooperativeKernel((const void*)k_jacobi_multi<T, 4>, dim3(coop_blocks),
-                                                            dim3(256), args, 0, c->stream);
-            if (e == hipSuccess) return DFT_OK;
-            // refused before anything was queued (e.g. the cooperative queue
-            // is held by another process): clear the error, use plain launches
-            (void)hipGetLastError();
-            c->coop_refused = true;
-        }
         for (int s = 0; s < nsweeps; ++s)
             if (int rc = sweep((s & 1) ? b : a, (s & 1) ? a : b, slot + s)) return rc;
         return DFT_OK;
@@ -289,17 +251,6 @@ void jacobi_setup(dft_ctx* c, const dft_grid* g, const T* rho, Jacobi<T>& J) {
     J.zc = (nz + nzc - 1) / nzc;
     nzc = (nz + J.zc - 1) / J.zc;
     J.grid = dim3(gx, gy, nzc);
-    // The cooperative kernel needs every workgroup resident at once.  It is
-    // used only when the grid has at most one workgroup per CU, where that
-    // holds by construction (one 256-thread workgroup always fits an idle
-    // CU); larger grids run one launch per sweep, whose launch overhead is
-    // small next to a sweep of that size.
-    int per_cu = 0;
-    const int64_t tiles = base * nzc;
-    if (tiles <= c->num_cus &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi_multi<T, 4>, 256, 0) == hipSuccess &&
-        per_cu > 0)
-        J.coop_blocks = (int)tiles;
 }
 
 template <typename T>

@@ -5,10 +5,19 @@
 // The reference preconditions each Davidson correction with the diagonal of
 // H only (davidsonCorrection, EigenvalueSolverMolecule.cpp:483-555, called at
 // :1042-1047; its non-finite / tiny-denominator fallbacks are :1049-1104).
-// Here that diagonal correction stays the fallback: a column whose residual
-// norm did not go down (by at least 5%) after a learned correction gets the
-// diagonal one at the next iteration, and after two such failures for the
-// rest of the solve.
+// Here that diagonal correction stays the fallback, per iteration: a column
+// whose residual norm did not go down after a learned correction gets the
+// diagonal one at the next iteration, and the learned one again after that.
+//
+// Two kinds of model (dft_cnn_model::base):
+//   0: the network's output is the whole correction (c / (max|r| h2) or
+//      c / max|c|), on the full grid;
+//   1: the network's output, times max|r| h2, is added to the exact kinetic
+//      correction t_kin = (-1/2 Lap + sigma_j)^-1 r (per-state shifts
+//      sigma_j = max(0.5, 0.35 |theta_j|)) over a box around the nuclei,
+//      ramped to zero over 4 points at box faces inside the grid; outside the
+//      box the correction is t_kin.  The network then carries only what the
+//      potential adds to the kinetic inverse, which is largest near nuclei.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -64,22 +73,31 @@ std::vector<float> cnn_initial_params(int ch, uint64_t seed) {
 
 class CnnPreconditioner final : public Preconditioner {
    public:
-    CnnPreconditioner(dft_ctx* c, const dft_cnn_model& m, const dft_grid& g) : c_(c), g_(g), n_(grid_n(g)) {
+    CnnPreconditioner(dft_ctx* c, const dft_cnn_model& m, const dft_grid& g, const int64_t* box)
+        : c_(c), g_(g), n_(grid_n(g)), base_(m.base == 1),
+          kin_(0, 0.5, 2.0 / 3.0, KineticPreconditioner::kExact, 0.35) {
         if (m.channels != kChannels || !m.params || m.count != param_count(m.channels))
             throw std::invalid_argument("learned preconditioner: model does not match the 16-channel network");
+        if (m.base != 0 && m.base != 1) throw std::invalid_argument("learned preconditioner: model base must be 0 or 1");
+        const int64_t full[6] = {0, 0, 0, g.nx, g.ny, g.nz};
+        for (int i = 0; i < 6; ++i) box_[i] = base_ && box ? box[i] : full[i];
+        if (box_[0] < 0 || box_[1] < 0 || box_[2] < 0 || box_[3] < 1 || box_[4] < 1 || box_[5] < 1 ||
+            box_[0] + box_[3] > g.nx || box_[1] + box_[4] > g.ny || box_[2] + box_[5] > g.nz)
+            throw std::invalid_argument("learned preconditioner: box outside the grid");
+        nb_ = box_[3] * box_[4] * box_[5];
         count_ = m.count;
         w32_ = Buf(c, sizeof(float) * count_);
         w32_.upload(m.params, sizeof(float) * count_);
         wbf_ = Buf(c, sizeof(uint16_t) * count_);
         check(c, dft_cnn_to_bf16(c, count_, w32_.as<float>(), wbf_.as<uint16_t>()), "dft_cnn_to_bf16");
         // columns per forward pass: activations of 2 x 16 channels plus input/output per column
-        const double per_col = 4.0 * n_ * (2 * kChannels + 3);
+        const double per_col = 4.0 * nb_ * (2 * kChannels + 3);
         chunk_ = (int)std::max(1.0, std::min(32.0, std::floor(kBudget / per_col)));
         int64_t act = 0;
-        check(c, dft_cnn_act_count(kChannels, g.nx, g.ny, g.nz, chunk_, 0, &act), "dft_cnn_act_count");
+        check(c, dft_cnn_act_count(kChannels, box_[3], box_[4], box_[5], chunk_, 0, &act), "dft_cnn_act_count");
         act_ = Buf(c, sizeof(float) * act);
-        in_ = Buf(c, sizeof(float) * 2 * n_ * chunk_);
-        out_ = Buf(c, sizeof(float) * n_ * chunk_);
+        in_ = Buf(c, sizeof(float) * 2 * nb_ * chunk_);
+        out_ = Buf(c, sizeof(float) * nb_ * chunk_);
     }
 
     int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* X, int64_t ldx, const void* R,
@@ -94,31 +112,43 @@ class CnnPreconditioner final : public Preconditioner {
             scale_ = Buf(c, sizeof(double) * ncols);
             scale_cap_ = ncols;
         }
+        if (base_) kin_.apply(c, g, prec, veff, X, ldx, R, ldr, first, ncols, theta_dev, T, ldt);
         double* sc = scale_.as<double>();
         for (int j0 = 0; j0 < ncols; j0 += chunk_) {
             const int nc = std::min(chunk_, ncols - j0);
             const void* Rj = static_cast<const char*>(R) + (size_t)j0 * ldr * es;
             void* Tj = static_cast<char*>(T) + (size_t)j0 * ldt * es;
             check(c, dft_column_maxabs(c, prec, n_, nc, Rj, ldr, sc + j0), "dft_column_maxabs");
-            check(c, dft_cnn_prepare(c, &g, prec, veff, Rj, ldr, nc, theta_dev + j0, sc + j0, in_.as<float>()),
-                  "dft_cnn_prepare");
-            check(c, dft_cnn_forward(c, kChannels, g.nx, g.ny, g.nz, nc, wbf_.as<uint16_t>(), in_.as<float>(),
-                                     out_.as<float>(), act_.as<float>(), 0),
-                  "dft_cnn_forward");
-            check(c, dft_cnn_finish(c, &g, prec, nc, out_.as<float>(), sc + j0, Tj, ldt), "dft_cnn_finish");
+            if (base_) {
+                check(c, dft_cnn_prepare_box(c, &g, prec, veff, Rj, ldr, nc, theta_dev + j0, sc + j0, box_,
+                                             in_.as<float>()),
+                      "dft_cnn_prepare_box");
+                check(c, dft_cnn_forward(c, kChannels, box_[3], box_[4], box_[5], nc, wbf_.as<uint16_t>(),
+                                         in_.as<float>(), out_.as<float>(), act_.as<float>(), 0),
+                      "dft_cnn_forward");
+                check(c, dft_cnn_add_box(c, &g, prec, nc, out_.as<float>(), sc + j0, box_, kTaper, Tj, ldt),
+                      "dft_cnn_add_box");
+            } else {
+                check(c, dft_cnn_prepare(c, &g, prec, veff, Rj, ldr, nc, theta_dev + j0, sc + j0, in_.as<float>()),
+                      "dft_cnn_prepare");
+                check(c, dft_cnn_forward(c, kChannels, g.nx, g.ny, g.nz, nc, wbf_.as<uint16_t>(), in_.as<float>(),
+                                         out_.as<float>(), act_.as<float>(), 0),
+                      "dft_cnn_forward");
+                check(c, dft_cnn_finish(c, &g, prec, nc, out_.as<float>(), sc + j0, Tj, ldt), "dft_cnn_finish");
+            }
         }
         // diagonal correction for the columns in fallback, in contiguous runs
         int nf = 0;
         for (int j = 0; j < ncols;) {
             const int gj = first + j;
-            const bool fb = fails_[gj] >= 2 || mode_[gj] == kFallbackNext;
+            const bool fb = mode_[gj] == kFallbackNext;
             if (!fb) {
                 mode_[gj] = kLearned;
                 ++j;
                 continue;
             }
             int j1 = j;
-            while (j1 < ncols && (fails_[first + j1] >= 2 || mode_[first + j1] == kFallbackNext)) ++j1;
+            while (j1 < ncols && mode_[first + j1] == kFallbackNext) ++j1;
             DiagonalPreconditioner d;
             d.apply(c, g, prec, veff, X, ldx, static_cast<const char*>(R) + (size_t)j * ldr * es, ldr, gj, j1 - j,
                     theta_dev + j, static_cast<char*>(T) + (size_t)j * ldt * es, ldt);
@@ -135,8 +165,7 @@ class CnnPreconditioner final : public Preconditioner {
         const size_t n = std::min({before.size(), after.size(), mode_.size()});
         for (size_t j = 0; j < n; ++j) {
             if (mode_[j] == kLearned) {
-                // "reduced" asks for real progress: a 5% drop of the column's residual norm
-                const bool reduced = after[j] < kProgress * before[j];
+                const bool reduced = after[j] < before[j];
                 if (!reduced) ++fails_[j];
                 mode_[j] = reduced ? kNone : kFallbackNext;
             } else if (mode_[j] == kDiagonal) {
@@ -154,11 +183,14 @@ class CnnPreconditioner final : public Preconditioner {
 
    private:
     static constexpr double kBudget = 6.0e9;  // bytes of activations per forward pass
-    static constexpr double kProgress = 0.95;
+    static constexpr int kTaper = 4;
     enum : int { kNone = 0, kLearned = 1, kDiagonal = 2, kFallbackNext = 3 };
     dft_ctx* c_;
     dft_grid g_;
-    int64_t n_, count_ = 0;
+    int64_t n_, count_ = 0, nb_ = 0;
+    int64_t box_[6];
+    bool base_;
+    KineticPreconditioner kin_;
     int chunk_ = 1;
     Buf w32_, wbf_, act_, in_, out_, scale_;
     int scale_cap_ = 0;
@@ -166,9 +198,10 @@ class CnnPreconditioner final : public Preconditioner {
     int64_t applied_ = 0, fallbacks_ = 0;
 };
 
-std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g) {
+std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g,
+                                                        const int64_t* box) {
     if (!model) return nullptr;
-    return std::unique_ptr<Preconditioner>(new CnnPreconditioner(c, *static_cast<const dft_cnn_model*>(model), g));
+    return std::unique_ptr<Preconditioner>(new CnnPreconditioner(c, *static_cast<const dft_cnn_model*>(model), g, box));
 }
 
 // ---- training-data harvest ---------------------------------------------------
@@ -191,8 +224,10 @@ class HarvestPreconditioner final : public Preconditioner {
    public:
     bool needs_ritz_vectors() const override { return true; }  // projects the CG targets against X
     HarvestPreconditioner(dft_ctx* c, const dft_grid& g, PatchSet& ps, const dft_cnn_harvest_config& hc)
-        : c_(c), g_(g), n_(grid_n(g)), ps_(ps), hc_(hc), rng_(hc.seed) {
+        : c_(c), g_(g), n_(grid_n(g)), ps_(ps), hc_(hc), rng_(hc.seed),
+          kin_(0, 0.5, 2.0 / 3.0, KineticPreconditioner::kExact, 0.35) {
         if (hc.patch != ps.P) throw std::invalid_argument("harvest: patch size differs from the trainer's");
+        if (hc.target_scale < 0 || hc.target_scale > 2) throw std::invalid_argument("harvest: target_scale must be 0, 1 or 2");
         if (g.nx < ps.P || g.ny < ps.P || g.nz < ps.P) throw std::invalid_argument("harvest: grid smaller than a patch");
         if (hc.max_columns < 1 || hc.patches_per_column < 1 || hc.every < 1 || hc.cg_max_iter < 1 || !(hc.cg_tol > 0))
             throw std::invalid_argument("harvest: bad configuration");
@@ -206,11 +241,15 @@ class HarvestPreconditioner final : public Preconditioner {
 
     int apply(dft_ctx* c, const dft_grid& g, int prec, const void* veff, const void* X, int64_t ldx, const void* R,
               int64_t ldr, int first, int ncols, const double* theta_dev, void* T, int64_t ldt) override {
-        DiagonalPreconditioner d;
-        d.apply(c, g, prec, veff, X, ldx, R, ldr, first, ncols, theta_dev, T, ldt);
+        if (hc_.target_scale == 2) {
+            kin_.apply(c, g, prec, veff, X, ldx, R, ldr, first, ncols, theta_dev, T, ldt);
+        } else {
+            DiagonalPreconditioner d;
+            d.apply(c, g, prec, veff, X, ldx, R, ldr, first, ncols, theta_dev, T, ldt);
+        }
         if (prec != DFT_F64 || ps_.count >= ps_.cap || ncols <= 0) return 0;
         if (calls_++ % hc_.every != 0) return 0;
-        harvest(veff, X, ldx, R, ldr, first, std::min(ncols, hc_.max_columns), theta_dev);
+        harvest(veff, X, ldx, R, ldr, first, std::min(ncols, hc_.max_columns), theta_dev, T, ldt);
         return 0;
     }
     void reset() override { calls_ = 0; }
@@ -233,15 +272,15 @@ class HarvestPreconditioner final : public Preconditioner {
               "dft_block_update");
     }
 
-    void dots(const void* A, const void* B, int nc, std::vector<double>& out) {
+    void dots(const void* A, const void* B, int nc, std::vector<double>& out, int64_t lda = 0) {
         double* d = small_.as<double>() + 6 * hc_.max_columns;
-        check(c_, dft_column_dots(c_, DFT_F64, n_, nc, A, n_, B, n_, d), "dft_column_dots");
+        check(c_, dft_column_dots(c_, DFT_F64, n_, nc, A, lda ? lda : n_, B, n_, d), "dft_column_dots");
         out.resize(nc);
         check(c_, dft_download(c_, out.data(), d, sizeof(double) * nc), "dft_download");
     }
 
     void harvest(const void* veff, const void* X, int64_t ldx, const void* R, int64_t ldr, int first, int nc,
-                 const double* theta_dev) {
+                 const double* theta_dev, const void* Tb, int64_t ldt) {
         const int mc = hc_.max_columns;
         double* a = small_.as<double>();
         double* b = a + mc;
@@ -297,6 +336,24 @@ class HarvestPreconditioner final : public Preconditioner {
         }
         cg_iterations += it;
         solves += nc;
+        if (hc_.target_scale == 2) {
+            // what the network must add to t_kin (the base correction, in Tb) beyond the span of the
+            // Ritz vectors, which Davidson projects out: with t = (I - QQ^T) t_kin (Q as in the CG),
+            // the target is beta c - t, beta = <t, c> / <c, c> (the scale of c closest to t)
+            double* t = Pd_.as<double>();
+            for (int j = 0; j < nc; ++j)
+                check(c_, dft_copy(c_, t + (size_t)j * n_, static_cast<const char*>(Tb) + (size_t)j * ldt * sizeof(double),
+                                   sizeof(double) * n_), "dft_copy");
+            project(X, ldx, first, nc, t);
+            std::vector<double> tc, cc;
+            dots(t, Cc_.get(), nc, tc);
+            dots(Cc_.get(), Cc_.get(), nc, cc);
+            std::vector<double> beta(nc), m1(nc, -1.0);
+            for (int j = 0; j < nc; ++j) beta[j] = cc[j] > 0.0 ? tc[j] / cc[j] : 0.0;
+            check(c_, dft_upload(c_, a, m1.data(), sizeof(double) * nc), "dft_upload");
+            check(c_, dft_upload(c_, b, beta.data(), sizeof(double) * nc), "dft_upload");
+            check(c_, dft_cols_axpby(c_, DFT_F64, n_, nc, a, t, n_, b, Cc_.get(), n_), "dft_cols_axpby");
+        }
         // patches: the highest-residual aligned blocks of each converged column, jittered
         const int P = ps_.P;
         const int nbx = (int)(g_.nx / P), nby = (int)(g_.ny / P), nbz = (int)(g_.nz / P), nb = nbx * nby * nbz;
@@ -317,7 +374,12 @@ class HarvestPreconditioner final : public Preconditioner {
             std::iota(order.begin(), order.end(), 0);
             const double* e = E_h_.data() + (size_t)j * nb;
             std::sort(order.begin(), order.end(), [&](int p, int q) { return e[p] > e[q]; });
-            for (int k = 0; k < std::min(nb, hc_.patches_per_column); ++k) {
+            // the highest-residual blocks, then as many blocks drawn at random from the rest (the
+            // network also sees where the residual, and so the correction, is small)
+            const int ntop = std::min(nb, (hc_.patches_per_column + 1) / 2);
+            const int nrand = std::min(nb - ntop, hc_.patches_per_column - ntop);
+            std::shuffle(order.begin() + ntop, order.end(), rng_);
+            for (int k = 0; k < ntop + nrand; ++k) {
                 if (ps_.count + (int64_t)desc.size() / 4 >= ps_.cap) break;
                 const int blk = order[k];
                 const int bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
@@ -329,7 +391,7 @@ class HarvestPreconditioner final : public Preconditioner {
         if (np == 0) return;
         double* cscale = a + 5 * mc;
         check(c_, dft_column_maxabs(c_, DFT_F64, n_, nc, R, ldr, scale), "dft_column_maxabs");
-        if (hc_.target_scale == 1) {
+        if (hc_.target_scale >= 1) {
             // cs = max|r| h2: the target keeps the input's scale (the correction operator stays linear)
             const double h2 = 1.0 / (1.0 / (g_.dx * g_.dx) + 1.0 / (g_.dy * g_.dy) + 1.0 / (g_.dz * g_.dz));
             std::vector<double> hs(nc);
@@ -356,6 +418,7 @@ class HarvestPreconditioner final : public Preconditioner {
     PatchSet& ps_;
     dft_cnn_harvest_config hc_;
     std::mt19937_64 rng_;
+    KineticPreconditioner kin_;
     int64_t calls_ = 0;
     Buf Cc_, Res_, Pd_, Ap_, small_, S_, E_;
     std::vector<double> S_h_, E_h_;

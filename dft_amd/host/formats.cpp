@@ -14,6 +14,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "capi_util.hpp"
 #include "dft_io.h"
@@ -141,6 +142,24 @@ uint32_t read_payload(std::FILE* f, void* dst, uint64_t nbytes, const char* what
         const size_t nb = (size_t)std::min<uint64_t>(kChunk, nbytes - off);
         if (std::fread(p + off, 1, nb, f) != nb) throw std::runtime_error(std::string(what) + ": file is truncated");
         crc = crc32_update(crc, p + off, nb);
+    }
+    return crc;
+}
+
+// Read one whole payload section of nbytes from the current position for its
+// CRC-32, keeping bytes [first, first + count) of it in dst: a range read is
+// verified like a whole read.
+uint32_t read_range_verified(std::FILE* f, uint64_t nbytes, uint64_t first, uint64_t count, void* dst,
+                             const char* what) {
+    std::vector<unsigned char> buf((size_t)std::min<uint64_t>(kChunk, std::max<uint64_t>(nbytes, 1)));
+    unsigned char* out = static_cast<unsigned char*>(dst);
+    uint32_t crc = 0;
+    for (uint64_t off = 0; off < nbytes; off += kChunk) {
+        const size_t nb = (size_t)std::min<uint64_t>(kChunk, nbytes - off);
+        if (std::fread(buf.data(), 1, nb, f) != nb) throw std::runtime_error(std::string(what) + ": file is truncated");
+        crc = crc32_update(crc, buf.data(), nb);
+        const uint64_t lo = std::max(off, first), hi = std::min(off + nb, first + count);
+        if (lo < hi) std::memcpy(out + (lo - first), buf.data() + (lo - off), (size_t)(hi - lo));
     }
     return crc;
 }
@@ -378,21 +397,24 @@ int dft_patch_shard_read(const char* path, int64_t first, int64_t count, float* 
         File f(path, "rb");
         if (!f.f) throw std::invalid_argument(std::string("dft_patch_shard_read: cannot open ") + path);
         const PatchHeader r = read_patch_header(f.f, path);
-        if (first + count > r.pub.count) throw std::invalid_argument(std::string("dft_patch_shard_read: range beyond the shard: ") + path);
+        if (first > r.pub.count || count > r.pub.count - first)
+            throw std::invalid_argument(std::string("dft_patch_shard_read: range beyond the shard: ") + path);
         if (file_size(f.f) != kPatchHeader + r.in_bytes + r.tgt_bytes)
             throw std::runtime_error(std::string("dft_patch_shard_read: file length disagrees with its header: ") + path);
         if (count == 0) return;
+        // every read covers its whole section for the checksum, so a range read is verified too
         const uint64_t p3 = (uint64_t)r.pub.patch * r.pub.patch * r.pub.patch;
-        const bool whole = first == 0 && count == r.pub.count;
         if (in) {
-            std::fseek(f.f, (long)(kPatchHeader + 2 * 4 * p3 * (uint64_t)first), SEEK_SET);
-            const uint32_t crc = read_payload(f.f, in, 2 * 4 * p3 * (uint64_t)count, "dft_patch_shard_read");
-            if (whole && crc != r.pub.crc_in) throw std::runtime_error(std::string("dft_patch_shard_read: input checksum mismatch: ") + path);
+            std::fseek(f.f, (long)kPatchHeader, SEEK_SET);
+            const uint32_t crc = read_range_verified(f.f, r.in_bytes, 2 * 4 * p3 * (uint64_t)first,
+                                                     2 * 4 * p3 * (uint64_t)count, in, "dft_patch_shard_read");
+            if (crc != r.pub.crc_in) throw std::runtime_error(std::string("dft_patch_shard_read: input checksum mismatch: ") + path);
         }
         if (tgt) {
-            std::fseek(f.f, (long)(kPatchHeader + r.in_bytes + 4 * p3 * (uint64_t)first), SEEK_SET);
-            const uint32_t crc = read_payload(f.f, tgt, 4 * p3 * (uint64_t)count, "dft_patch_shard_read");
-            if (whole && crc != r.pub.crc_tgt) throw std::runtime_error(std::string("dft_patch_shard_read: target checksum mismatch: ") + path);
+            std::fseek(f.f, (long)(kPatchHeader + r.in_bytes), SEEK_SET);
+            const uint32_t crc = read_range_verified(f.f, r.tgt_bytes, 4 * p3 * (uint64_t)first,
+                                                     4 * p3 * (uint64_t)count, tgt, "dft_patch_shard_read");
+            if (crc != r.pub.crc_tgt) throw std::runtime_error(std::string("dft_patch_shard_read: target checksum mismatch: ") + path);
         }
     });
 }

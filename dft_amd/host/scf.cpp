@@ -24,6 +24,8 @@ constexpr double kF32EnergyFloor = 3.0e-4, kF32DensityFloor = 1.0e-5;
 // ... or, below this relative density change, once three FP32 steps in a row
 // have not lowered it by 10% (the FP32 phase has reached its own noise floor)
 constexpr double kF32StallDensity = 1.0e-3;
+// grid points between the nuclei's bounding box and the faces of the learned correction's box
+constexpr int kCnnBoxMargin = 16;
 
 }  // namespace
 
@@ -255,8 +257,26 @@ struct ScfDriver::Fields {
     }
 };
 
+void nuclei_box(const dft_grid& g, int natoms, const double* nuclei, int margin, int64_t box[6]) {
+    const int64_t n[3] = {g.nx, g.ny, g.nz};
+    const double lo[3] = {g.xmin, g.ymin, g.zmin}, h[3] = {g.dx, g.dy, g.dz};
+    for (int a = 0; a < 3; ++a) {
+        double mn = 1e300, mx = -1e300;
+        for (int i = 0; i < natoms; ++i) {
+            mn = std::min(mn, nuclei[4 * i + 1 + a]);
+            mx = std::max(mx, nuclei[4 * i + 1 + a]);
+        }
+        int64_t i0 = natoms ? (int64_t)std::floor((mn - lo[a]) / h[a]) - margin : 0;
+        int64_t i1 = natoms ? (int64_t)std::ceil((mx - lo[a]) / h[a]) + margin + 1 : n[a];
+        i0 = std::max<int64_t>(0, std::min(i0, n[a] - 1));
+        i1 = std::max<int64_t>(i0 + 1, std::min(i1, n[a]));
+        box[a] = i0;
+        box[3 + a] = i1 - i0;
+    }
+}
+
 std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g,
-                                                    int sweeps, double shift, double shift_scale) {
+                                                    int sweeps, double shift, double shift_scale, const int64_t* box) {
     switch (kind) {
         case DFT_PRECOND_DIAG:
             return std::unique_ptr<Preconditioner>(new DiagonalPreconditioner());
@@ -282,7 +302,7 @@ std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const 
                 0, shift > 0.0 ? shift : scale > 0.0 ? 0.5 : 2.0, 2.0 / 3.0, KineticPreconditioner::kExact, scale));
         }
         case DFT_PRECOND_CNN: {
-            auto p = make_cnn_preconditioner(c, model, g);
+            auto p = make_cnn_preconditioner(c, model, g, box);
             if (!p) throw std::invalid_argument("learned preconditioner requested without a model");
             return p;
         }
@@ -340,9 +360,12 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         }
 
     std::unique_ptr<Preconditioner> own_precond;
-    if (!ext_precond_)
+    if (!ext_precond_) {
+        int64_t box[6];
+        nuclei_box(g_, natoms, nuc_.data(), kCnnBoxMargin, box);
         own_precond = make_preconditioner(c_, cfg_.precond, cfg_.cnn, g_, cfg_.precond_sweeps, cfg_.precond_shift,
-                                          cfg_.precond_shift_scale);
+                                          cfg_.precond_shift_scale, box);
+    }
     Preconditioner* precond = ext_precond_ ? ext_precond_ : own_precond.get();
 
     const bool mixed = cfg_.precision == DFT_SCF_MIXED;

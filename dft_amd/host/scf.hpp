@@ -41,10 +41,16 @@ double fermi_occupations(const double* eps, int n, double nelec, double kT, doub
 double fermi_entropy(const double* occ, int n);
 
 // Factory for the learned preconditioner (cnn.cpp); null when unavailable.
-std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g);
+// box (optional, {x0, y0, z0, bx, by, bz}): the region a base-1 model covers (null: the whole grid).
+std::unique_ptr<Preconditioner> make_cnn_preconditioner(dft_ctx* c, const void* model, const dft_grid& g,
+                                                        const int64_t* box = nullptr);
 // DFT_PRECOND_* -> preconditioner (throws invalid_argument for CNN without a model)
 std::unique_ptr<Preconditioner> make_preconditioner(dft_ctx* c, int kind, const void* model, const dft_grid& g,
-                                                    int sweeps = 0, double shift = 0.0, double shift_scale = 0.0);
+                                                    int sweeps = 0, double shift = 0.0, double shift_scale = 0.0,
+                                                    const int64_t* box = nullptr);
+// The learned correction's box for a molecule: the nuclei's bounding box widened by `margin` grid points on
+// every side, clipped to the grid ({x0, y0, z0, bx, by, bz}).
+void nuclei_box(const dft_grid& g, int natoms, const double* nuclei, int margin, int64_t box[6]);
 
 class ScfDriver {
    public:

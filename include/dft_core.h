@@ -157,6 +157,9 @@ typedef struct {
     int channels;          /* 16 */
     int64_t count;         /* dft_cnn_param_count(channels) */
     const float* params;   /* FP32 master parameters (host) */
+    int base;              /* 0: the network's output is the correction; 1: it is added, scaled by max|r| h2, to the
+                              exact kinetic correction (-1/2 Lap + sigma_j)^-1 r over the molecule's box (harvested
+                              with target_scale 2) */
 } dft_cnn_model;
 
 typedef struct {
@@ -167,7 +170,10 @@ typedef struct {
     double cg_tol;         /* relative residual of the projected CG */
     int cg_max_iter;
     uint64_t seed;         /* patch jitter */
-    int target_scale;      /* 0: target c / max|c|;  1: c / (max|r| h2), the input's scale */
+    int target_scale;      /* 0: target c / max|c|;  1: c / (max|r| h2), the input's scale;
+                              2: the Davidson runs with the exact kinetic correction t_kin (per-state shifts) and the
+                              target is (beta c - t_kin) / (max|r| h2), beta = <t_kin, c> / <c, c>: what the network
+                              must add to t_kin (a model with base 1) */
 } dft_cnn_harvest_config;
 
 typedef struct dft_cnn_trainer dft_cnn_trainer;

@@ -317,6 +317,15 @@ int dft_cnn_prepare(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff,
                     const double* theta_dev, const double* scale_dev, float* in);
 int dft_cnn_finish(dft_ctx* ctx, const dft_grid* g, int prec, int ncols, const float* out, const double* scale_dev,
                    void* t, int64_t ldt);
+/* The same two maps on a sub-box of the grid, box = {x0, y0, z0, bx, by, bz}
+ * (host array, inside the grid): in [ncols][2][bx by bz] from the box, and
+ * t[box] += out s h2 w, with w a linear ramp over `taper` points at each box
+ * face that lies inside the grid (the learned correction added to a base
+ * correction over the region the network covers). */
+int dft_cnn_prepare_box(dft_ctx* ctx, const dft_grid* g, int prec, const void* veff, const void* r, int64_t ldr,
+                        int ncols, const double* theta_dev, const double* scale_dev, const int64_t* box, float* in);
+int dft_cnn_add_box(dft_ctx* ctx, const dft_grid* g, int prec, int ncols, const float* out, const double* scale_dev,
+                    const int64_t* box, int taper, void* t, int64_t ldt);
 
 /* ---- training-data harvest (projected CG on (H - theta) c = -r) ---- */
 /* out_j = x_j . y_j (FP64, device) */

@@ -541,6 +541,29 @@ int og_integrate_density(long n, const double* rho, double dx, double dy, double
  * ones (MolecularSCFIteration.cpp:146-161), summed with one accumulator. */
 static double kinetic_accumulate(const og_grid* g, int norb, const double* psi, const int* electrons, double e);
 
+int og_ks_residual(const og_grid* g, const double* va, const double* vb, int norb, const double* psi,
+                   const double* eig, const int* spins, double* out) {
+    /* MolecularSCFHelpers.cpp:119-183 */
+    const long n = gsize(g);
+    const double dv = volume_element(g);
+    double* hp = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!hp) return fail(-2, "out of memory");
+    double worst = 0.0;
+    for (int o = 0; o < norb; ++o) {
+        const double* p = psi + (size_t)o * (size_t)n;
+        og_hpsi(g, spins[o] == 0 ? va : vb, p, hp);
+        double r2 = 0.0;
+        for (long i = 0; i < n; ++i) {
+            const double r = hp[i] - eig[o] * p[i];
+            r2 += r * r;
+        }
+        worst = dmax(worst, sqrt(r2 * dv));
+    }
+    free(hp);
+    *out = worst;
+    return 0;
+}
+
 double og_kinetic_energy(const og_grid* g, int norb, const double* psi, const int* electrons) {
     return kinetic_accumulate(g, norb, psi, electrons, 0.0);
 }

@@ -71,6 +71,10 @@ int og_electron_density(long n, int norb, const double* psi, const int* electron
 /* ElectronDensityMolecule.cpp:116-140: sum(rho) dx dy dz. */
 int og_integrate_density(long n, const double* rho, double dx, double dy, double dz, double* electrons);
 
+/* MolecularSCFHelpers.cpp:119-183: max over the orbitals of ||H psi - eps psi|| sqrt(dV), H built
+ * on va for spins[o] == 0 and on vb otherwise; orbitals column-major, N each. */
+int og_ks_residual(const og_grid* g, const double* va, const double* vb, int norb, const double* psi,
+                   const double* eig, const int* spins, double* out);
 /* TotalEnergyMolecule.cpp:13-199 (Neumann-like stencil) */
 double og_kinetic_energy(const og_grid* g, int norb, const double* psi, const int* electrons);
 /* TotalEnergyMolecule.cpp:201-235 */

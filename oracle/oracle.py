@@ -173,6 +173,16 @@ class Oracle:
                                               C.byref(e)))
         return e.value
 
+    def ks_residual(self, g, va, vb, psi, eig, spins):
+        """calculateMaximumMolecularKSResidual (MolecularSCFHelpers.cpp:119-183); spins[o] 0 alpha, 1 beta."""
+        psi = _f64(psi).reshape(-1, g.size)
+        sp = np.ascontiguousarray(spins, dtype=np.int32)
+        out = C.c_double()
+        self._chk(self.L.og_ks_residual(C.byref(self._g(g)), _d(_f64(va, g.size)), _d(_f64(vb, g.size)),
+                                        C.c_int(psi.shape[0]), _d(psi), _d(_f64(eig, psi.shape[0])),
+                                        sp.ctypes.data_as(_ip), C.byref(out)))
+        return out.value
+
     def kinetic_energy(self, g, psi, electrons):
         psi = _f64(psi).reshape(-1, g.size)
         el = np.ascontiguousarray(electrons, dtype=np.int32)
@@ -322,6 +332,14 @@ class Reference:
         self._chk(self.L.rf_integrate_density(C.c_long(r.size), _d(r), C.c_double(dx), C.c_double(dy), C.c_double(dz),
                                               C.byref(e)))
         return e.value
+
+    def ks_residual(self, g, va, vb, psi, eig, spins):
+        psi = _f64(psi).reshape(-1, g.size)
+        sp = np.ascontiguousarray(spins, dtype=np.int32)
+        out = C.c_double()
+        self._chk(self.L.rf_ks_residual(*self._g(g), _d(_f64(va, g.size)), _d(_f64(vb, g.size)), C.c_int(psi.shape[0]),
+                                        _d(psi), _d(_f64(eig, psi.shape[0])), sp.ctypes.data_as(_ip), C.byref(out)))
+        return out.value
 
     def kinetic_energy(self, g, psi, electrons):
         psi = _f64(psi).reshape(-1, g.size)

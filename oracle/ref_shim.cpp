@@ -11,6 +11,7 @@
 #include "ExchangeCorrelationMolecule.h"
 #include "HartreePotentialMolecule.h"
 #include "KohnShamHamiltonianMolecule.h"
+#include "MolecularSCFHelpers.h"
 #include "MolecularSCFIteration.h"
 #include "Molecule.h"
 #include "NuclearPotential.h"
@@ -225,6 +226,23 @@ int rf_davidson(const long* n, const double* b, const double* v, int norb, const
 // the seed vectors (no previous orbitals).  Outputs as og_scf_step: vh, vxa,
 // vxb, ora, orb, ovh (N each), eig (alpha then beta), E[6], dd (effective
 // density difference), ks (max KS residual).
+// calculateMaximumMolecularKSResidual (MolecularSCFHelpers.cpp:119-183); spins[o]: 0 alpha, 1 beta
+int rf_ks_residual(const long* n, const double* b, const double* va, const double* vb, int norb, const double* psi,
+                   const double* eig, const int* spins, double* out) {
+    return guarded([&] {
+        auto g = make_grid(n, b);
+        const size_t N = g.getSize();
+        std::vector<MolecularOrbital> orbs((size_t)norb);
+        for (int o = 0; o < norb; ++o) {
+            orbs[o].spin = spins[o] ? SpinChannel::Beta : SpinChannel::Alpha;
+            orbs[o].electrons = 1;
+            orbs[o].eigenvalue = eig[o];
+            orbs[o].psi = vec(psi + (size_t)o * N, N);
+        }
+        *out = calculateMaximumMolecularKSResidual(g, vec(va, N), vec(vb, N), orbs);
+    });
+}
+
 int rf_scf_step(const long* n, const double* b, int natoms, const double* nuc, int id, int mult, const double* ra,
                 const double* rb, double* vh, double* vxa, double* vxb, double* ora, double* orb, double* ovh,
                 double* eig, double* E, double* dd, double* ks) {

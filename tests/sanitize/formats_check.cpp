@@ -135,6 +135,18 @@ int main(int argc, char** argv) {
             CHECK(rc == DFT_OK || rc == DFT_EINVAL || rc == DFT_ENUMERIC);
         }
     }
+    // ranges: huge first/count values are rejected (no overflow); a corrupted byte is detected by a
+    // range read that does not cover it
+    {
+        const int64_t big = INT64_MAX - 1;
+        CHECK(dft_patch_shard_read(spath.c_str(), big, 2, in2.data(), tg2.data()) == DFT_EINVAL);
+        CHECK(dft_patch_shard_read(spath.c_str(), 1, big, in2.data(), tg2.data()) == DFT_EINVAL);
+        CHECK(dft_patch_shard_read(spath.c_str(), 0, 1, in2.data(), tg2.data()) == DFT_OK);
+        std::vector<unsigned char> b = sgood;
+        b[b.size() - 1] ^= 0x40;  // the last target value
+        spit(sbad, b, b.size());
+        CHECK(dft_patch_shard_read(sbad.c_str(), 0, 1, nullptr, tg2.data()) == DFT_ENUMERIC);
+    }
     // the result document: random values (non-finite ones included), enums out of range
     for (int t = 0; t < 500; ++t) {
         dft_scf_config cfg{};

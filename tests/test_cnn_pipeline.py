@@ -102,8 +102,8 @@ class _FakeTrainer:
     def count(self):
         return len(self.tgt)
 
-    def save_patches(self, path, h=0.0):
-        dft_amd.save_patch_shard(path, self.xin, self.tgt, h=h)
+    def save_patches(self, path, target_scale=0, h=0.0):
+        dft_amd.save_patch_shard(path, self.xin, self.tgt, target_scale=target_scale, h=h)
 
     def load_patches(self, path):
         xin, tgt, hdr = dft_amd.load_patch_shard(path)
@@ -118,8 +118,8 @@ class _FakeTrainer:
         self.calls.append(("eval", first, count))
         return 0.25
 
-    def model(self):
-        return dft_amd.CnnModel(np.zeros(dft_amd.cnn_param_count(), np.float32))
+    def model(self, base=0):
+        return dft_amd.CnnModel(np.zeros(dft_amd.cnn_param_count(), np.float32), base=base)
 
     def close(self):
         pass
@@ -175,3 +175,43 @@ def test_evaluate_compares_diagonal_and_learned(tmp_path, monkeypatch):
     assert s["molecules"][0]["happly"] == [400, 300] and s["molecules"][0]["time_gain"] == pytest.approx(1.5)
     assert s["max_abs_energy_difference"] == pytest.approx(1e-9)
     assert (tmp_path / "eval.json").exists() and (tmp_path / "eval_seed01001_learned.json").exists()
+
+
+def test_harvest_refuses_to_resume_with_other_settings(tmp_path, monkeypatch):
+    """A resumed harvest must use the grid, spacing, patch edge, functional and base it started with;
+    shards of different spacings or targets never train together."""
+    monkeypatch.setattr(cp, "CnnTrainer", _FakeTrainer)
+    monkeypatch.setattr(cp, "Grid", type("G", (), {"centered": staticmethod(lambda n, h: (n, h))}))
+    cp.harvest(tmp_path, [1], n=16, h=0.5, patch=4, target_patches=10)
+    for kw in (dict(h=0.4), dict(n=20), dict(base="diagonal")):
+        args = dict(n=16, h=0.5, patch=4, target_patches=10)
+        args.update(kw)
+        with pytest.raises(ValueError, match="harvested with"):
+            cp.harvest(tmp_path, [2], **args)
+    # a shard of another spacing dropped into the directory
+    xin = np.zeros((2, 2, 64), np.float32)
+    dft_amd.save_patch_shard(cp.shard_path(tmp_path, 7), xin, np.zeros((2, 64), np.float32), target_scale=2, h=0.3)
+    with pytest.raises(ValueError, match="spacings"):
+        cp.shard_plan(tmp_path)
+
+
+def test_kinetic_base_harvest_trains_a_base_one_model(tmp_path, monkeypatch):
+    """base "kinetic": shards carry target_scale 2 and the trained model adds to the kinetic correction;
+    evaluation compares it with the exact kinetic correction."""
+    monkeypatch.setattr(cp, "CnnTrainer", _FakeTrainer)
+    monkeypatch.setattr(cp, "Grid", type("G", (), {"centered": staticmethod(lambda n, h: (n, h))}))
+    cp.harvest(tmp_path, [1, 2], n=16, h=0.5, patch=4, target_patches=20, base="kinetic")
+    assert cp.patch_shard_info(cp.shard_path(tmp_path, 1))["target_scale"] == 2
+    model, _ = cp.train(tmp_path, steps=10, batch=2, chunk=10)
+    assert model.base == dft_amd.CNN_BASE_KINETIC
+    runs = []
+
+    def fake_scf(grid, atoms, precond=None, cnn=None, **kw):
+        runs.append(precond)
+        return dict(converged=True, happly_columns=100, davidson_iterations=10, seconds_total=1.0, cnn_applied=0,
+                    cnn_fallbacks=0, total=-1.0)
+
+    monkeypatch.setattr(cp, "scf_run", fake_scf)
+    monkeypatch.setattr(cp, "scf_result_json", lambda g, a, r: "{}")
+    cp.evaluate(tmp_path, [1001], n=16, h=0.5)
+    assert runs == [dft_amd.PRECOND_KINETIC_EXACT, dft_amd.PRECOND_CNN]

@@ -182,3 +182,71 @@ def test_harvest_kernels(dev):
     with pytest.raises(dft_amd.DftError):
         dev.extract_patches(g, P, np.array([[4, 0, 0, 0]]), 4, dev.array(V), dX, dev.array(Cm), dev.array(theta),
                             dev.array(sc), dev.array(cs), xin, tgt)
+
+
+def _ramp(l, b, lo_in, hi_in, taper):
+    w = np.ones(b)
+    idx = np.arange(b)
+    if lo_in:
+        w = np.minimum(w, (idx + 1) / (taper + 1))
+    if hi_in:
+        w = np.minimum(w, (b - idx) / (taper + 1))
+    return w
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_cnn_box_maps(dt):
+    """dft_cnn_prepare_box / dft_cnn_add_box against numpy: the inputs cut from a sub-box, and
+    t[box] += out s h2 w with the face ramps (a box touching the grid's x faces has no ramp there)."""
+    import dft_amd
+    from oracle.oracle import Grid
+    og = Grid(20, 18, 16, -2.0, 2.1, -1.9, 1.6, -1.7, 1.4)
+    g = dft_amd.Grid.make(*og.n, *og.bounds)
+    dev = dft_amd.Device(0)
+    rng = np.random.default_rng(4)
+    n, nc = og.size, 3
+    V = rng.standard_normal(n)
+    R = rng.standard_normal((nc, n))
+    th = rng.standard_normal(nc)
+    sc = np.abs(R).max(axis=1)
+    box = [0, 3, 2, 20, 11, 9]  # x spans the grid; y, z inside
+    bx, by, bz = box[3:]
+    nb = bx * by * bz
+    xin = dev.array(shape=(nc, 2, nb), dtype=np.float32)
+    dev.cnn_prepare_box(g, dev.array(V, dtype=dt), dev.array(R, dtype=dt), nc, dev.array(th), dev.array(sc), box, xin)
+    Vb = V.astype(dt).reshape(og.shape)[2:2 + bz, 3:3 + by, 0:bx].ravel()
+    Rb = R.astype(dt).reshape(nc, *og.shape)[:, 2:2 + bz, 3:3 + by, 0:bx].reshape(nc, -1)
+    h2 = 1.0 / (1 / og.dx ** 2 + 1 / og.dy ** 2 + 1 / og.dz ** 2)
+    got = xin.download()
+    np.testing.assert_allclose(got[:, 0], Rb / sc[:, None], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(got[:, 1], (Vb[None, :] - th[:, None]) * h2, rtol=1e-6, atol=1e-6)
+    out = rng.standard_normal((nc, nb)).astype(np.float32)
+    T0 = rng.standard_normal((nc, n))
+    t = dev.array(T0, dtype=dt)
+    dev.cnn_add_box(g, nc, dev.array(out, dtype=np.float32), dev.array(sc), box, 3, t)
+    w = (_ramp(0, bz, True, True, 3)[:, None, None] * _ramp(0, by, True, True, 3)[None, :, None] *
+         _ramp(0, bx, False, False, 3)[None, None, :]).ravel()
+    ref = T0.astype(dt).astype(np.float64).reshape(nc, *og.shape).copy()
+    add = (out.astype(np.float64) * sc[:, None] * h2 * w[None, :]).reshape(nc, bz, by, bx)
+    ref[:, 2:2 + bz, 3:3 + by, 0:bx] += add
+    tol = 1e-12 if dt == np.float64 else 1e-5
+    np.testing.assert_allclose(t.download().astype(np.float64), ref.reshape(nc, n), rtol=tol, atol=tol)
+    dev.close()
+
+
+def test_zero_network_on_the_kinetic_base():
+    """A base-1 model with all-zero parameters adds nothing to the exact kinetic correction: the
+    Davidson reaches the kinetic correction's eigenvalues, with the network applied and the diagonal
+    fallback used only for the columns whose residual did not go down."""
+    import dft_amd
+    from oracle.oracle import Grid, Oracle
+    og = Grid.centered(40, 0.3)
+    g = dft_amd.Grid.make(*og.n, *og.bounds)
+    nuc = [[6, 0.05, 0.02, -0.03], [1, 2.0, 0.1, 0.0]]
+    v = Oracle().nuclear_potential(og, nuc)
+    zero = dft_amd.CnnModel(np.zeros(dft_amd.cnn_param_count(), np.float32), base=dft_amd.CNN_BASE_KINETIC)
+    rk = dft_amd.davidson_run(g, v, 6, nconv=4, tol=1e-7, max_iter=400, precond=dft_amd.PRECOND_KINETIC_EXACT)
+    rz = dft_amd.davidson_run(g, v, 6, nconv=4, tol=1e-7, max_iter=400, cnn=zero)
+    print("zero-net", rk["happly_columns"], rz["happly_columns"], rz["cnn_applied"], rz["cnn_fallbacks"])
+    assert np.max(np.abs(rz["eigenvalues"][:4] - rk["eigenvalues"][:4])) < 1e-8
+    assert rz["cnn_applied"] > 0

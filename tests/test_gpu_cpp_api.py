@@ -19,15 +19,14 @@ def rel(a, b):
 
 def test_reference_cpp_api_against_oracle(tmp_path, oracle):
     """The program runs while this process holds a device context that has
-    made a cooperative (multi-sweep) Jacobi launch: the device's cooperative
-    queue can then be unavailable to the child, whose solves must fall back to
-    one launch per sweep and still match the oracle."""
+    just run a Jacobi solve on the same GPU; the child's own context and
+    solves must be unaffected and match the oracle."""
     import dft_amd
     from oracle.oracle import Grid
     exe = os.path.join(ROOT, "tests", "cpp", "bin", "molecule_api_check")  # built by __graft_entry__.build()
     assert os.path.exists(exe), "tests/cpp/bin/molecule_api_check is not built"
     dev = dft_amd.Device(0)
-    gs = dft_amd.Grid.centered(20, 0.4)  # few tiles: the cooperative kernel applies
+    gs = dft_amd.Grid.centered(20, 0.4)
     dev.poisson(gs, dev.array(np.full(gs.size, 1e-3)), dev.zeros((gs.size,)), max_sweeps=300, tol=0.0,
                 zero_start=True, batch=100)
     dev.sync()

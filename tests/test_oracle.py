@@ -460,3 +460,32 @@ def test_xc_potential_flux_against_energy_derivative(oracle, fid):
         fdb = (oracle.xc_energy(g, fid, na, nb + e * dn) - oracle.xc_energy(g, fid, na, nb - e * dn)) / (2 * e)
         lin_b = np.sum(oracle.xc_potential(g, fid, na, nb, 1, consistent=True) * dn) * dV
         assert abs(lin_b - fdb) < 1e-6 * abs(fdb)
+
+
+@needs_ref
+def test_ks_residual_against_live_reference(oracle):
+    """og_ks_residual (the KS residual's checker) equals the reference's calculateMaximumMolecularKSResidual
+    (MolecularSCFHelpers.cpp:119-183) bit for bit, on random orbitals of both spins; and on exact discrete
+    sine-mode eigenvectors of a constant potential it is zero to rounding (the closed form)."""
+    R = Reference()
+    g = Grid(11, 9, 10, -2.1, 2.4, -1.8, 1.9, -2.2, 2.0)
+    rng = np.random.default_rng(17)
+    va, vb = rng.standard_normal(g.size), rng.standard_normal(g.size)
+    psi = rng.standard_normal((5, g.size))
+    eig = rng.standard_normal(5)
+    spins = np.array([0, 1, 0, 1, 1], dtype=np.int32)
+    assert oracle.ks_residual(g, va, vb, psi, eig, spins) == R.ks_residual(g, va, vb, psi, eig, spins)
+    # sine modes: H u = (lambda + c) u exactly
+    nx, ny, nz = g.n
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    modes, lam = [], []
+    for (p, q, r) in ((1, 1, 1), (2, 1, 3)):
+        u = (np.sin(np.pi * p * (x + 1) / (nx + 1)) * np.sin(np.pi * q * (y + 1) / (ny + 1)) *
+             np.sin(np.pi * r * (z + 1) / (nz + 1))).ravel()
+        modes.append(u)
+        lam.append(sum((1 - np.cos(np.pi * m / (nn + 1))) / d ** 2
+                       for m, nn, d in ((p, nx, g.dx), (q, ny, g.dy), (r, nz, g.dz))))
+    c = 0.3
+    vc = np.full(g.size, c)
+    ks = oracle.ks_residual(g, vc, vc, np.array(modes), np.array(lam) + c, np.array([0, 1], dtype=np.int32))
+    assert ks < 1e-12
