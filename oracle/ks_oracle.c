@@ -475,6 +475,24 @@ int og_xc_potential_flux(const og_grid* g, int functional, const double* na, con
     return 0;
 }
 
+int og_xc_energy_terms(const og_grid* g, int functional, const double* na, const double* nb, double* terms) {
+    /* the summands of ExchangeCorrelationMolecule.cpp:749-842, one per point */
+    if (functional != 1 && functional != 2) return fail(-1, "unknown functional");
+    const double dv = volume_element(g);
+    for (long k = 0; k < g->nz; ++k)
+        for (long j = 0; j < g->ny; ++j)
+            for (long i = 0; i < g->nx; ++i) {
+                const long c = IDX(g, i, j, k);
+                const double n = na[c] + nb[c];
+                terms[c] = 0.0;
+                if (n <= RHO_FLOOR) continue;
+                double in[4] = {na[c], nb[c], grad_mag(g, na, i, j, k), grad_mag(g, nb, i, j, k)}, out[5];
+                og_xc_point(functional, in, out);
+                terms[c] = n * out[0] * dv;
+            }
+    return 0;
+}
+
 double og_xc_energy(const og_grid* g, int functional, const double* na, const double* nb) {
     /* ExchangeCorrelationMolecule.cpp:749-842 */
     const double dv = volume_element(g);

@@ -61,6 +61,9 @@ int og_xc_potential(const og_grid* g, int functional, const double* na,
  * (build option; no counterpart in the reference). */
 int og_xc_potential_flux(const og_grid* g, int functional, const double* na,
                          const double* nb, int spin, int consistent, double* v);
+/* The per-point terms n eps_xc dV of og_xc_energy (0 at or below the floor), so that a test can sum
+ * them exactly (math.fsum) instead of in the reference's sequential order. */
+int og_xc_energy_terms(const og_grid* g, int functional, const double* na, const double* nb, double* terms);
 /* ExchangeCorrelationMolecule.cpp:749-842 */
 double og_xc_energy(const og_grid* g, int functional, const double* na, const double* nb);
 

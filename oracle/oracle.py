@@ -145,6 +145,13 @@ class Oracle:
                                               _d(out)))
         return out
 
+    def xc_energy_terms(self, g, fid, na, nb):
+        """The per-point summands n eps dV of xc_energy."""
+        out = np.empty(g.size)
+        self._chk(self.L.og_xc_energy_terms(C.byref(self._g(g)), C.c_int(fid), _d(_f64(na, g.size)),
+                                            _d(_f64(nb, g.size)), _d(out)))
+        return out
+
     def xc_energy(self, g, fid, na, nb):
         return self.L.og_xc_energy(C.byref(self._g(g)), C.c_int(fid), _d(_f64(na, g.size)), _d(_f64(nb, g.size)))
 

@@ -489,3 +489,22 @@ def test_ks_residual_against_live_reference(oracle):
     vc = np.full(g.size, c)
     ks = oracle.ks_residual(g, vc, vc, np.array(modes), np.array(lam) + c, np.array([0, 1], dtype=np.int32))
     assert ks < 1e-12
+
+
+@pytest.mark.parametrize("fid", [1, 2])
+def test_xc_energy_terms_sum_to_the_energy(oracle, fid):
+    """og_xc_energy_terms are the summands of og_xc_energy: their sum in index order is the energy bit
+    for bit, and the exactly rounded sum (math.fsum) differs by less than the sequential bound."""
+    import math
+    g = Grid(17, 15, 13, -2.0, 2.2, -1.9, 1.7, -1.6, 1.5)
+    xs, ys, zs = g.coords()
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    r2 = (X ** 2 + Y ** 2 + Z ** 2).ravel()
+    na, nb = 0.7 * np.exp(-1.3 * r2), 0.4 * np.exp(-0.9 * r2)
+    t = oracle.xc_energy_terms(g, fid, na, nb)
+    s = 0.0
+    for v in t:
+        s += v
+    e = oracle.xc_energy(g, fid, na, nb)
+    assert s == e
+    assert abs(math.fsum(t) - e) <= (t.size - 1) * 2.0 ** -53 * np.sum(np.abs(t))
