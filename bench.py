@@ -28,6 +28,8 @@ from dft_amd.geometry import benzene, random_cluster  # noqa: E402  (workload ge
 
 METRIC = "SCF time to |dE|<1e-5 Ha: mixed+learned vs plain FP64 on 1 GPU; H·psi GB/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+DEFAULT_MODEL = os.path.join(ROOT, "dft_amd", "data", "cnn_kinetic_c4.npz")
 PROFILES = os.path.join(ROOT, "profiles")
 
 
@@ -111,6 +113,49 @@ def time_hpsi(dft_amd, grid, nb, prec, reps=20):
     bytes_alg = n * nb * (2 * es + es / nb)  # read psi, write H psi, V_eff once per block
     dev.close()
     return ms, bytes_alg
+
+
+def time_dense(dft_amd, grid, w, reps=5):
+    """The Davidson's dense FP32 products at the workload's subspace limit (m = 3 x block, k = block),
+    timed with HIP events on the library stream: the split-BF16 Gram V^T X (three MFMA products per
+    tile), the block update V C and the fused Ritz residual [W | V] [Q ; -Q theta] with its norms (bf16
+    three-piece split, six MFMA products per tile).  Reported: time, HBM rate of the operands moved,
+    useful and executed MFMA rates, and the executed rate as a fraction of the dense BF16 peak."""
+    dev = dft_amd.Device(0)
+    n, k = grid.size, w["nstates"]
+    m = 3 * k
+    V = dev.array(shape=(m, n), dtype=np.float32)
+    dev.seed_vectors(grid, V, 0, m)
+    X = dev.array(shape=(k, n), dtype=np.float32)
+    dev.seed_vectors(grid, X, 0, k)
+    G = dev.zeros((k, m))
+    C = dev.array(np.random.default_rng(2).standard_normal((k, m)) / m)
+    theta = dev.array(np.linspace(-1.0, 1.0, k))
+    rn = dev.zeros((k,))
+    rows = {}
+
+    def timed(fn):
+        fn()
+        dev.sync()
+        e0, e1 = dev.event(), dev.event()
+        dev.record(e0)
+        for _ in range(reps):
+            fn()
+        dev.record(e1)
+        return dev.elapsed_ms(e0, e1) / reps
+
+    for name, fn, byts, flops, mult in (
+            ("gram_split_bf16", lambda: dev.gram(dft_amd.GRAM_SPLIT_BF16, V, X, G, n, m, k), 4.0 * n * (m + k),
+             2.0 * n * m * k, 3),
+            ("update_bf16x6", lambda: dev.block_update(V, C, X, n, m, k), 4.0 * n * (m + k), 2.0 * n * m * k, 6),
+            ("ritz_residual_bf16x6", lambda: dev.ritz_residual(V, V, C, theta, X, rn, n, m, k),
+             4.0 * n * (2 * m + k), 4.0 * n * m * k, 6)):
+        ms = timed(fn)
+        rows[name] = dict(shape=f"{n} x {m} x {k}", ms=ms, GBps=byts / ms / 1e6, hbm_frac=byts / ms / 1e6 / HBM_PEAK_GBS,
+                          useful_TFLOPs=flops / ms / 1e9, executed_TFLOPs=mult * flops / ms / 1e9,
+                          mfma_frac=mult * flops / ms / 1e9 / BF16_PEAK_TFLOPS)
+    dev.close()
+    return rows
 
 
 def kernel_table(dft_amd, grid, w, nb, reps=10):
@@ -263,7 +308,52 @@ def cpu_baseline(w, atoms, grid_n, h, rep, nocc):
               f"32^3 and 40^3 -> ~n^{pw:.1f}, {t_bound:.0f} s at {grid_n}^3, plus 10000 sweeps per call; "
               f"XC at 48^3 scaled ({t_xc:.2f} s); single-vector Davidson at 24^3 ({t_dav * 1e3:.0f} ms per H apply "
               f"scaled); extrapolated to {steps} SCF steps x (2 Poisson + 2 XC) + {rep['happly_columns']} H applies")
-    return dict(value=float(est), unit="s", cores=1, kind=kind, sample=sample)
+    return dict(value=float(est), unit="s", cores=1, kind=kind, sample=sample, cpu_model=cpu_model(),
+                nproc=os.cpu_count())
+
+
+def learned_sample(dft_amd, model_path, seed=1003, n=128, h=0.2):
+    """The learned correction against the exact kinetic one it builds on, on a bounded sample: one
+    Davidson solve (FP64, block N/2 + 2, the lowest N/2 states to residual 1e-6) on the bare-nucleus
+    potential of the held-out molecule `seed` of the training set's definition, at the training
+    grid.  Reports H applications, Davidson iterations and seconds of each, and their ratios."""
+    from dft_amd.geometry import training_molecule
+    model = dft_amd.CnnModel.load(model_path)
+    grid = dft_amd.Grid.centered(n, h)
+    atoms = np.array(training_molecule(seed), dtype=float)
+    ne = int(round(atoms[:, 0].sum()))
+    dev = dft_amd.Device(0)
+    v = dev.zeros((grid.size,))
+    dev.nuclear_potential(grid, atoms, v)
+    vh = v.download()
+    dev.close()
+    k, nconv = ne // 2 + 2, ne // 2
+    out = {}
+    for name, kw in (("kinetic_exact", dict(precond=dft_amd.PRECOND_KINETIC_EXACT, max_iter=2000)),
+                     ("learned", dict(cnn=model, max_iter=6000))):
+        t = time.perf_counter()
+        r = dft_amd.davidson_run(grid, vh, k, nconv=nconv, tol=1e-6, **kw)
+        out[name] = dict(seconds=time.perf_counter() - t, happly=int(r["happly_columns"]),
+                         iterations=int(r["iterations"]), cnn_applied=int(r.get("cnn_applied", 0)),
+                         cnn_fallbacks=int(r.get("cnn_fallbacks", 0)), eig0=float(r["eigenvalues"][0]))
+    out["happly_gain"] = out["kinetic_exact"]["happly"] / max(out["learned"]["happly"], 1)
+    out["time_gain"] = out["kinetic_exact"]["seconds"] / out["learned"]["seconds"]
+    out["sample"] = (f"held-out molecule seed {seed} ({len(atoms)} atoms, {ne} electrons), {n}^3, h={h}, bare-nucleus "
+                     f"potential, block {k}, {nconv} states to residual 1e-6, model {os.path.relpath(model_path, ROOT)}")
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's processor string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 def main():
@@ -271,14 +361,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default=os.environ.get("DFT_BENCH_WORKLOAD", "c3"), choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=os.environ.get("DFT_BENCH_WORKLOAD", "c5"), choices=sorted(WORKLOADS))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--hpsi-only", type=int, default=0, help="only run N H psi launches (profiling)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--kernels", action="store_true", help="only time the hot kernels at the workload size")
-    ap.add_argument("--precond", default="kinetic-exact", choices=["diag", "kinetic", "kinetic-cheb", "kinetic-exact", "cnn"],
-                    help="Davidson correction of the timed (mixed) run; the FP64 run uses the diagonal one")
-    ap.add_argument("--cnn-model", default=None, help="trained network (.npz) for --precond cnn")
+    ap.add_argument("--precond", default="kinetic-exact",
+                    choices=["diag", "kinetic", "kinetic-cheb", "kinetic-exact", "cnn"],
+                    help="Davidson correction of the timed (mixed) run (cnn: the learned one on the exact kinetic "
+                         "base, diagonal fallback); the plain FP64 run uses the diagonal one")
+    ap.add_argument("--learned-sample", type=int, default=1,
+                    help="1: also time the learned correction against its exact kinetic base on a bounded sample "
+                         "(one Davidson solve of a held-out 128^3 molecule); 0: skip")
+    ap.add_argument("--cnn-model", default=DEFAULT_MODEL, help="trained network (.npz) for --precond cnn")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="steps of the plain FP64 run (diagonal correction) timed and scaled to the FP64 reference's "
+                         "step count; -1: run it to convergence; 0: skip")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the timed mixed SCF (profiling): no FP64 runs, no CPU baseline")
     ap.add_argument("--profile-scf", type=int, default=0,
@@ -337,7 +435,8 @@ def main():
         return
     ne = int(round(atoms[:, 0].sum()))
     nocc = ne // 2
-    print(f"[bench] workload {a.workload}: {w['desc']}; {len(atoms)} atoms, {ne} electrons", file=sys.stderr, flush=True)
+    print(f"[bench] workload {a.workload}: {w['desc']}; {len(atoms)} atoms, {ne} electrons; timed correction "
+          f"{a.precond}", file=sys.stderr, flush=True)
     for i in range(a.warmup):  # loads the code objects (rocBLAS kernels included) the timed run uses
         dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=dft_amd.SCF_MIXED, nstates=w["nstates"],
                         max_iterations=2, poisson_batch=64, **poisson_args(dft_amd, w), stop_rule=dft_amd.STOP_DE,
@@ -350,61 +449,90 @@ def main():
         print(json.dumps({"mixed_seconds": time.perf_counter() - t, "energy": r["total"], "steps": r["iterations"],
                           "refine_steps": r["refine_iterations"], "davidson_iterations": r["davidson_iterations"],
                           "block_passes": r["block_passes"], "block_fallbacks": r["block_fallbacks"],
-                          "happly": r["happly_columns"],
+                          "happly": r["happly_columns"], "cnn_applied": r["cnn_applied"],
+                          "cnn_fallbacks": r["cnn_fallbacks"],
                           "phase": {k: r[f"seconds_{k}"] for k in ("poisson", "xc", "davidson", "other")}}))
         return
-    # plain FP64 GPU path (the accuracy reference and the speedup denominator)
-    t = time.perf_counter()
-    r64 = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, verbose=a.verbose)
-    t64 = time.perf_counter() - t
-    print(f"[bench] fp64 SCF (diagonal correction): E={r64['total']:.10f} steps={r64['iterations']} {t64:.2f} s",
-          file=sys.stderr, flush=True)
-    t64p, r64p = None, None
-    if pc != dft_amd.PRECOND_DIAG:
-        # the same correction in FP64: the precision-only speed-up
-        t = time.perf_counter()
-        r64p = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, verbose=a.verbose, precond=pc, cnn=model)
-        t64p = time.perf_counter() - t
-        print(f"[bench] fp64 SCF ({a.precond}): E={r64p['total']:.10f} steps={r64p['iterations']} {t64p:.2f} s",
-              file=sys.stderr, flush=True)
-    # timed region: K mixed-precision solves
+    # ---- timed region: K mixed-precision solves with the timed correction
     runs = []
     t0 = time.perf_counter()
     for k in range(a.steps):
         runs.append(run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_MIXED, verbose=a.verbose, precond=pc, cnn=model))
         print(f"[bench] mixed SCF {k + 1}: E={runs[-1]['total']:.10f} steps={runs[-1]['iterations']} "
-              f"(refine {runs[-1]['refine_iterations']})", file=sys.stderr, flush=True)
+              f"(refine {runs[-1]['refine_iterations']}), learned {runs[-1]['cnn_applied']} / fallbacks "
+              f"{runs[-1]['cnn_fallbacks']}", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     per = (t1 - t0) / a.steps
     rm = runs[-1]
-    # H psi roofline at the workload's grid and block (FP32: the mixed path's operator)
+    # ---- FP64 reference: the FP64 SCF with this build's strongest correction (the exact kinetic one)
+    t = time.perf_counter()
+    r64 = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, verbose=a.verbose, precond=dft_amd.PRECOND_KINETIC_EXACT)
+    t64 = time.perf_counter() - t
+    print(f"[bench] fp64 SCF (exact kinetic correction): E={r64['total']:.10f} steps={r64['iterations']} {t64:.2f} s",
+          file=sys.stderr, flush=True)
+    # ---- plain FP64: the reference's diagonal correction.  Its full SCF at this size runs longer than one
+    # bench allows, so a bounded number of its steps is timed and scaled to the FP64 reference's step count
+    plain = None
+    if a.plain_steps != 0:
+        cap = a.plain_steps if a.plain_steps > 0 else 200
+        t = time.perf_counter()
+        rp = run_scf(dft_amd, grid, atoms, w, dft_amd.SCF_FP64, max_iterations=cap, verbose=a.verbose,
+                     precond=dft_amd.PRECOND_DIAG)
+        tp = time.perf_counter() - t
+        full = bool(rp["converged"])
+        per_step = tp / max(rp["iterations"], 1)
+        plain = dict(seconds=tp if full else per_step * r64["iterations"], measured_seconds=tp,
+                     steps_run=rp["iterations"], converged=full, seconds_per_step=per_step,
+                     davidson_iterations=rp["davidson_iterations"],
+                     how="measured to convergence" if full else
+                     f"extrapolated: {rp['iterations']} steps timed ({per_step:.1f} s each, Davidson at its "
+                     f"200-iteration cap) x the {r64['iterations']} steps of the converged FP64 run")
+        print(f"[bench] plain fp64 (diagonal): {plain['how']} -> {plain['seconds']:.1f} s", file=sys.stderr,
+              flush=True)
+    # ---- H psi roofline at the workload's grid and block (FP32: the mixed path's operator)
     ms, by = time_hpsi(dft_amd, grid, nb, dft_amd.F32)
     ms64, by64 = time_hpsi(dft_amd, grid, nb, dft_amd.F64)
     ach = by / ms / 1e6
+    dense = time_dense(dft_amd, grid, w) if a.workload == "c5" else None
+    learned = a.precond == "cnn"
+    lsample = None
+    if a.learned_sample and os.path.exists(a.cnn_model):
+        print("[bench] learned-correction sample", file=sys.stderr, flush=True)
+        lsample = learned_sample(dft_amd, a.cnn_model)
+        print(f"[bench] learned sample: H applies {lsample['kinetic_exact']['happly']} (kinetic) -> "
+              f"{lsample['learned']['happly']} (learned), gain {lsample['happly_gain']:.2f}x", file=sys.stderr,
+              flush=True)
     out = {
-        "metric": METRIC, "value": per, "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "metric": METRIC if learned else METRIC.replace("mixed+learned", f"mixed ({a.precond} correction)"),
+        "value": per, "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": per * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+bf16x3 (f64 subspace, f64 refinement)", "data": "synthetic",
+        "dtype": "f32 (stencil, XC, density), bf16x3/bf16x6 MFMA (Gram, updates), f64 (subspace, refinement)",
+        "data": "synthetic",
         "config": {"workload": f"{a.workload}: {w['desc']}", "grid": w["n"], "h": w["h"], "atoms": len(atoms),
                    "electrons": ne, "davidson_block": w["nstates"],
                    "poisson": {"cheb": "chebyshev to max|dV|<1e-8",
                                "direct": "Jacobi fixed point by sine transforms"}.get(
                                    w.get("poisson"), f"{w['poisson_fixed']} warm Jacobi sweeps per call"),
-                   "stop": "|dE| < 1e-5 Ha",
-                   "preconditioner": {"diag": "diagonal (the reference's)", "kinetic": "kinetic smoothing",
-                                      "kinetic-cheb": "kinetic smoothing, 8 Chebyshev-combined steps, sigma 1.5",
-                                      "kinetic-exact": "(-1/2 Lap + s_j)^-1 by BF16 sine transforms, s_j = max(0.5, 0.35 |theta_j|)",
-                                      "cnn": f"learned ({a.cnn_model}) with diagonal fallback"}[a.precond],
-                   "fp64_preconditioner": "diagonal"},
-        "fp64_seconds": t64, "speedup_vs_fp64": t64 / per if per > 0 else None,
-        "fp64_same_precond_seconds": t64p, "speedup_vs_fp64_same_precond": (t64p / per) if t64p else None,
-        "energy_fp64_same_precond": r64p["total"] if r64p else None,
+                   "stop": "|dE| < 1e-5 Ha", "scf_options": w.get("scf", {}),
+                   "timed_correction": {"diag": "diagonal (the reference's)", "kinetic": "kinetic smoothing",
+                                        "kinetic-cheb": "kinetic smoothing, 8 Chebyshev-combined steps, sigma 1.5",
+                                        "kinetic-exact": "(-1/2 Lap + s_j)^-1 by BF16 sine transforms, "
+                                                         "s_j = max(0.5, 0.35 |theta_j|)",
+                                        "cnn": f"learned ({a.cnn_model}): the network added to the exact kinetic "
+                                               "correction over the nuclei's box, diagonal fallback per iteration"
+                                        }[a.precond]},
+        "learned": {"applied": rm["cnn_applied"], "fallbacks": rm["cnn_fallbacks"]} if learned else None,
+        "learned_sample": lsample,
+        "fp64_kinetic_seconds": t64, "speedup_vs_fp64_kinetic": t64 / per if per > 0 else None,
+        "fp64_plain": plain, "speedup_vs_plain_fp64": (plain["seconds"] / per) if plain and per > 0 else None,
         "energy_fp64": r64["total"], "energy_mixed": rm["total"], "energy_error_vs_fp64": abs(rm["total"] - r64["total"]),
         "eig_error_vs_fp64": float(np.max(np.abs(np.array(rm["eigenvalues"]) - np.array(r64["eigenvalues"])))),
         "scf_steps_fp64": r64["iterations"], "scf_steps_mixed": rm["iterations"], "refine_steps": rm["refine_iterations"],
+        "davidson_iterations": {"mixed": rm["davidson_iterations"], "fp64": r64["davidson_iterations"]},
         "phase_seconds_mixed": {k: rm[f"seconds_{k}"] for k in ("poisson", "xc", "davidson", "other")},
         "hpsi_f32_GBps": ach, "hpsi_f64_GBps": by64 / ms64 / 1e6,
         "hpsi_f32_gpt_orb_per_s": grid.size * nb / (ms * 1e-3), "hpsi_f64_gpt_orb_per_s": grid.size * nb / (ms64 * 1e-3),
+        "dense_products": dense,
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(f"hpsi_f32_{w['n']}x{w['n']}x{w['n']}_nb{nb}"),
                      "kernel": f"H psi FP32, {w['n']}^3 x {nb} orbitals",

@@ -618,6 +618,11 @@ class HarvestConfig(C.Structure):
                 ("cg_tol", C.c_double), ("cg_max_iter", C.c_int), ("seed", C.c_uint64), ("target_scale", C.c_int)]
 
 
+class HarvestStats(C.Structure):
+    _fields_ = [("solves", C.c_int64), ("failed", C.c_int64), ("cg_iterations", C.c_int64), ("patches", C.c_int64),
+                ("max_true_residual", C.c_double)]
+
+
 class CnnTrainer:
     """Device patch set + network + Adam state (dft_cnn_trainer)."""
 
@@ -665,6 +670,14 @@ class CnnTrainer:
         n = C.c_int64()
         self.L.dft_cnn_trainer_count(self.h, C.byref(n))
         return n.value
+
+    def harvest_stats(self):
+        """Totals over this trainer's harvests (dft_cnn_trainer_harvest_stats)."""
+        st = HarvestStats()
+        rc = self.L.dft_cnn_trainer_harvest_stats(self.h, C.byref(st))
+        if rc != 0:
+            _core_err(rc)
+        return {k: getattr(st, k) for k, _ in HarvestStats._fields_}
 
     def train(self, steps, batch=8, lr=1e-3, train_count=0):
         loss = np.zeros(max(steps, 1))

@@ -255,6 +255,9 @@ class HarvestPreconditioner final : public Preconditioner {
     void reset() override { calls_ = 0; }
 
     int64_t added = 0, solves = 0, failed = 0, cg_iterations = 0;
+    // max over converged columns of |b - A c| / |b|, recomputed from c after the CG (b = -(I - QQ^T) r,
+    // A = (I - QQ^T)(H - theta)(I - QQ^T)): the recurrence's residual can drift from the true one
+    double max_true_residual = 0.0;
 
    private:
     void project(const void* X, int64_t ldx, int first, int nc, void* Y) {
@@ -336,6 +339,20 @@ class HarvestPreconditioner final : public Preconditioner {
         }
         cg_iterations += it;
         solves += nc;
+        if (std::any_of(state.begin(), state.end(), [](int s) { return s == 1; })) {
+            // true residual: Pd = b (recomputed), Ap = A c, Pd -= Ap
+            check(c_, dft_scale_columns(c_, DFT_F64, n_, nc, R, ldr, ma, Pd_.get(), n_), "dft_scale_columns");
+            project(X, ldx, first, nc, Pd_.get());
+            check(c_, dft_hpsi(c_, &g_, DFT_F64, veff, Cc_.get(), n_, nc, Ap_.get(), n_), "dft_hpsi");
+            check(c_, dft_residual(c_, DFT_F64, n_, nc, Ap_.get(), n_, Cc_.get(), n_, theta_dev, Ap_.get(), n_),
+                  "dft_residual");
+            project(X, ldx, first, nc, Ap_.get());
+            check(c_, dft_cols_axpby(c_, DFT_F64, n_, nc, ma, Ap_.get(), n_, ones, Pd_.get(), n_), "dft_cols_axpby");
+            std::vector<double> tr;
+            dots(Pd_.get(), Pd_.get(), nc, tr);
+            for (int j = 0; j < nc; ++j)
+                if (state[j] == 1) max_true_residual = std::max(max_true_residual, std::sqrt(tr[j] / b2[j]));
+        }
         if (hc_.target_scale == 2) {
             // what the network must add to t_kin (the base correction, in Tb) beyond the span of the
             // Ritz vectors, which Davidson projects out: with t = (I - QQ^T) t_kin (Q as in the CG),
@@ -436,6 +453,7 @@ struct dft_cnn_trainer {
     dftc::Buf w32, wbf, grad, m, v;
     int step = 0;
     std::mt19937_64 rng;
+    dft_cnn_harvest_stats hstats{};
     // batch buffers
     int bcap = 0;
     dftc::Buf bin, btg, bout, act, scratch;
@@ -509,7 +527,18 @@ int dft_cnn_trainer_harvest(dft_cnn_trainer* t, const dft_grid* g, int natoms, c
         dftc::ScfOutputs out;
         drv.run(*rep, out);
         if (added) *added = hp.added;
+        t->hstats.solves += hp.solves;
+        t->hstats.failed += hp.failed;
+        t->hstats.cg_iterations += hp.cg_iterations;
+        t->hstats.patches += hp.added;
+        t->hstats.max_true_residual = std::max(t->hstats.max_true_residual, hp.max_true_residual);
     });
+}
+
+int dft_cnn_trainer_harvest_stats(dft_cnn_trainer* t, dft_cnn_harvest_stats* out) {
+    if (!t || !out) return DFT_EINVAL;
+    *out = t->hstats;
+    return DFT_OK;
 }
 
 int dft_cnn_trainer_count(dft_cnn_trainer* t, int64_t* patches) {

@@ -188,6 +188,19 @@ int dft_cnn_trainer_harvest(dft_cnn_trainer* t, const dft_grid* g, int natoms, c
                             const dft_scf_config* cfg, const dft_cnn_harvest_config* hc, dft_scf_report* rep,
                             int64_t* added);
 int dft_cnn_trainer_count(dft_cnn_trainer* t, int64_t* patches);
+/* Totals over the trainer's harvests: columns solved by the projected CG,
+ * columns that failed (no convergence or negative curvature), CG iterations,
+ * patches added, and the largest true relative residual |b - A c| / |b| of a
+ * converged column, recomputed from c after the CG (the CG stops on its
+ * recurrence's residual <= cg_tol). */
+typedef struct {
+    int64_t solves;
+    int64_t failed;
+    int64_t cg_iterations;
+    int64_t patches;
+    double max_true_residual;
+} dft_cnn_harvest_stats;
+int dft_cnn_trainer_harvest_stats(dft_cnn_trainer* t, dft_cnn_harvest_stats* out);
 /* Adam steps on random batches of the first `train_count` patches
  * (0: all); loss (optional) receives one value per step. */
 int dft_cnn_trainer_train(dft_cnn_trainer* t, int steps, int batch, float lr, int64_t train_count, double* loss);

@@ -250,3 +250,25 @@ def test_zero_network_on_the_kinetic_base():
     print("zero-net", rk["happly_columns"], rz["happly_columns"], rz["cnn_applied"], rz["cnn_fallbacks"])
     assert np.max(np.abs(rz["eigenvalues"][:4] - rk["eigenvalues"][:4])) < 1e-8
     assert rz["cnn_applied"] > 0
+
+
+@pytest.mark.parametrize("target_scale", [0, 2])
+def test_harvest_cg_true_residual(target_scale):
+    """The projected CG of the harvest stops when its recurrence's residual is below cg_tol; the
+    residual recomputed from the solution afterwards, |b - A c| / |b| with
+    A = (I - QQ^T)(H - theta)(I - QQ^T) and b = -(I - QQ^T) r, is within 2 cg_tol of it for every
+    converged column (the two differ by FP64 rounding drift, orders of magnitude below cg_tol here).
+    Both the diagonal-corrected harvest and the kinetic-base one (target_scale 2)."""
+    import dft_amd
+    g = dft_amd.Grid.centered(40, 0.35)
+    tr = dft_amd.CnnTrainer(patch=16, capacity=600, seed=11)
+    nuc = [[6, 0.05, 0.02, -0.03], [1, 2.0, 0.1, 0.0], [1, -1.1, 1.6, 0.2]]
+    cg_tol = 1e-6
+    tr.harvest(g, nuc, nstates=6, max_iterations=3, poisson_fixed=200, davidson_tol=1e-5, max_columns=4,
+               patches_per_column=2, every=2, cg_tol=cg_tol, cg_max_iter=400, seed=3, multiplicity=1,
+               target_scale=target_scale)
+    st = tr.harvest_stats()
+    tr.close()
+    print("harvest stats", st)
+    assert st["solves"] > 0 and st["solves"] > st["failed"] and st["patches"] > 0
+    assert 0.0 < st["max_true_residual"] <= 2.0 * cg_tol

@@ -87,11 +87,14 @@ def test_scf_he_pbe_golden(goldens):
     assert abs(r["eigenvalues"][0] - goldens["scf_he_pbe_eig"][0]) <= 1e-8
 
 
-@pytest.mark.parametrize("case", ["h2o_pz81", "ne_pbe_open"])
+@pytest.mark.parametrize("case", ["h2o_pz81", "ne_pbe_open", "n_pbe_open_sc"])
 def test_scf_step_parity(oracle, case):
     """One SCF iteration on identical input densities, GPU vs the FP64 CPU
     path: V_H and V_xc to relative 1e-12, occupied eigenvalues to 1e-8 Ha,
-    energies to 1e-7 Ha, output densities to the Davidson tolerance."""
+    energy components to 1e-6 Ha, output densities to the Davidson tolerance;
+    the total energy to 1e-7 Ha at a self-consistent input (closed-shell H2O,
+    PZ81; the open-shell N quartet, PBE96, at the density of its converged
+    SCF)."""
     import dft_amd
     from oracle.oracle import Grid
     if case == "h2o_pz81":
@@ -99,6 +102,13 @@ def test_scf_step_parity(oracle, case):
         nuc, fid, mult = [[8, 0, 0, 0], [1, 1.430, 0, 1.108], [1, -1.430, 0, 1.108]], 1, 1
         base = oracle.scf(og, nuc, fid, want_density=True)  # self-consistent input density
         ra, rb = base["rho_alpha"], base["rho_beta"]
+    elif case == "n_pbe_open_sc":
+        og = Grid.centered(20, 0.45)
+        nuc, fid, mult = [[7, 0.05, -0.02, 0.03]], 2, 4
+        sc = dft_amd.scf_run(dgrid(og), nuc, functional=fid, multiplicity=mult, stop_rule=dft_amd.STOP_DE,
+                             energy_tol=1e-10, max_iterations=200, want_density=True)
+        assert sc["converged"]
+        ra, rb = sc["rho_alpha"], sc["rho_beta"]
     else:
         og = Grid.centered(20, 0.45)
         nuc, fid, mult = [[7, 0.05, -0.02, 0.03]], 2, 4  # N atom, quartet: 5 alpha + 2 beta
@@ -124,7 +134,7 @@ def test_scf_step_parity(oracle, case):
     # At a self-consistent input the total energy is second order in the
     # orbital error (orbitals converged to residual 1e-7 on both sides); on a
     # non-self-consistent input it is first order, like the components.
-    if case == "h2o_pz81":
+    if case in ("h2o_pz81", "n_pbe_open_sc"):
         assert abs(got["energy"][5] - ref["energy"][5]) <= 1e-7
     assert np.max(np.abs(got["energy"] - ref["energy"])) <= 1e-6
     assert rel(got["rho_alpha"], ref["ora"]) <= 1e-6 and rel(got["rho_beta"], ref["orb"]) <= 1e-6
