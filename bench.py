@@ -252,6 +252,39 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
     return rows
 
 
+def time_xc(dft_amd, grid, atoms, w, reps=20):
+    """XC on the workload's own density (the output density of one mixed SCF step from the workload's
+    initial guess, with the workload's functional and flux): average launch time (HIP events on the
+    library stream) in FP32 and FP64, and the rate of the algorithmic bytes (read n_alpha, n_beta,
+    write v_alpha, v_beta: 16 B per point in FP32, 32 in FP64)."""
+    r = dft_amd.scf_run(grid, atoms, functional=w["functional"], precision=dft_amd.SCF_MIXED, nstates=w["nstates"],
+                        max_iterations=1, poisson_batch=64, **poisson_args(dft_amd, w), stop_rule=dft_amd.STOP_DE,
+                        energy_tol=1e-5, init_guess=w["init"], refine_iterations=0, want_density=True,
+                        precond=dft_amd.PRECOND_KINETIC_EXACT, **w.get("scf", {}))
+    fid = w["functional"] | (dft_amd.XC_FLAG_FLUX_CONSISTENT if w.get("scf", {}).get("xc_flux") else 0)
+    dev = dft_amd.Device(0)
+    out = {}
+    for prec, dt, es in ((dft_amd.F32, np.float32, 4), (dft_amd.F64, np.float64, 8)):
+        na, nb = dev.array(r["rho_alpha"], dtype=dt), dev.array(r["rho_beta"], dtype=dt)
+        va, vb = dev.array(shape=(grid.size,), dtype=dt), dev.array(shape=(grid.size,), dtype=dt)
+        for _ in range(3):
+            dev.xc(grid, fid, na, nb, va, vb)
+        dev.sync()
+        e0, e1 = dev.event(), dev.event()
+        dev.record(e0)
+        for _ in range(reps):
+            dev.xc(grid, fid, na, nb, va, vb)
+        dev.record(e1)
+        ms = dev.elapsed_ms(e0, e1) / reps
+        gbs = 4.0 * es * grid.size / ms / 1e6
+        out["f32" if es == 4 else "f64"] = dict(ms=ms, GBps=gbs, hbm_frac=gbs / HBM_PEAK_GBS)
+        del na, nb, va, vb
+    dev.close()
+    out["density"] = "output density of one mixed SCF step from the workload's initial guess"
+    out["functional"] = {1: "PZ81", 2: "PBE96"}[w["functional"]] + (" (consistent flux)" if fid & 0x100 else "")
+    return out
+
+
 def cpu_baseline(w, atoms, grid_n, h, rep, nocc):
     """The reference's CPU path on this host's cores (one thread: the reference
     is single-threaded), timed on bounded samples and extrapolated to the
@@ -383,6 +416,8 @@ def main():
                     help="only run this many SCF steps of each precision (profiling)")
     ap.add_argument("--profile-precision", default="both", choices=["both", "fp64", "mixed"],
                     help="with --profile-scf: which precisions to run")
+    ap.add_argument("--xc-only", type=int, default=0,
+                    help="only time N XC launches on the workload's one-step SCF density (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="with --hpsi-only: also run 5 device copies of this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -411,6 +446,9 @@ def main():
         ms64, by64 = time_hpsi(dft_amd, grid, nb, dft_amd.F64, reps=a.hpsi_only)
         print(json.dumps({"hpsi_f32_ms": ms, "hpsi_f32_GBps": by / ms / 1e6, "hpsi_f64_ms": ms64,
                           "hpsi_f64_GBps": by64 / ms64 / 1e6}))
+        return
+    if a.xc_only:
+        print(json.dumps(time_xc(dft_amd, grid, atoms, w, reps=a.xc_only)))
         return
     pc = {"diag": dft_amd.PRECOND_DIAG, "kinetic": dft_amd.PRECOND_KINETIC, "kinetic-cheb": dft_amd.PRECOND_KINETIC_CHEB,
           "kinetic-exact": dft_amd.PRECOND_KINETIC_EXACT, "cnn": dft_amd.PRECOND_CNN}[a.precond]

@@ -1,13 +1,16 @@
 // poisson.hip -- (b) Jacobi Poisson solver for V_H with the reference's
 // boundary rule and stop rule.
 //
-// One sweep = one launch of k_jacobi: the same z-streaming layout as the
-// H psi kernel (lane = x with DPP wave shifts for x-neighbours, TY y-rows
-// per lane, z-1/z/z+1 planes in registers), updating interior points only;
-// boundary points keep their values in both ping-pong buffers.  Each sweep
-// reduces max|V_new - V_old| over its workgroup and atomically maxes it into
-// its own slot, so a batch of sweeps runs without host round trips and the
-// host finds the exact sweep at which the reference would have stopped.
+// Two sweeps per launch (k_jacobi_tb, temporal blocking with overlapping
+// tiles, no grid-wide wait): the same z-streaming layout as the H psi
+// kernel (lane = x with DPP wave shifts for x-neighbours, R y-rows per lane,
+// z-1/z/z+1 planes in registers, one register queue per sweep), updating
+// interior points only; boundary points keep their values in both
+// ping-pong buffers.  Each sweep reduces max|V_new - V_old| over the points
+// a workgroup owns and atomically maxes it into its own slot, so a batch of
+// sweeps runs without host round trips and the host finds the exact sweep
+// at which the reference would have stopped.  The Chebyshev solver runs one
+// sweep per launch (k_jacobi with its extrapolation epilogue).
 #include "common.hpp"
 
 #include <rocblas/rocblas.h>
@@ -150,6 +153,159 @@ __global__ __launch_bounds__(256) void k_jacobi(int nx, int ny, int nz, int zc, 
     block_max_to((double)md, red, slot);
 }
 
+// ns <= KT sweeps per launch by temporal blocking, with no grid-wide wait.
+// A workgroup loads an extended tile of 64 (x, one lane each) x 4R (y, R
+// rows per thread) points and streams it along z; the tile overlaps its
+// neighbours by KT points on each side in x and y, and its z-range by KT
+// planes, so every sweep it needs is computed inside the workgroup (the
+// overlap is computed twice, by both neighbours, with identical inputs and
+// operations, hence identical values).  Sweep t (level t) is pipelined one
+// plane behind sweep t - 1: at step s the thread loads plane s of the input
+// (level 0) and computes plane s - t of level t from level t - 1's planes
+// s - t - 1 .. s - t + 1, which it holds in a three-plane register queue per
+// level.  x-neighbours come from the adjacent lanes (DPP); y-neighbours from
+// the thread's own rows, or across waves from the row each wave publishes
+// per level and plane in LDS (two plane slots per level: what a level reads
+// at step s was written at step s - 1, so one barrier per step suffices).
+// Values near the extended tile's edges are wrong after a few levels (their
+// neighbours lie outside the tile); they never reach a level-ns value of
+// the owned region [KT, 64 - KT) x [KT, 4R - KT) x [z0, z1), which alone is
+// stored.  Each level's max|V_new - V_old| over the owned interior points
+// goes to its own slot, so the per-sweep stop rule and replay work as with
+// one sweep per launch; the arithmetic per point is jacobi_tile's, so the
+// results are bitwise those of ns single sweeps.
+// Barrier for LDS traffic only: waits for this wave's LDS operations, not
+// for its global loads, so the next plane's loads stay in flight across it.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename T, int R, int KT>
+__global__ __launch_bounds__(256) void k_jacobi_tb(int nx, int ny, int nz, int zc, int ns, JCoef<T> co,
+                                                   const T* __restrict__ rho, const T* __restrict__ in,
+                                                   T* __restrict__ out, unsigned long long* __restrict__ slot) {
+    constexpr int EY = 4 * R, OX = 64 - 2 * KT, OY = EY - 2 * KT;
+    __shared__ T edge[KT][2][4][2][64];  // [level][plane parity][wave][first / last row][lane]
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t tile = xcd_tile(blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z),
+                                  (int64_t)gridDim.x * gridDim.y * gridDim.z);
+    const int bx = (int)(tile % gridDim.x), by = (int)((tile / gridDim.x) % gridDim.y);
+    const int bz = (int)(tile / ((int64_t)gridDim.x * gridDim.y));
+    const int plane = nx * ny;  // dft_poisson guarantees nx*ny*nz < 2^31: 32-bit offsets
+    const int gx = bx * OX - KT + lane;
+    const bool xg = gx >= 0 && gx < nx, xi = gx > 0 && gx + 1 < nx;
+    const bool xown = lane >= KT && lane < 64 - KT && xg;
+    bool yg[R], yi[R], yown[R];
+    int off[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int ly = w * R + r, gy = by * OY - KT + ly;
+        yg[r] = gy >= 0 && gy < ny;
+        yi[r] = gy > 0 && gy + 1 < ny;
+        yown[r] = ly >= KT && ly < EY - KT && yg[r];
+        off[r] = gy * nx + gx;
+    }
+    const int z0 = bz * zc, z1 = min(nz, z0 + zc);
+    // loads for step s: plane s of the input, and rho at plane s - t for level t; issued one step
+    // ahead, so they are in flight while the previous step computes
+    auto load_in = [&](int s, T (&v)[R]) {
+        const bool pin = s >= 0 && s < nz;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = (pin && xg && yg[r]) ? in[s * plane + off[r]] : T(0);
+    };
+    auto load_rho = [&](int s, T (&v)[KT][R]) {
+#pragma unroll
+        for (int t = 1; t <= KT; ++t) {
+            const int p = s - t;
+            const bool pint = t <= ns && p > 0 && p + 1 < nz;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[t - 1][r] = (pint && xi && yi[r]) ? rho[p * plane + off[r]] : T(0);
+        }
+    };
+    T q[KT][3][R];  // level t's planes p - 1, p, p + 1 (oldest first)
+#pragma unroll
+    for (int t = 0; t < KT; ++t)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int r = 0; r < R; ++r) q[t][j][r] = T(0);
+    T md[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) md[t] = T(0);
+    T nin[R], nrho[KT][R];
+    load_in(z0 - ns, nin);
+    load_rho(z0 - ns, nrho);
+    for (int s = z0 - ns; s < z1 + ns; ++s) {
+        T rh[KT][R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {  // level 0: plane s of the input
+            q[0][0][r] = q[0][1][r];
+            q[0][1][r] = q[0][2][r];
+            q[0][2][r] = nin[r];
+        }
+#pragma unroll
+        for (int t = 0; t < KT; ++t)
+#pragma unroll
+            for (int r = 0; r < R; ++r) rh[t][r] = nrho[t][r];
+        if (s + 1 < z1 + ns) {
+            load_in(s + 1, nin);
+            load_rho(s + 1, nrho);
+        }
+        edge[0][s & 1][w][0][lane] = q[0][2][0];
+        edge[0][s & 1][w][1][lane] = q[0][2][R - 1];
+#pragma unroll
+        for (int t = 1; t <= KT; ++t) {
+            if (t > ns) break;
+            const int p = s - t;
+            const bool pint = p > 0 && p + 1 < nz;
+            const bool pown = p >= z0 && p < z1;
+            // rows of level t - 1's plane p next to this thread's rows, published at step s - 1
+            const T above = w > 0 ? edge[t - 1][p & 1][w - 1][1][lane] : q[t - 1][1][0];
+            const T below = w < 3 ? edge[t - 1][p & 1][w + 1][0][lane] : q[t - 1][1][R - 1];
+            T nv[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const T c = q[t - 1][1][r];
+                const T xm = shr1(c, c);
+                const T xp = shl1(c, c);
+                const T ymv = r > 0 ? q[t - 1][1][r > 0 ? r - 1 : 0] : above;
+                const T ypv = r < R - 1 ? q[t - 1][1][r < R - 1 ? r + 1 : 0] : below;
+                nv[r] = c;
+                if (xi && yi[r] && pint) {
+                    // HartreePotentialMolecule.cpp:295-322
+                    const T src = co.fourpi * rh[t - 1][r];
+                    const T v = (co.cx * (xm + xp) + co.cy * (ymv + ypv) + co.cz * (q[t - 1][0][r] + q[t - 1][2][r]) + src) /
+                                co.den;
+                    nv[r] = v;
+                    if (xown && yown[r] && pown) {
+                        const T d = v > c ? v - c : c - v;
+                        md[t - 1] = md[t - 1] < d ? d : md[t - 1];
+                    }
+                }
+            }
+            if (t == ns) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (xown && yown[r] && pown && xi && yi[r] && pint) out[p * plane + off[r]] = nv[r];
+            } else if (t < KT) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    q[t][0][r] = q[t][1][r];
+                    q[t][1][r] = q[t][2][r];
+                    q[t][2][r] = nv[r];
+                }
+                edge[t][p & 1][w][0][lane] = nv[0];
+                edge[t][p & 1][w][1][lane] = nv[R - 1];
+            }
+        }
+        lds_barrier();
+    }
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+        if (t >= ns) break;
+        block_max_to((double)md[t], red, slot + t);
+    }
+}
+
 // Boundary values (HartreePotentialMolecule.cpp:37-95, 171-202): 0 on a
 // face, else Q/|r| measured from the origin.  Written to both buffers.
 template <typename T>
@@ -187,6 +343,37 @@ __global__ __launch_bounds__(256) void k_charge(const T* __restrict__ rho, int64
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// Sweep blocking: KT = 2 sweeps per launch on 64 x 16 tiles (R = 4 rows per
+// thread; owned 60 x 12), zc planes per workgroup.  Chosen from a sweep of
+// R in {4, 8}, KT in {1, 2, 4} and the z-chunking at 48^3 .. 256^3 in FP64
+// and FP32 on MI355X: KT = 2, R = 4 was fastest at every size, 1.2-2.1x the
+// one-sweep-per-launch kernel (7.4 us against 15.3 us per sweep at 48^3,
+// 13.4 against 17.3 at 96^3, 24 against 33 at 160^3, 77 against 109 at
+// 256^3, FP64); KT = 4 loses to the wider overlap and the longer pipeline.
+constexpr int kTbRows = 4, kTbSweeps = 2;
+
+struct TbShape {
+    int zc = 0;
+    dim3 grid;
+};
+
+TbShape tb_shape(const dft_grid* g, int num_cus) {
+    TbShape t;
+    const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    const int OX = 64 - 2 * kTbSweeps, OY = 4 * kTbRows - 2 * kTbSweeps;
+    const int64_t gx = (nx + OX - 1) / OX, gy = (ny + OY - 1) / OY;
+    // z-chunks: about 4 workgroups per CU with chunks of at least 2 KT planes; small grids
+    // (below 64^3) 8 per CU with chunks down to KT planes, where latency, not traffic, sets the rate
+    const bool small = nx * ny * nz < 262144;
+    const int64_t wpc = small ? 8 : 4, minz = (small ? 1 : 2) * kTbSweeps;
+    int64_t nzc = std::max<int64_t>(1, (wpc * num_cus + gx * gy - 1) / (gx * gy));
+    nzc = std::min<int64_t>(nzc, std::max<int64_t>(1, nz / minz));
+    t.zc = (int)((nz + nzc - 1) / nzc);
+    nzc = (nz + t.zc - 1) / t.zc;
+    t.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)nzc);
+    return t;
+}
+
 template <typename T>
 struct Jacobi {
     dft_ctx* c;
@@ -195,17 +382,26 @@ struct Jacobi {
     dim3 grid;
     int zc;
     const T* rho;
+    TbShape tb;
 
-    int sweep(const T* in, T* out, unsigned long long* slot) {
-        k_jacobi<T, 4><<<grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, zc, co, rho, in, out, slot, T(1));
+    // ns <= kTbSweeps sweeps in one launch
+    int block(const T* in, T* out, int ns, unsigned long long* slot) {
+        k_jacobi_tb<T, kTbRows, kTbSweeps><<<tb.grid, 256, 0, c->stream>>>((int)g->nx, (int)g->ny, (int)g->nz, tb.zc, ns, co,
+                                                                           rho, in, out, slot);
         DFT_CHECK_LAUNCH(c);
         return DFT_OK;
     }
 
-    // nsweeps sweeps starting from buffer a (even sweeps a -> b, odd b -> a), one launch each
-    int sweeps(T* a, T* b, int nsweeps, unsigned long long* slot) {
-        for (int s = 0; s < nsweeps; ++s)
-            if (int rc = sweep((s & 1) ? b : a, (s & 1) ? a : b, slot + s)) return rc;
+    // nsweeps sweeps starting from buffer a, ping-ponging a -> b -> a per launch; *launches
+    // receives the number of launches (the result is in b when it is odd)
+    int sweeps(T* a, T* b, int nsweeps, unsigned long long* slot, int* launches) {
+        int L = 0;
+        for (int s = 0; s < nsweeps; ++L) {
+            const int ns = std::min(kTbSweeps, nsweeps - s);
+            if (int rc = block((L & 1) ? b : a, (L & 1) ? a : b, ns, slot + s)) return rc;
+            s += ns;
+        }
+        *launches = L;
         return DFT_OK;
     }
 };
@@ -251,6 +447,7 @@ void jacobi_setup(dft_ctx* c, const dft_grid* g, const T* rho, Jacobi<T>& J) {
     J.zc = (nz + nzc - 1) / nzc;
     nzc = (nz + J.zc - 1) / J.zc;
     J.grid = dim3(gx, gy, nzc);
+    J.tb = tb_shape(g, c->num_cus);
 }
 
 template <typename T>
@@ -278,8 +475,9 @@ int poisson_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t max_swe
         DFT_HIP(c, hipMemsetAsync(c->sweep_max, 0, sizeof(unsigned long long) * nb, c->stream));
         if (exact) DFT_HIP(c, hipMemcpyAsync(snap, bufs[curi], bytes, hipMemcpyDeviceToDevice, c->stream));
         const int start = curi;
-        if (int rc = J.sweeps(bufs[curi], bufs[curi ^ 1], nb, c->sweep_max)) return rc;
-        if (nb & 1) curi ^= 1;
+        int launches = 0;
+        if (int rc = J.sweeps(bufs[curi], bufs[curi ^ 1], nb, c->sweep_max, &launches)) return rc;
+        if (launches & 1) curi ^= 1;
         if (!exact && done + nb < max_sweeps) { done += nb; continue; }
         DFT_HIP(c, hipMemcpyAsync(hm.data(), c->sweep_max, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, c->stream));
         DFT_HIP(c, hipStreamSynchronize(c->stream));
@@ -301,8 +499,9 @@ int poisson_t(dft_ctx* c, const dft_grid* g, const T* rho, T* v, int64_t max_swe
             curi = start;
             DFT_HIP(c, hipMemcpyAsync(bufs[curi], snap, bytes, hipMemcpyDeviceToDevice, c->stream));
             // replay into the upper slots (the batch's maxima are already read)
-            if (int rc = J.sweeps(bufs[curi], bufs[curi ^ 1], stop + 1, c->sweep_max + c->sweep_max_cap / 2)) return rc;
-            if ((stop + 1) & 1) curi ^= 1;
+            if (int rc = J.sweeps(bufs[curi], bufs[curi ^ 1], stop + 1, c->sweep_max + c->sweep_max_cap / 2, &launches))
+                return rc;
+            if (launches & 1) curi ^= 1;
         }
         done += stop + 1;
         break;

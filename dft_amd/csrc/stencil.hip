@@ -229,16 +229,17 @@ int launch_hpsi(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t
     return DFT_OK;
 }
 
-// Tile shape: 4 y-rows per wave (a 64 x 16 column per workgroup, so the
-// y-halo is 2 rows per 16) and NB orbitals per group sharing each V_eff load.
-// Measured on MI355X at 160^3 x 32 and 256^3 x 96 (non-temporal stores in
-// all variants): FP32 blocks below 64 orbitals run NB = 2 (4.4 TB/s at
-// 160^3 x 32); FP32 blocks of 64 and more NB = 4 with the one-plane prefetch
-// (4.7 TB/s at 256^3 x 96, against 4.0 without either); FP64 NB = 4 (4.2 and
-// 4.9 TB/s).  The remainder of a block runs in groups of 2, then 1.
-template <typename T, int NB, bool PF>
+// Tile shape: TY y-rows per wave (a 64 x 4TY column per workgroup, so the
+// y-halo is 2 rows per 4TY) and NB orbitals per group sharing each V_eff load.
+// Chosen from a sweep of TY in {2, 4, 8}, NB in {2, 4, 8}, with and without
+// the one-plane prefetch, at 160^3 x 32 and 256^3 x 96 on MI355X
+// (non-temporal stores in all variants): FP32 blocks below 64 orbitals run
+// TY = 4, NB = 2 (4.2 TB/s at 160^3 x 32); FP32 blocks of 64 and more TY = 8,
+// NB = 4 without prefetch (4.67 TB/s at 256^3 x 96, against 4.38 for TY = 4
+// with prefetch and 3.6-4.5 for the others); FP64 TY = 4, NB = 4 (4.0 and
+// 4.8-5.1 TB/s).  The remainder of a block runs in groups of 2, then 1.
+template <typename T, int NB, bool PF, int TY = 4>
 int hpsi_groups(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int nb, T* out, int64_t ldo) {
-    constexpr int TY = 4;
     const int ng = nb / NB;
     if (ng > 0)
         if (int rc = launch_hpsi<T, TY, NB, 0, PF>(c, g, V, psi, ld, ng, out, ldo)) return rc;
@@ -257,7 +258,7 @@ template <typename T>
 int hpsi_t(dft_ctx* c, const dft_grid* g, const T* V, const T* psi, int64_t ld, int nb, T* out, int64_t ldo) {
     DFT_REQUIRE(c, g->nx * g->ny * g->nz < (1LL << 31), "dft_hpsi: grid exceeds 2^31 points");
     if constexpr (sizeof(T) == 4) {
-        if (nb >= 64) return hpsi_groups<T, 4, true>(c, g, V, psi, ld, nb, out, ldo);
+        if (nb >= 64) return hpsi_groups<T, 4, false, 8>(c, g, V, psi, ld, nb, out, ldo);
         return hpsi_groups<T, 2, false>(c, g, V, psi, ld, nb, out, ldo);
     }
     return hpsi_groups<T, 4, false>(c, g, V, psi, ld, nb, out, ldo);
