@@ -252,6 +252,36 @@ def kernel_table(dft_amd, grid, w, nb, reps=10):
     return rows
 
 
+def time_jacobi(dft_amd, grid, reps=200):
+    """Jacobi sweeps at the workload's grid (the reference's update and boundary rule, two sweeps per
+    launch), FP64 and FP32: `reps` sweeps from a smooth density, minus a call of 8 sweeps (which
+    carries the same set-up: charge, boundary values), per sweep; rate of the algorithmic bytes
+    (read V and rho, write V: 24 B per point in FP64, 12 in FP32) and sweeps per second."""
+    dev = dft_amd.Device(0)
+    out = {}
+    for prec, dt, es in ((dft_amd.F64, np.float64, 8), (dft_amd.F32, np.float32, 4)):
+        rho = dev.array(shape=(grid.size,), dtype=dt)
+        dev.seed_vectors(grid, rho, 0, 1)
+        v = dev.zeros((grid.size,), dtype=dt)
+        t = {}
+        for ns in (8, reps + 8):
+            dev.poisson(grid, rho, v, max_sweeps=ns, tol=0.0, zero_start=True, batch=64)
+            dev.sync()
+            e0, e1 = dev.event(), dev.event()
+            dev.record(e0)
+            for _ in range(3):
+                dev.poisson(grid, rho, v, max_sweeps=ns, tol=0.0, zero_start=True, batch=64)
+            dev.record(e1)
+            t[ns] = dev.elapsed_ms(e0, e1) / 3
+        ms = (t[reps + 8] - t[8]) / reps
+        gbs = 3.0 * es * grid.size / ms / 1e6
+        out["f64" if es == 8 else "f32"] = dict(us_per_sweep=ms * 1e3, sweeps_per_s=1e3 / ms, GBps=gbs,
+                                                hbm_frac=gbs / HBM_PEAK_GBS)
+        del rho, v
+    dev.close()
+    return out
+
+
 def time_xc(dft_amd, grid, atoms, w, reps=20):
     """XC on the workload's own density (the output density of one mixed SCF step from the workload's
     initial guess, with the workload's functional and flux): average launch time (HIP events on the
@@ -416,6 +446,10 @@ def main():
                     help="only run this many SCF steps of each precision (profiling)")
     ap.add_argument("--profile-precision", default="both", choices=["both", "fp64", "mixed"],
                     help="with --profile-scf: which precisions to run")
+    ap.add_argument("--jacobi-only", type=int, default=0,
+                    help="only time N Jacobi sweeps at the workload's grid (profiling)")
+    ap.add_argument("--dense-only", type=int, default=0,
+                    help="only time N launches of each dense product at the workload's subspace limit (profiling)")
     ap.add_argument("--xc-only", type=int, default=0,
                     help="only time N XC launches on the workload's one-step SCF density (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -449,6 +483,12 @@ def main():
         return
     if a.xc_only:
         print(json.dumps(time_xc(dft_amd, grid, atoms, w, reps=a.xc_only)))
+        return
+    if a.jacobi_only:
+        print(json.dumps(time_jacobi(dft_amd, grid, reps=a.jacobi_only)))
+        return
+    if a.dense_only:
+        print(json.dumps(time_dense(dft_amd, grid, w, reps=a.dense_only)))
         return
     pc = {"diag": dft_amd.PRECOND_DIAG, "kinetic": dft_amd.PRECOND_KINETIC, "kinetic-cheb": dft_amd.PRECOND_KINETIC_CHEB,
           "kinetic-exact": dft_amd.PRECOND_KINETIC_EXACT, "cnn": dft_amd.PRECOND_CNN}[a.precond]
