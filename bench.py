@@ -29,6 +29,7 @@ from dft_amd.geometry import benzene, random_cluster  # noqa: E402  (workload ge
 METRIC = "SCF time to |dE|<1e-5 Ha: mixed+learned vs plain FP64 on 1 GPU; H·psi GB/s"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
 DEFAULT_MODEL = os.path.join(ROOT, "dft_amd", "data", "cnn_kinetic_c4.npz")
 PROFILES = os.path.join(ROOT, "profiles")
 
@@ -118,8 +119,8 @@ def time_hpsi(dft_amd, grid, nb, prec, reps=20):
 def time_dense(dft_amd, grid, w, reps=5):
     """The Davidson's dense FP32 products at the workload's subspace limit (m = 3 x block, k = block),
     timed with HIP events on the library stream: the split-BF16 Gram V^T X (three MFMA products per
-    tile), the block update V C and the fused Ritz residual [W | V] [Q ; -Q theta] with its norms (bf16
-    three-piece split, six MFMA products per tile).  Reported: time, HBM rate of the operands moved,
+    tile), the FP32-product Gram (FP32 MFMA), the block update V C and the fused Ritz residual
+    [W | V] [Q ; -Q theta] with its norms (bf16 three-piece split, six MFMA products per tile).  Reported: time, HBM rate of the operands moved,
     useful and executed MFMA rates, and the executed rate as a fraction of the dense BF16 peak."""
     dev = dft_amd.Device(0)
     n, k = grid.size, w["nstates"]
@@ -144,16 +145,19 @@ def time_dense(dft_amd, grid, w, reps=5):
         dev.record(e1)
         return dev.elapsed_ms(e0, e1) / reps
 
-    for name, fn, byts, flops, mult in (
+    for name, fn, byts, flops, mult, peak in (
             ("gram_split_bf16", lambda: dev.gram(dft_amd.GRAM_SPLIT_BF16, V, X, G, n, m, k), 4.0 * n * (m + k),
-             2.0 * n * m * k, 3),
-            ("update_bf16x6", lambda: dev.block_update(V, C, X, n, m, k), 4.0 * n * (m + k), 2.0 * n * m * k, 6),
+             2.0 * n * m * k, 3, BF16_PEAK_TFLOPS),
+            ("gram_f32", lambda: dev.gram(dft_amd.GRAM_F32, V, X, G, n, m, k), 4.0 * n * (m + k),
+             2.0 * n * m * k, 1, FP32_MFMA_PEAK_TFLOPS),
+            ("update_bf16x6", lambda: dev.block_update(V, C, X, n, m, k), 4.0 * n * (m + k), 2.0 * n * m * k, 6,
+             BF16_PEAK_TFLOPS),
             ("ritz_residual_bf16x6", lambda: dev.ritz_residual(V, V, C, theta, X, rn, n, m, k),
-             4.0 * n * (2 * m + k), 4.0 * n * m * k, 6)):
+             4.0 * n * (2 * m + k), 4.0 * n * m * k, 6, BF16_PEAK_TFLOPS)):
         ms = timed(fn)
         rows[name] = dict(shape=f"{n} x {m} x {k}", ms=ms, GBps=byts / ms / 1e6, hbm_frac=byts / ms / 1e6 / HBM_PEAK_GBS,
                           useful_TFLOPs=flops / ms / 1e9, executed_TFLOPs=mult * flops / ms / 1e9,
-                          mfma_frac=mult * flops / ms / 1e9 / BF16_PEAK_TFLOPS)
+                          mfma_frac=mult * flops / ms / 1e9 / peak)
     dev.close()
     return rows
 
@@ -310,6 +314,9 @@ def time_xc(dft_amd, grid, atoms, w, reps=20):
         out["f32" if es == 4 else "f64"] = dict(ms=ms, GBps=gbs, hbm_frac=gbs / HBM_PEAK_GBS)
         del na, nb, va, vb
     dev.close()
+    tot = r["rho_alpha"] + r["rho_beta"]
+    out["points_below_1e-20"] = float(np.mean(tot <= 1e-20))
+    out["points_1e-20_to_1e-10"] = float(np.mean((tot > 1e-20) & (tot < 1e-10)))
     out["density"] = "output density of one mixed SCF step from the workload's initial guess"
     out["functional"] = {1: "PZ81", 2: "PBE96"}[w["functional"]] + (" (consistent flux)" if fid & 0x100 else "")
     return out
@@ -437,9 +444,10 @@ def main():
                     help="1: also time the learned correction against its exact kinetic base on a bounded sample "
                          "(one Davidson solve of a held-out 128^3 molecule); 0: skip")
     ap.add_argument("--cnn-model", default=DEFAULT_MODEL, help="trained network (.npz) for --precond cnn")
-    ap.add_argument("--plain-steps", type=int, default=2,
+    ap.add_argument("--plain-steps", type=int, default=None,
                     help="steps of the plain FP64 run (diagonal correction) timed and scaled to the FP64 reference's "
-                         "step count; -1: run it to convergence; 0: skip")
+                         "step count; -1: run it to convergence; 0: skip (default: 2 for c5, whose plain run takes "
+                         "~25 min, else -1)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the timed mixed SCF (profiling): no FP64 runs, no CPU baseline")
     ap.add_argument("--profile-scf", type=int, default=0,
@@ -551,6 +559,8 @@ def main():
     # ---- plain FP64: the reference's diagonal correction.  Its full SCF at this size runs longer than one
     # bench allows, so a bounded number of its steps is timed and scaled to the FP64 reference's step count
     plain = None
+    if a.plain_steps is None:
+        a.plain_steps = 2 if a.workload == "c5" else -1
     if a.plain_steps != 0:
         cap = a.plain_steps if a.plain_steps > 0 else 200
         t = time.perf_counter()
@@ -563,8 +573,9 @@ def main():
                      steps_run=rp["iterations"], converged=full, seconds_per_step=per_step,
                      davidson_iterations=rp["davidson_iterations"],
                      how="measured to convergence" if full else
-                     f"extrapolated: {rp['iterations']} steps timed ({per_step:.1f} s each, Davidson at its "
-                     f"200-iteration cap) x the {r64['iterations']} steps of the converged FP64 run")
+                     f"extrapolated: {rp['iterations']} steps timed ({per_step:.1f} s each, "
+                     f"{rp['davidson_iterations']} Davidson iterations) x the {r64['iterations']} steps of the "
+                     f"converged FP64 run")
         print(f"[bench] plain fp64 (diagonal): {plain['how']} -> {plain['seconds']:.1f} s", file=sys.stderr,
               flush=True)
     # ---- H psi roofline at the workload's grid and block (FP32: the mixed path's operator)

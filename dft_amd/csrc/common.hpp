@@ -122,6 +122,11 @@ __device__ inline T block_sum(T v, T* lds32) {
     return s;
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, not for its global loads, so loads issued for the next step
+// stay in flight across it.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs (linear workgroup id b runs on XCD b % 8); this maps b to a tile index
 // so that each XCD owns one contiguous range of tiles, whose neighbouring

@@ -174,10 +174,6 @@ __global__ __launch_bounds__(256) void k_jacobi(int nx, int ny, int nz, int zc, 
 // goes to its own slot, so the per-sweep stop rule and replay work as with
 // one sweep per launch; the arithmetic per point is jacobi_tile's, so the
 // results are bitwise those of ns single sweeps.
-// Barrier for LDS traffic only: waits for this wave's LDS operations, not
-// for its global loads, so the next plane's loads stay in flight across it.
-__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
 template <typename T, int R, int KT>
 __global__ __launch_bounds__(256) void k_jacobi_tb(int nx, int ny, int nz, int zc, int ns, JCoef<T> co,
                                                    const T* __restrict__ rho, const T* __restrict__ in,

@@ -29,6 +29,35 @@ __host__ __device__ inline T tmax(T a, T b) { return (a < b) ? b : a; }
 template <typename T>
 __host__ __device__ inline T tclamp(T v, T lo, T hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
 
+// Transcendental functions.  FP64 keeps the math library's (the reference's
+// results to a few ulps).  FP32 uses the hardware's base-2 logarithm and
+// exponential (v_log_f32, v_exp_f32, about 1 ulp each): log x = log2(x) ln 2,
+// exp x = 2^(x log2 e), cbrt x = 2^(log2(x) / 3), x^y = 2^(y log2 x) for
+// x >= 0 (every power here has a non-negative base); a few ulps of FP32,
+// far inside the FP32 kernels' tolerances, at a fraction of the cost of the
+// correctly rounded library versions.  log1p and expm1, which guard
+// cancellations at low density, stay the library's.
+template <typename T>
+__device__ inline T xlog(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+    else return log(x);
+}
+template <typename T>
+__device__ inline T xexp(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    else return exp(x);
+}
+template <typename T>
+__device__ inline T xcbrt(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
+    else return cbrt(x);
+}
+template <typename T>
+__device__ inline T xpow(T x, T y) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * y);
+    else return pow(x, y);
+}
+
 // ---------------------------------------------------------------- PZ81
 template <typename T>
 struct PZPar { T A, B, C, D, gamma, beta1, beta2; };
@@ -40,14 +69,14 @@ __host__ __device__ inline PZPar<T> pz_pol() { return {T(0.01555), T(-0.0269), T
 
 template <typename T>
 __device__ inline T pz_ec(T rs, const PZPar<T>& p) {  // PZ81.cpp:52-81
-    if (rs < T(1)) return p.A * log(rs) + p.B + p.C * rs * log(rs) + p.D * rs;
+    if (rs < T(1)) return p.A * xlog(rs) + p.B + p.C * rs * xlog(rs) + p.D * rs;
     const T sr = sqrt(rs);
     const T den = T(1) + p.beta1 * sr + p.beta2 * rs;
     return p.gamma / den;
 }
 template <typename T>
 __device__ inline T pz_dec(T rs, const PZPar<T>& p) {  // PZ81.cpp:83-120
-    if (rs < T(1)) return p.A / rs + p.C * (log(rs) + T(1)) + p.D;
+    if (rs < T(1)) return p.A / rs + p.C * (xlog(rs) + T(1)) + p.D;
     const T sr = sqrt(rs);
     const T den = T(1) + p.beta1 * sr + p.beta2 * rs;
     const T dden = p.beta1 / (T(2) * sr) + p.beta2;
@@ -61,16 +90,16 @@ __device__ inline Out<T> pz81(T a_in, T b_in) {  // PZ81.cpp:437-501 with 293-43
     const T n = a + b;
     if (n <= T(kRhoFloor)) return o;
     const T zeta = tclamp((a - b) / n, T(-1), T(1));
-    const T rs = cbrt(T(3) / (T(4.0 * kPi) * n));
+    const T rs = xcbrt(T(3) / (T(4.0 * kPi) * n));
     const T f43 = T(4.0 / 3.0), f13 = T(1.0 / 3.0);
-    const T p43 = pow(T(1) + zeta, f43), m43 = pow(T(1) - zeta, f43);
-    const T p13 = pow(T(1) + zeta, f13), m13 = pow(T(1) - zeta, f13);
-    const T fden = pow(T(2), f43) - T(2);
+    const T p43 = xpow(T(1) + zeta, f43), m43 = xpow(T(1) - zeta, f43);
+    const T p13 = xpow(T(1) + zeta, f13), m13 = xpow(T(1) - zeta, f13);
+    const T fden = xpow(T(2), f43) - T(2);
     const T f = (p43 + m43 - T(2)) / fden;
     const T df = f43 * (p13 - m13) / fden;
     const T sx = (p43 + m43) / T(2);
     const T dsx = T(2.0 / 3.0) * (p13 - m13);
-    const T ex = -T(kExchangeCoefficient) * cbrt(n);
+    const T ex = -T(kExchangeCoefficient) * xcbrt(n);
     const T exs = ex * sx;
     const PZPar<T> U = pz_unpol<T>(), P = pz_pol<T>();
     const T e0 = pz_ec(rs, U), e1 = pz_ec(rs, P);
@@ -105,7 +134,7 @@ __device__ inline T sign_of(T v) { return v > T(0) ? T(1) : (v < T(0) ? T(-1) : 
 template <typename T>
 __device__ inline T log_1_plus_inv(T q1) {
     if constexpr (sizeof(T) == 4) return log1p(T(1) / q1);
-    else return log(T(1) + T(1) / q1);
+    else return xlog(T(1) + T(1) / q1);
 }
 
 template <typename T>
@@ -129,11 +158,11 @@ __device__ inline void pw92(T rs, T zeta, T& e, T& ers, T& ez) {  // PBE96.cpp:1
     const T al = gcor2(T(0.0168869), T(0.11125), T(10.357), T(3.6231), T(0.88026), T(0.49671), sr, ald);
     const T z4 = zeta * zeta * zeta * zeta;
     const T f43 = T(4.0 / 3.0);
-    const T f = (pow(T(1) + zeta, f43) + pow(T(1) - zeta, f43) - T(2)) / T(kSpinGamma);
+    const T f = (xpow(T(1) + zeta, f43) + xpow(T(1) - zeta, f43) - T(2)) / T(kSpinGamma);
     const T fzz = T(kSpinFzz);
     e = eu * (T(1) - f * z4) + ep * f * z4 - al * f * (T(1) - z4) / fzz;
     ers = eud * (T(1) - f * z4) + epd * f * z4 - ald * f * (T(1) - z4) / fzz;
-    const T df = f43 * (pow(T(1) + zeta, T(1.0 / 3.0)) - pow(T(1) - zeta, T(1.0 / 3.0))) / T(kSpinGamma);
+    const T df = f43 * (xpow(T(1) + zeta, T(1.0 / 3.0)) - xpow(T(1) - zeta, T(1.0 / 3.0))) / T(kSpinGamma);
     ez = T(4) * zeta * zeta * zeta * f * (ep - eu + al / fzz) + df * (z4 * ep - z4 * eu - (T(1) - z4) * al / fzz);
 }
 
@@ -141,7 +170,7 @@ template <typename T>
 __device__ inline void pbe_x(T rho, T gm, T& ed, T& dd, T& gd) {  // PBE96.cpp:347-444
     ed = dd = gd = T(0);
     if (rho <= T(0)) return;
-    const T rt = cbrt(rho);
+    const T rt = xcbrt(rho);
     const T exu = T(kExchangeA) * rt;
     const T exud = rho * exu;
     const T kf = T(kKf) * rt;
@@ -165,9 +194,9 @@ __device__ inline void pbe_c(T na, T nb, T ga, T gb, T out[4]) {  // PBE96.cpp:4
     if (n <= T(0)) return;
     const T zeta = tclamp((na - nb) / n, T(-1), T(1));
     const T gm = fabs(ga + gb);
-    const T kf = cbrt(T(3.0 * kPi * kPi) * n);
-    const T rs = cbrt(T(3) / (T(4.0 * kPi) * n));
-    const T phi = T(0.5) * (pow(T(1) + zeta, T(2.0 / 3.0)) + pow(T(1) - zeta, T(2.0 / 3.0)));
+    const T kf = xcbrt(T(3.0 * kPi * kPi) * n);
+    const T rs = xcbrt(T(3) / (T(4.0 * kPi) * n));
+    const T phi = T(0.5) * (xpow(T(1) + zeta, T(2.0 / 3.0)) + xpow(T(1) - zeta, T(2.0 / 3.0)));
     const T ks = sqrt(T(4) * kf / T(kPi));
     const T tkp = T(2) * ks * phi;
     T t = T(0);
@@ -179,11 +208,11 @@ __device__ inline void pbe_c(T na, T nb, T ga, T gb, T out[4]) {  // PBE96.cpp:4
     const T ex = -ec / (T(kGamma) * phi3);
     T b;
     if (ex < T(700)) b = T(kDelta) / expm1(ex);
-    else b = T(kDelta) * exp(-ex);
+    else b = T(kDelta) * xexp(-ex);
     const T b2 = b * b, t2 = t * t, t4 = t2 * t2, t6 = t4 * t2;
     const T q4 = T(1) + b * t2;
     const T q5 = T(1) + b * t2 + b2 * t4;
-    const T h = phi3 * T(kBeta / kDelta) * log(T(1) + T(kDelta) * q4 * t2 / q5);
+    const T h = phi3 * T(kBeta / kDelta) * xlog(T(1) + T(kDelta) * q4 * t2 / q5);
     const T rsthrd = rs / T(3);
     const T fac = T(kDelta) / b + T(1);
     const T bec = b2 * fac / (T(kBeta) * phi3);
@@ -193,7 +222,7 @@ __device__ inline void pbe_c(T na, T nb, T ga, T gb, T out[4]) {  // PBE96.cpp:4
     const T hrs = -rsthrd * hb * bec * ecrs;
     const T ht = T(2.0 * kBeta) * phi3 * q9 / q8;
     const T comm = h + hrs - T(7) * t2 * ht / T(6);
-    const T gz = (pow(T(1) + zeta + T(kEta), T(-1.0 / 3.0)) - pow(T(1) - zeta + T(kEta), T(-1.0 / 3.0))) / T(3);
+    const T gz = (xpow(T(1) + zeta + T(kEta), T(-1.0 / 3.0)) - xpow(T(1) - zeta + T(kEta), T(-1.0 / 3.0))) / T(3);
     const T bg = T(-3) * b2 * ec * fac / (T(kBeta) * phi4);
     const T hz = T(3) * gz * h / phi + hb * (bg * gz + bec * ecz);
     const T pref = hz - gz * t2 * ht / phi;

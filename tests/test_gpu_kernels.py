@@ -414,8 +414,13 @@ def test_nuclear_potential_singular(dev):
 
 
 @pytest.mark.parametrize("n,ma,mb", [(5000, 8, 8), (70001, 45, 17), (300000, 96, 96), (70000, 45, 17), (100004, 96, 32),
-                                     (400000, 87, 29), (49380, 1, 1), (333336, 17, 16)])
+                                     (400000, 87, 29), (49380, 1, 1), (333336, 17, 16), (200000, 288, 96),
+                                     (100000, 200, 40), (50004, 301, 7), (60000, 150, 130), (40000, 320, 96),
+                                     (30001, 288, 96), (4096, 97, 33)])
 def test_gram_modes(dev, n, ma, mb):
+    """A^T B against numpy in each mode and kernel: the streaming kernel (FP32 data, 16-byte aligned
+    columns, up to 96 x 32) and the tiled kernel (FP64; FP32 beyond the streaming kernel's shapes,
+    up to 320 x 130 in 96 x 96 tiles; unaligned FP32 columns, n = 30001 and 70001)."""
     import dft_amd
     rng = np.random.default_rng(n)
     A = rng.standard_normal((ma, n))
