@@ -106,6 +106,17 @@ class PulayMixer {
             kr_ = Buf(c, (size_t)L_ * esize(prec));
         }
     }
+    // Continue another mixer's history in this mixer's precision (the FP32 steps' history at the
+    // hand-over of a mixed-precision run): densities and residuals converted, the DIIS matrix kept,
+    // so the first FP64 steps extrapolate over the FP32 steps instead of starting from one entry.
+    void take_history(const PulayMixer& o) {
+        if (o.depth_ != depth_ || o.L_ != L_) return;
+        check(c_, dft_convert(c_, o.hist_.get(), o.prec_, hist_.get(), prec_, (int64_t)2 * depth_ * L_), "dft_convert");
+        A_ = o.A_;
+        count_ = o.count_;
+        head_ = o.head_;
+    }
+
     void mix(void* ra, void* rb, const void* ora, const void* orb) {
         const size_t es = esize(prec_), cb = (size_t)L_ * es;
         char* h = static_cast<char*>(hist_.get());
@@ -532,7 +543,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     if (cfg_.mixer != DFT_MIX_LINEAR && cfg_.mixer != DFT_MIX_PULAY) throw std::invalid_argument("unknown density mixer");
     if (cfg_.mix_depth < 0 || cfg_.mix_depth > 32) throw std::invalid_argument("mix_depth must be in 0..32");
     if (!(cfg_.mix_kerker >= 0.0) || !std::isfinite(cfg_.mix_kerker)) throw std::invalid_argument("mix_kerker must be >= 0");
-    std::unique_ptr<PulayMixer> pulay;  // rebuilt for each precision phase
+    std::unique_ptr<PulayMixer> pulay;  // rebuilt for each precision phase, keeping the history
     int pulay_prec = -1;
     bool refining = false;
     double best_dd = std::numeric_limits<double>::infinity();  // FP32 phase: lowest dRho so far
@@ -665,8 +676,10 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         }
         if (cfg_.mixer == DFT_MIX_PULAY) {
             if (!pulay || pulay_prec != p) {
+                std::unique_ptr<PulayMixer> prev = std::move(pulay);
                 pulay.reset(new PulayMixer(c_, g_, p, n, closed, cfg_.mix_depth > 0 ? cfg_.mix_depth : 6,
                                            cfg_.mix_beta > 0.0 ? cfg_.mix_beta : 0.3, cfg_.mix_kerker));
+                if (prev) pulay->take_history(*prev);
                 pulay_prec = p;
             }
             pulay->mix(f.ra.get(), f.rb.get(), f.ora.get(), f.orb.get());
