@@ -59,9 +59,9 @@ WORKLOADS = {
     "c3": dict(desc="benzene, 160^3, h=0.2, LSDA/PZ81, Davidson block 29, Poisson at the Jacobi fixed point",
                atoms=benzene(), n=160, h=0.2, functional=1, nstates=29, poisson_fixed=400, poisson="direct", init=1),
     "c5": dict(desc="40-atom random cluster (seed 5, radius 7 bohr, 150-170 electrons), 256^3, h=0.18, PBE96 with the "
-                    "consistent GGA flux, block 96, Poisson at the Jacobi fixed point, Fermi-Dirac kT 0.01 Ha, Pulay depth 10",
+                    "consistent GGA flux, block 96, Poisson at the Jacobi fixed point, Fermi-Dirac kT 0.01 Ha, Pulay depth 20",
                atoms=None, n=256, h=0.18, functional=2, nstates=96, poisson_fixed=600, poisson="direct", init=1,
-               scf=dict(smearing=0.01, mixer=1, mix_depth=10, xc_flux=1)),
+               scf=dict(smearing=0.01, mixer=1, mix_depth=20, xc_flux=1)),
 }
 
 
@@ -458,6 +458,9 @@ def main():
                     help="only time N Jacobi sweeps at the workload's grid (profiling)")
     ap.add_argument("--dense-only", type=int, default=0,
                     help="only time N launches of each dense product at the workload's subspace limit (profiling)")
+    ap.add_argument("--scf-options", default="",
+                    help="JSON object merged into the workload's SCF options for every SCF of the run, e.g. "
+                         "'{\"mix_beta\": 0.5}' (keys of dft_amd.scf_run)")
     ap.add_argument("--xc-only", type=int, default=0,
                     help="only time N XC launches on the workload's one-step SCF density (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -469,7 +472,9 @@ def main():
         sys.exit(2)
     import dft_amd
 
-    w = WORKLOADS[a.workload]
+    w = dict(WORKLOADS[a.workload])
+    if a.scf_options:
+        w["scf"] = {**w.get("scf", {}), **json.loads(a.scf_options)}
     atoms = workload_atoms(w)
     grid = dft_amd.Grid.centered(w["n"], w["h"])
     nb = w["nstates"] if w["nstates"] % 4 == 0 else 4 * ((w["nstates"] + 3) // 4)

@@ -22,8 +22,8 @@
 // the functional is evaluated (TX*TY + 2TX + 2TY)/(TX*TY) times per point
 // in a plane (halo recomputation) plus two planes per z-chunk.
 // In FP32 the functional is evaluated in FP64 where its FP32 form is unsafe:
-// total density below 1e-10 (PBE's t^6 and b^2 t^4 terms overflow FP32 in
-// the far tails), |grad n| / n > 1e4, or a non-finite FP32 result.
+// |grad n| / n > 1e4, or a non-finite FP32 result.  (The FP32 correlation is
+// regrouped so that the tails' large b t^2 does not overflow it.)
 #include "common.hpp"
 #include "xc_functionals.hpp"
 
@@ -41,7 +41,6 @@ static __device__ __align__(256) double scratch[kXcScratchDoubles];
 
 namespace {
 
-constexpr double kGgaF32Floor = 1.0e-10;
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 double* g_scratch_ptr[kMaxDevices] = {};
@@ -124,7 +123,7 @@ __device__ inline GgaPoint<T> gga_point(T a, T b, T axm, T axp, T aym, T ayp, T 
     T va, vb, ca, cb;
     if constexpr (sizeof(T) == 4) {
         bool done = false;
-        if (!((double)tot < kGgaF32Floor || (double)ga + (double)gb > 1.0e4 * (double)tot)) {
+        if (!((double)ga + (double)gb > 1.0e4 * (double)tot)) {
             const xc::Out<T> r = xc::pbe96<T>(a, b, ga, gb);
             if (finite_out(r)) {
                 va = r.va; vb = r.vb; ca = r.ca; cb = r.cb;

@@ -3,9 +3,10 @@
 // operation order (PZ81.cpp:41-501, PBE96.cpp:99-966) so the FP64 path
 // agrees with the CPU reference to a few ulps of the math library.  With
 // T = float, log(1 + 1/q1) in the PW92 interpolation is evaluated as
-// log1p(1/q1), the only change needed to keep FP32 finite at low density;
-// callers fall back to T = double below kF32Floor and for extreme relative
-// gradients (|grad n| / n > 1e4, where the GGA's t^6 terms overflow FP32).
+// log1p(1/q1), and PBE's H and its derivatives are regrouped (pbe_c), which
+// keeps FP32 finite down to the density floor; callers fall back to
+// T = double for extreme relative gradients (|grad n| / n > 1e4) and on a
+// non-finite FP32 result.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,8 +17,6 @@ namespace xc {
 constexpr double kPi = 3.141592653589793238462643383279502884;  // DFTConstants.h:7
 constexpr double kRhoFloor = 1.0e-20;                             // DFTConstants.h:9
 constexpr double kExchangeCoefficient = 0.7385587663820223;       // DFTConstants.h:25
-// Below this total density the FP32 kernels evaluate the functional in FP64.
-constexpr double kF32Floor = 1.0e-6;
 
 template <typename T>
 struct Out {
@@ -209,18 +208,36 @@ __device__ inline void pbe_c(T na, T nb, T ga, T gb, T out[4]) {  // PBE96.cpp:4
     T b;
     if (ex < T(700)) b = T(kDelta) / expm1(ex);
     else b = T(kDelta) * xexp(-ex);
-    const T b2 = b * b, t2 = t * t, t4 = t2 * t2, t6 = t4 * t2;
-    const T q4 = T(1) + b * t2;
-    const T q5 = T(1) + b * t2 + b2 * t4;
-    const T h = phi3 * T(kBeta / kDelta) * xlog(T(1) + T(kDelta) * q4 * t2 / q5);
+    const T b2 = b * b, t2 = t * t;
+    T h, hb, ht;
+    if constexpr (sizeof(T) == 4) {
+        // The same H, dH/db and dH/dt regrouped so that nothing grows beyond q5 ~ (b t^2)^2: in a
+        // density's tail b t^2 reaches ~1e13 and the reference's q8 = q5^2 + ... and b t^6 (2 + b t^2)
+        // would overflow FP32 (FP64 keeps the reference's form below).  With y = b t^2:
+        //   q4 / q5 = 1 / (1 + y (y / (1 + y))),
+        //   hb = -beta phi^3 [y (2 + y) / q5] [t^4 / (q5 + delta q4 t^2)],
+        //   ht = 2 beta phi^3 [q9 / q5] / (q5 + delta q4 t^2).
+        const T y = b * t2;
+        const T q4 = T(1) + y;
+        const T q5 = T(1) + y + y * y;
+        h = phi3 * T(kBeta / kDelta) * xlog(T(1) + T(kDelta) * t2 / (T(1) + y * (y / q4)));
+        const T q8r = q5 + T(kDelta) * q4 * t2;  // q8 / q5
+        hb = -T(kBeta) * phi3 * (y * (T(2) + y) / q5) * (t2 * t2 / q8r);
+        ht = T(2.0 * kBeta) * phi3 * ((T(1) + T(2) * y) / q5) / q8r;
+    } else {
+        const T t4 = t2 * t2, t6 = t4 * t2;
+        const T q4 = T(1) + b * t2;
+        const T q5 = T(1) + b * t2 + b2 * t4;
+        h = phi3 * T(kBeta / kDelta) * xlog(T(1) + T(kDelta) * q4 * t2 / q5);
+        const T q8 = q5 * q5 + T(kDelta) * q4 * q5 * t2;
+        const T q9 = T(1) + T(2) * b * t2;
+        hb = -T(kBeta) * phi3 * b * t6 * (T(2) + b * t2) / q8;
+        ht = T(2.0 * kBeta) * phi3 * q9 / q8;
+    }
     const T rsthrd = rs / T(3);
     const T fac = T(kDelta) / b + T(1);
     const T bec = b2 * fac / (T(kBeta) * phi3);
-    const T q8 = q5 * q5 + T(kDelta) * q4 * q5 * t2;
-    const T q9 = T(1) + T(2) * b * t2;
-    const T hb = -T(kBeta) * phi3 * b * t6 * (T(2) + b * t2) / q8;
     const T hrs = -rsthrd * hb * bec * ecrs;
-    const T ht = T(2.0 * kBeta) * phi3 * q9 / q8;
     const T comm = h + hrs - T(7) * t2 * ht / T(6);
     const T gz = (xpow(T(1) + zeta + T(kEta), T(-1.0 / 3.0)) - xpow(T(1) - zeta + T(kEta), T(-1.0 / 3.0))) / T(3);
     const T bg = T(-3) * b2 * ec * fac / (T(kBeta) * phi4);

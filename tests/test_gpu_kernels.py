@@ -335,6 +335,35 @@ def test_xc_f32_vs_oracle(dev, oracle, fid):
     assert abs(e - ref_e) <= 1e-5 * abs(ref_e)
 
 
+@pytest.mark.parametrize("consistent", [False, True])
+def test_xc_f32_density_tail(dev, oracle, consistent):
+    """FP32 PBE96 on an atom-like density whose tail falls to ~1e-19 (16 decades across the grid, the
+    regime of a cluster's grid beyond its atoms): evaluated in FP32 down to the density floor, finite
+    everywhere, V_xc within 2e-4 of the oracle's FP64 values relative to their maximum, and the
+    energy within 1e-5 relative, for the reference's and the consistent GGA flux."""
+    import dft_amd
+    from oracle.oracle import Grid
+    og = Grid.centered(40, 0.35)
+    xs, ys, zs = og.coords()
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2).ravel()
+    na = (5.0 * np.exp(-3.8 * r)).astype(np.float32).astype(np.float64)
+    nb = (4.0 * np.exp(-4.1 * r)).astype(np.float32).astype(np.float64)
+    assert np.min(na + nb) < 1e-18
+    fid = 2 | (dft_amd.XC_FLAG_FLUX_CONSISTENT if consistent else 0)
+    ref_a = oracle.xc_potential(og, 2, na, nb, 0, consistent=consistent)
+    ref_e = oracle.xc_energy(og, 2, na, nb)
+    g = dgrid(og)
+    a32, b32 = dev.array(na, dtype=np.float32), dev.array(nb, dtype=np.float32)
+    va, vb = dev.array(shape=(og.size,), dtype=np.float32), dev.array(shape=(og.size,), dtype=np.float32)
+    e = dev.xc(g, fid, a32, b32, va, vb)
+    got = va.download().astype(np.float64)
+    assert np.all(np.isfinite(got))
+    print("xc f32 tail", consistent, rel(got, ref_a), abs(e - ref_e) / abs(ref_e))
+    assert rel(got, ref_a) <= 2e-4
+    assert abs(e - ref_e) <= 1e-5 * abs(ref_e)
+
+
 def test_xc_f32_extreme_gradients(dev, oracle):
     """FP32 PBE96 on a density whose values jump by up to 11 decades between
     neighbours (an extrapolated density's tails): finite everywhere and, where
