@@ -119,7 +119,7 @@ def time_hpsi(dft_amd, grid, nb, prec, reps=20):
 def time_dense(dft_amd, grid, w, reps=5):
     """The Davidson's dense FP32 products at the workload's subspace limit (m = 3 x block, k = block),
     timed with HIP events on the library stream: the split-BF16 Gram V^T X (three MFMA products per
-    tile), the FP32-product Gram (FP32 MFMA), the block update V C and the fused Ritz residual
+    tile), the FP32-product Grams (FP32 MFMA; six bf16 products), the block update V C and the fused Ritz residual
     [W | V] [Q ; -Q theta] with its norms (bf16 three-piece split, six MFMA products per tile).  Reported: time, HBM rate of the operands moved,
     useful and executed MFMA rates, and the executed rate as a fraction of the dense BF16 peak."""
     dev = dft_amd.Device(0)
@@ -150,6 +150,8 @@ def time_dense(dft_amd, grid, w, reps=5):
              2.0 * n * m * k, 3, BF16_PEAK_TFLOPS),
             ("gram_f32", lambda: dev.gram(dft_amd.GRAM_F32, V, X, G, n, m, k), 4.0 * n * (m + k),
              2.0 * n * m * k, 1, FP32_MFMA_PEAK_TFLOPS),
+            ("gram_bf16x6", lambda: dev.gram(dft_amd.GRAM_SPLIT6_BF16, V, X, G, n, m, k), 4.0 * n * (m + k),
+             2.0 * n * m * k, 6, BF16_PEAK_TFLOPS),
             ("update_bf16x6", lambda: dev.block_update(V, C, X, n, m, k), 4.0 * n * (m + k), 2.0 * n * m * k, 6,
              BF16_PEAK_TFLOPS),
             ("ritz_residual_bf16x6", lambda: dev.ritz_residual(V, V, C, theta, X, rn, n, m, k),

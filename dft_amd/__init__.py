@@ -25,7 +25,7 @@ HIP_SO = os.path.join(LIBDIR, "libdft_hip.so")
 CORE_SO = os.path.join(LIBDIR, "libdft_core.so")
 
 F64, F32 = 0, 1
-GRAM_F64, GRAM_F32, GRAM_SPLIT_BF16 = 0, 1, 2
+GRAM_F64, GRAM_F32, GRAM_SPLIT_BF16, GRAM_SPLIT6_BF16 = 0, 1, 2, 3
 PZ81, PBE96 = 1, 2
 RED_SUM, RED_DOT, RED_MAXABS, RED_DIFF2 = 0, 1, 2, 3
 

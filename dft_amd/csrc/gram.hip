@@ -30,6 +30,7 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
 __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }  // round to nearest even
 
@@ -366,6 +367,218 @@ __global__ __launch_bounds__(256) void k_gram_stream(int64_t n, int ma, int mb, 
     }
 }
 
+// Wide Gram for FP32 data on the bf16 matrix cores (SPLIT_BF16: x = hi + lo,
+// three products; SPLIT6_BF16: x = x1 + x2 + x3, six products, FP32-level),
+// outputs up to 288 x 96.  One workgroup of 12 waves per CU computes the
+// whole output over its share of the points (every G-th 32-point step), so
+// each operand value is read from HBM once and split into bf16 pieces once.  Per step of 32 KS points the threads
+// load the step's A and B columns in 16-byte chunks (8 KS threads per column:
+// 128 KS contiguous bytes), split them and write the pieces to LDS in MFMA
+// operand order slab[piece][slice][strip][lane] (strip = 16 columns, A's
+// strips first; slice = 32 points).  Two LDS buffers: step s + 1 is split and
+// stored while step s's products run, and the loads of step s + 3 are in
+// flight meanwhile.  The waves form a 6 x 2 grid: wave (wa, wb) owns A
+// strips wa SA .. + SA - 1 and B strips wb SB .. + SB - 1, A's pieces held in
+// registers across its B strips.  Output element (i, j) of tile (a, b) sits
+// in lane 16 (i % 16 / 4) + j % 16, register i % 4 (the f32 layout above).
+constexpr int kWideWaves = 12, kWideWA = 6, kWideWB = 2;
+constexpr int kWideMaxTA = 18, kWideMaxTB = 6;  // ma <= 288, mb <= 96
+constexpr int kWideStrips = kWideMaxTA + kWideMaxTB;
+constexpr int kWideKS = 1;                                                               // 32-point slices per step
+constexpr int kWideChunks = kWideStrips * 16 * 8 * kWideKS / (64 * kWideWaves);         // 16-byte chunks per thread
+
+template <int MODE, int SA, int SB>
+__global__ __launch_bounds__(64 * kWideWaves) void k_gram_wide(int64_t n, int ma, int mb, const float* __restrict__ A,
+                                                                int64_t lda, const float* __restrict__ B, int64_t ldb,
+                                                                double* __restrict__ part) {
+    constexpr int P = MODE == DFT_GRAM_SPLIT6_BF16 ? 3 : 2;  // bf16 pieces per value
+    constexpr int KS = kWideKS, SP = 32 * KS, CPC = 8 * KS;  // slices, points and chunks per column per step
+    constexpr int NT = 64 * kWideWaves;
+    constexpr int PIECE = KS * kWideStrips * 64 * 8;  // bf16 per piece plane
+    constexpr int BUF = P * PIECE;                    // bf16 per step buffer
+    __shared__ __attribute__((aligned(16))) __bf16 sl[2 * BUF];  // two step buffers
+    const int ta = (ma + 15) >> 4, tb = (mb + 15) >> 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int wa = w % kWideWA, wb = w / kWideWA;
+    // steps: workgroup b takes the whole steps b, b + G, b + 2G, ... (G = gridDim.x), so
+    // at any time the workgroups read neighbouring segments of each column; the
+    // partial last step, if any, goes to the workgroup whose turn it is
+    const int64_t G = gridDim.x, total_full = n / SP;
+    const int64_t nfull = total_full > blockIdx.x ? (total_full - blockIdx.x + G - 1) / G : 0;
+    const int64_t first = (int64_t)blockIdx.x * SP, stride = G * SP;
+    const int64_t ptail = total_full * SP;
+    const bool tail = ptail < n && total_full % G == blockIdx.x;
+
+    // this thread's chunks: chunk c = threadIdx.x + NT u -> column cc = c / CPC of
+    // the strip list, points 4 h .. + 3 of the step (h = c % CPC).  NT is a
+    // multiple of 16 CPC, so h is the same for every u and cc = cc0 + (NT / CPC) u
+    // moves by whole strips: the LDS offset is linear in u.  A chunk of a
+    // padding column (or past the strip list) reads column 0 of A and stores
+    // zeros; slots past the strip list are never read.
+    static_assert(NT % (16 * CPC) == 0, "chunk map");
+    constexpr int CSTEP = NT / CPC;  // columns between a thread's chunks
+    const int h = threadIdx.x % CPC, cc0 = threadIdx.x / CPC;
+    const int off = 4 * h;  // the chunks' first point within the step
+    // operand lane ci + 16 (hh / 2) of slice ks holds points 8 (hh / 2) .. + 7; a chunk is its half hh % 2
+    const int dst0 = (((h >> 3) * kWideStrips + (cc0 >> 4)) * 64 + (cc0 & 15) + 16 * ((h & 7) >> 1)) * 8 + 4 * (h & 1);
+    constexpr int DSTEP = CSTEP / 16 * 64 * 8;  // bf16 between a thread's chunks in a piece plane
+    const float* src[kWideChunks];
+    bool colv[kWideChunks];
+#pragma unroll
+    for (int u = 0; u < kWideChunks; ++u) {
+        const int cc = cc0 + CSTEP * u;
+        const int s = cc >> 4, ci = cc & 15;
+        const bool isA = s < ta;
+        const int col = 16 * (isA ? s : s - ta) + ci;
+        colv[u] = s < ta + tb && col < (isA ? ma : mb);
+        src[u] = (colv[u] ? (isA ? A + (int64_t)col * lda : B + (int64_t)col * ldb) : A) + off;
+    }
+    float raw0[kWideChunks][4], raw1[kWideChunks][4];
+    auto load_full = [&](float (&raw)[kWideChunks][4], int64_t p) {  // every point of the step is below k1
+#pragma unroll
+        for (int u = 0; u < kWideChunks; ++u) {
+            // padding columns are zeroed when the step is stored, so no
+            // instruction touches the loaded registers before then
+            const float4 v = *reinterpret_cast<const float4*>(src[u] + p);
+            raw[u][0] = v.x;
+            raw[u][1] = v.y;
+            raw[u][2] = v.z;
+            raw[u][3] = v.w;
+        }
+    };
+    auto load_tail = [&](float (&raw)[kWideChunks][4], int64_t p) {
+#pragma unroll
+        for (int u = 0; u < kWideChunks; ++u)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) raw[u][e] = (colv[u] && p + off + e < n) ? src[u][p + e] : 0.0f;
+    };
+
+    f4 acc[SA][SB];
+#pragma unroll
+    for (int a = 0; a < SA; ++a)
+#pragma unroll
+        for (int b = 0; b < SB; ++b) acc[a][b] = f4{0, 0, 0, 0};
+    // operand strips of this wave, clamped into the strip list (tiles past the
+    // output are computed on valid data and never stored)
+    int sa_[SA], sb_[SB];
+#pragma unroll
+    for (int a = 0; a < SA; ++a) sa_[a] = min(wa * SA + a, ta - 1);
+#pragma unroll
+    for (int b = 0; b < SB; ++b) sb_[b] = ta + min(wb * SB + b, tb - 1);
+    constexpr int PV = PIECE / 8;  // bf8 per piece plane
+
+    auto store = [&](const float (&raw)[kWideChunks][4], int buf) {  // split a step's values into LDS buffer buf
+#pragma unroll
+        for (int u = 0; u < kWideChunks; ++u) {
+            bf4 x1, x2, x3;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float v = colv[u] ? raw[u][e] : 0.0f;
+                const __bf16 h1 = to_bf16(v);
+                const float r1 = v - (float)h1;
+                const __bf16 h2 = to_bf16(r1);
+                x1[e] = h1;
+                x2[e] = h2;
+                if constexpr (P == 3) x3[e] = to_bf16(r1 - (float)h2);
+            }
+            __bf16* d = sl + buf * BUF + dst0 + DSTEP * u;
+            *reinterpret_cast<bf4*>(d) = x1;
+            *reinterpret_cast<bf4*>(d + PIECE) = x2;
+            if constexpr (P == 3) *reinterpret_cast<bf4*>(d + 2 * PIECE) = x3;
+        }
+    };
+    auto products = [&](int buf) {  // this wave's tiles over the step in LDS buffer buf
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const bf8* sk = reinterpret_cast<const bf8*>(sl + buf * BUF) + ks * kWideStrips * 64 + lane;
+            bf8 av[SA][P];
+#pragma unroll
+            for (int a = 0; a < SA; ++a)
+#pragma unroll
+                for (int pc = 0; pc < P; ++pc) av[a][pc] = sk[pc * PV + sa_[a] * 64];
+#pragma unroll
+            for (int b = 0; b < SB; ++b) {
+                bf8 bv[P];
+#pragma unroll
+                for (int pc = 0; pc < P; ++pc) bv[pc] = sk[pc * PV + sb_[b] * 64];
+                // the SA tiles' chains interleaved, smallest terms first:
+                // x1 y1 + (x1 y2 + x2 y1) [+ (x1 y3 + x2 y2 + x3 y1)]
+                if constexpr (P == 3) {
+#pragma unroll
+                    for (int a = 0; a < SA; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][2], bv[0], acc[a][b], 0, 0, 0);
+#pragma unroll
+                    for (int a = 0; a < SA; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[2], acc[a][b], 0, 0, 0);
+#pragma unroll
+                    for (int a = 0; a < SA; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[1], acc[a][b], 0, 0, 0);
+                }
+#pragma unroll
+                for (int a = 0; a < SA; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[0], acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < SA; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[1], acc[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < SA; ++a) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[0], acc[a][b], 0, 0, 0);
+            }
+        }
+    };
+
+    // Whole steps with two LDS buffers and one barrier per step: at step s the
+    // workgroup stores step s + 1 (raw set (s + 1) % 2) into the other buffer,
+    // reloads that raw set with step s + 3 and runs step s's products; then the
+    // partial step, if any.
+    int64_t st = 0;
+    if (nfull > 0) {
+        load_full(raw0, first);
+        if (nfull > 1) load_full(raw1, first + stride);
+        store(raw0, 0);
+        if (nfull > 2) load_full(raw0, first + 2 * stride);
+        __syncthreads();
+    }
+    for (; st + 4 < nfull; st += 2) {  // no conditions: a straight-line body for the wait-count pass
+        const int64_t p = first + st * stride;
+        store(raw1, 1);
+        load_full(raw1, p + 3 * stride);
+        products(0);
+        __syncthreads();
+        store(raw0, 0);
+        load_full(raw0, p + 4 * stride);
+        products(1);
+        __syncthreads();
+    }
+    for (; st < nfull; st += 2) {
+        const int64_t p = first + st * stride;
+        if (st + 1 < nfull) store(raw1, 1);
+        if (st + 3 < nfull) load_full(raw1, p + 3 * stride);
+        products(0);
+        __syncthreads();
+        if (st + 1 < nfull) {
+            if (st + 2 < nfull) store(raw0, 0);
+            if (st + 4 < nfull) load_full(raw0, p + 4 * stride);
+            products(1);
+            __syncthreads();
+        }
+    }
+    if (tail) {
+        load_tail(raw0, ptail);
+        store(raw0, 0);
+        __syncthreads();
+        products(0);
+    }
+    double* P0 = part + (int64_t)blockIdx.x * ma * mb;
+#pragma unroll
+    for (int a = 0; a < SA; ++a)
+#pragma unroll
+        for (int b = 0; b < SB; ++b) {
+            const int j = 16 * (wb * SB + b) + r16;
+            if (wa * SA + a >= ta || wb * SB + b >= tb || j >= mb) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * (wa * SA + a) + 4 * q + r;
+                if (i < ma) P0[(int64_t)j * ma + i] = (double)acc[a][b][r];
+            }
+        }
+}
+
 __global__ __launch_bounds__(256) void k_gram_reduce1(const double* __restrict__ part, int nks, int64_t tot, int step,
                                                       double* __restrict__ part2) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -437,6 +650,55 @@ int gram_stream(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, c
     return DFT_OK;
 }
 
+// true when the wide kernel takes the shape: FP32 data in 16-byte aligned columns, ma <= 288, mb <= 96
+bool wide_fits(int ma, int mb, const void* a, int64_t lda, const void* b, int64_t ldb) {
+    return ma <= 16 * kWideMaxTA && mb <= 16 * kWideMaxTB && lda % 4 == 0 && ldb % 4 == 0 &&
+           reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0;
+}
+
+template <int MODE, int SA, int SB>
+void launch_wide(dft_ctx* c, unsigned nblk, int64_t n, int ma, int mb, const float* a, int64_t lda, const float* b,
+                 int64_t ldb) {
+    k_gram_wide<MODE, SA, SB><<<nblk, 64 * kWideWaves, 0, c->stream>>>(n, ma, mb, a, lda, b, ldb, c->partials);
+}
+
+template <int MODE, int SA>
+void launch_wide_sb(dft_ctx* c, int sb, unsigned nblk, int64_t n, int ma, int mb, const float* a, int64_t lda,
+                    const float* b, int64_t ldb) {
+    if (sb == 1) launch_wide<MODE, SA, 1>(c, nblk, n, ma, mb, a, lda, b, ldb);
+    else if (sb == 2) launch_wide<MODE, SA, 2>(c, nblk, n, ma, mb, a, lda, b, ldb);
+    else launch_wide<MODE, SA, 3>(c, nblk, n, ma, mb, a, lda, b, ldb);
+}
+
+int gram_wide(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, const float* a, int64_t lda,
+              const float* b, int64_t ldb, double beta, double* out, int64_t ldo) {
+    const int ta = (ma + 15) / 16, tb = (mb + 15) / 16;
+    const int sa = (ta + kWideWA - 1) / kWideWA, sb = (tb + kWideWB - 1) / kWideWB;
+    // one workgroup per CU (at most one per 32-point step)
+    const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>(c->num_cus, (n + 31) / 32));
+    const int64_t tot = (int64_t)ma * mb;
+    const int nseg = (int)std::min<int64_t>(nblk, 32), step = (int)((nblk + nseg - 1) / nseg);
+    if (int rc = ensure_partials(c, (size_t)(nblk + nseg) * tot + 8)) return rc;
+#define DFT_GW(M)                                                                          \
+    if (sa == 1) launch_wide_sb<M, 1>(c, sb, (unsigned)nblk, n, ma, mb, a, lda, b, ldb);        \
+    else if (sa == 2) launch_wide_sb<M, 2>(c, sb, (unsigned)nblk, n, ma, mb, a, lda, b, ldb);   \
+    else launch_wide_sb<M, 3>(c, sb, (unsigned)nblk, n, ma, mb, a, lda, b, ldb)
+    if (mode == DFT_GRAM_SPLIT6_BF16) {
+        DFT_GW(DFT_GRAM_SPLIT6_BF16);
+    } else {
+        DFT_GW(DFT_GRAM_SPLIT_BF16);
+    }
+#undef DFT_GW
+    DFT_CHECK_LAUNCH(c);
+    double* part2 = c->partials + nblk * tot;
+    k_gram_reduce1<<<dim3((unsigned)((tot + 255) / 256), nseg), 256, 0, c->stream>>>(c->partials, (int)nblk, tot, step,
+                                                                                    part2);
+    DFT_CHECK_LAUNCH(c);
+    k_gram_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(part2, nseg, ma, mb, alpha, beta, out, ldo);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -445,10 +707,16 @@ int dft_gram(dft_ctx* c, int mode, int64_t n, int ma, int mb, double alpha, cons
              int64_t ldb, double beta, double* out, int64_t ldo) {
     if (!c) return DFT_EINVAL;
     DFT_REQUIRE(c, a && b && out && n > 0 && ma > 0 && mb > 0 && lda >= n && ldb >= n && ldo >= ma, "dft_gram: bad arguments");
-    DFT_REQUIRE(c, mode >= DFT_GRAM_F64 && mode <= DFT_GRAM_SPLIT_BF16, "dft_gram: bad mode");
+    DFT_REQUIRE(c, mode >= DFT_GRAM_F64 && mode <= DFT_GRAM_SPLIT6_BF16, "dft_gram: bad mode");
+    // SPLIT6 has FP32-product accuracy: where the FP32 MFMA is not the bound (small outputs) or the
+    // wide kernel does not take the shape, the FP32 kernels run it
+    const bool wide = (mode == DFT_GRAM_SPLIT_BF16 || mode == DFT_GRAM_SPLIT6_BF16) && wide_fits(ma, mb, a, lda, b, ldb);
     if (mode != DFT_GRAM_F64 && ma <= 96 && mb <= 32 && lda % 4 == 0 && ldb % 4 == 0 &&
         reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0)
-        return gram_stream(c, mode, n, ma, mb, alpha, (const float*)a, lda, (const float*)b, ldb, beta, out, ldo);
+        return gram_stream(c, mode == DFT_GRAM_SPLIT6_BF16 ? DFT_GRAM_F32 : mode, n, ma, mb, alpha, (const float*)a,
+                           lda, (const float*)b, ldb, beta, out, ldo);
+    if (wide) return gram_wide(c, mode, n, ma, mb, alpha, (const float*)a, lda, (const float*)b, ldb, beta, out, ldo);
+    if (mode == DFT_GRAM_SPLIT6_BF16) mode = DFT_GRAM_F32;
     const int ntm = (ma + GT - 1) / GT, ntn = (mb + GT - 1) / GT;
     const int ntiles = ntm * ntn;
     const int ks = mode == DFT_GRAM_SPLIT_BF16 ? 32 : 16;

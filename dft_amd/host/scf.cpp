@@ -351,8 +351,9 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     const double kT = cfg_.smearing;
     const bool smear = kT > 0.0;
     if (!(kT >= 0.0) || !std::isfinite(kT)) throw std::invalid_argument("smearing must be >= 0");
-    if (cfg_.ortho_gram != 0 && cfg_.ortho_gram != DFT_GRAM_F32 && cfg_.ortho_gram != DFT_GRAM_SPLIT_BF16)
-        throw std::invalid_argument("ortho_gram must be 0, DFT_GRAM_F32 or DFT_GRAM_SPLIT_BF16");
+    if (cfg_.ortho_gram != 0 && cfg_.ortho_gram != DFT_GRAM_F32 && cfg_.ortho_gram != DFT_GRAM_SPLIT_BF16 &&
+        cfg_.ortho_gram != DFT_GRAM_SPLIT6_BF16)
+        throw std::invalid_argument("ortho_gram must be 0, DFT_GRAM_F32, DFT_GRAM_SPLIT_BF16 or DFT_GRAM_SPLIT6_BF16");
     if (smear && k < nocc_max + 2)
         throw std::invalid_argument("smearing needs empty states: nstates must exceed the occupied count by 2 or more");
     const int mmax = cfg_.max_subspace > 0 ? cfg_.max_subspace : std::max(3 * k, k + 16);
@@ -440,7 +441,8 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         o.nconv = nocc;
         if (f.prec == DFT_F32) {
             o.gram_mode = DFT_GRAM_SPLIT_BF16;
-            o.ortho_gram_mode = cfg_.ortho_gram == DFT_GRAM_SPLIT_BF16 ? DFT_GRAM_SPLIT_BF16 : DFT_GRAM_F32;
+            // FP32-product Grams: six bf16 products (0, the default) or the FP32 MFMA
+            o.ortho_gram_mode = cfg_.ortho_gram == 0 ? DFT_GRAM_SPLIT6_BF16 : cfg_.ortho_gram;
             o.tol = std::max(o.tol, 3.0e-4);
             // a warm start needs few iterations; a cold start (seed vectors) keeps the full budget
             if (have_prev) o.max_iter = std::min(o.max_iter, 60);

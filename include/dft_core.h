@@ -66,7 +66,9 @@ typedef struct {
                               the eigensolver's unoccupied buffer when nstates exceeds the occupied count by 8 or more),
                               and the energy reported and tested by the stop rule is the free energy E - TS */
     int ortho_gram;        /* mixed mode: Gram products of the FP32 orthonormalisation (block Gram-Schmidt, CholQR):
-                              0 or DFT_GRAM_F32: FP32 products; DFT_GRAM_SPLIT_BF16: as the Rayleigh-Ritz products */
+                              0 or DFT_GRAM_SPLIT6_BF16: FP32-level products as six bf16 MFMA products;
+                              DFT_GRAM_F32: FP32 products on the FP32 MFMA; DFT_GRAM_SPLIT_BF16: as the
+                              Rayleigh-Ritz products */
     double mix_kerker;     /* DFT_MIX_PULAY: Kerker screening wave number q0 (1/bohr) of the update; 0: none */
     double precond_shift_scale; /* DFT_PRECOND_KINETIC_EXACT: per-state shifts sigma_j = max(precond_shift,
                                    -scale theta_j); 0: scale 0.35 with precond_shift 0.5 by default; < 0: one shift

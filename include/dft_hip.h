@@ -35,7 +35,7 @@ enum { DFT_OK = 0, DFT_EINVAL = -1, DFT_ENUMERIC = -2, DFT_EHIP = -3 };
 /* Arithmetic/storage precision of a field or orbital block. */
 enum { DFT_F64 = 0, DFT_F32 = 1 };
 /* Gram product modes (dft_gram). */
-enum { DFT_GRAM_F64 = 0, DFT_GRAM_F32 = 1, DFT_GRAM_SPLIT_BF16 = 2 };
+enum { DFT_GRAM_F64 = 0, DFT_GRAM_F32 = 1, DFT_GRAM_SPLIT_BF16 = 2, DFT_GRAM_SPLIT6_BF16 = 3 };
 /* XCFunctional subclasses (XCFunctional.h:15-21, PZ81.h, PBE96.h). */
 enum { DFT_XC_PZ81 = 1, DFT_XC_PBE96 = 2 };
 /* Flag OR-ed into a functional id for dft_xc: the GGA flux is
@@ -213,7 +213,10 @@ int dft_nuclear_potential(dft_ctx* ctx, const dft_grid* g, int prec, int natoms,
  * DFT_GRAM_F64: FP64 operands on FP64 MFMA.  DFT_GRAM_F32: FP32 operands,
  * FP32 FMA.  DFT_GRAM_SPLIT_BF16: FP32 operands split x = hi + lo (bf16),
  * A^T B ~ hi.hi + hi.lo + lo.hi on bf16 MFMA with FP32 accumulation,
- * partial sums combined in FP64. */
+ * partial sums combined in FP64.  DFT_GRAM_SPLIT6_BF16: FP32 operands split
+ * exactly into three bf16 pieces, the six products of order <= 2^-18 on bf16
+ * MFMA: FP32-product accuracy (as DFT_GRAM_F32) at the bf16 rate; outputs up
+ * to 96 x 32 run as DFT_GRAM_F32. */
 int dft_gram(dft_ctx* ctx, int mode, int64_t n, int ma, int mb, double alpha, const void* a, int64_t lda,
              const void* b, int64_t ldb, double beta, double* out, int64_t ldo);
 /* Block update Y = alpha * A C + beta * Y: A N x m (prec), C m x k FP64

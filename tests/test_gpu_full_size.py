@@ -186,7 +186,7 @@ def test_davidson_products_benzene_shape_exact(dev):
     ref_r = (Cm.astype(np.float32) @ W32 - th.astype(np.float32)[:, None] * (Cm.astype(np.float32) @ V32)).astype(np.float64)
     assert np.max(np.abs(ref_vv)) < 2 ** 24 and np.max(np.abs(ref_r)) < 2 ** 24
     dC, dth = dev.array(Cm), dev.array(th)
-    for dt, modes in ((np.float64, (dft_amd.GRAM_F64,)), (np.float32, (dft_amd.GRAM_F32, dft_amd.GRAM_SPLIT_BF16))):
+    for dt, modes in ((np.float64, (dft_amd.GRAM_F64,)), (np.float32, (dft_amd.GRAM_F32, dft_amd.GRAM_SPLIT_BF16, dft_amd.GRAM_SPLIT6_BF16))):
         dV, dW = dev.array(V8.astype(dt), dtype=dt), dev.array(W8.astype(dt), dtype=dt)
         for mode in modes:
             out = dev.zeros((m, m))

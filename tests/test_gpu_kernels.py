@@ -445,11 +445,14 @@ def test_nuclear_potential_singular(dev):
 @pytest.mark.parametrize("n,ma,mb", [(5000, 8, 8), (70001, 45, 17), (300000, 96, 96), (70000, 45, 17), (100004, 96, 32),
                                      (400000, 87, 29), (49380, 1, 1), (333336, 17, 16), (200000, 288, 96),
                                      (100000, 200, 40), (50004, 301, 7), (60000, 150, 130), (40000, 320, 96),
-                                     (30001, 288, 96), (4096, 97, 33)])
+                                     (30001, 288, 96), (4096, 97, 33), (12, 288, 96), (1000, 17, 96), (77780, 250, 90),
+                                     (8196, 97, 1), (262148, 288, 81)])
 def test_gram_modes(dev, n, ma, mb):
     """A^T B against numpy in each mode and kernel: the streaming kernel (FP32 data, 16-byte aligned
-    columns, up to 96 x 32) and the tiled kernel (FP64; FP32 beyond the streaming kernel's shapes,
-    up to 320 x 130 in 96 x 96 tiles; unaligned FP32 columns, n = 30001 and 70001)."""
+    columns, up to 96 x 32; SPLIT6 there runs as FP32), the wide kernel (both split modes, aligned
+    columns, up to 288 x 96, one workgroup per CU over contiguous point ranges; n = 12 is one partial
+    32-point step) and the tiled kernel (FP64; FP32 beyond those shapes, up to 320 x 130 in 96 x 96
+    tiles; unaligned FP32 columns, n = 30001 and 70001)."""
     import dft_amd
     rng = np.random.default_rng(n)
     A = rng.standard_normal((ma, n))
@@ -462,7 +465,7 @@ def test_gram_modes(dev, n, ma, mb):
     assert rel(got, ref) <= 1e-13
     a32, b32 = A.astype(np.float32), B.astype(np.float32)
     ref32 = a32.astype(np.float64) @ b32.astype(np.float64).T
-    for mode, tol in ((dft_amd.GRAM_F32, 1e-5), (dft_amd.GRAM_SPLIT_BF16, 1e-4)):
+    for mode, tol in ((dft_amd.GRAM_F32, 1e-5), (dft_amd.GRAM_SPLIT_BF16, 1e-4), (dft_amd.GRAM_SPLIT6_BF16, 1e-5)):
         dev.gram(mode, dev.array(a32, dtype=np.float32), dev.array(b32, dtype=np.float32), out, n, ma, mb)
         got = out.download().T
         print("gram mode", mode, rel(got, ref32))
@@ -536,7 +539,7 @@ def test_gram_trailing_columns(dev):
     dA = dev.array(A, dtype=np.float32)
     T = SimpleNamespace(ptr=dA.col(m, n), prec=dA.prec)
     out = dev.zeros((nt, m + nt))
-    for mode, tol in ((dft_amd.GRAM_F32, 1e-5), (dft_amd.GRAM_SPLIT_BF16, 1e-4)):
+    for mode, tol in ((dft_amd.GRAM_F32, 1e-5), (dft_amd.GRAM_SPLIT_BF16, 1e-4), (dft_amd.GRAM_SPLIT6_BF16, 1e-5)):
         dev.gram(mode, dA, T, out, n, m + nt, nt)
         assert rel(out.download().T, ref) <= tol
 
