@@ -282,6 +282,30 @@ class Device:
         self._chk(self.L.dft_nuclear_potential(self.ctx, C.byref(g), C.c_int(out.prec), C.c_int(nuc.shape[0]),
                                                nuc.ctypes.data_as(_dp), _vp(out.ptr)))
 
+    RADIAL_FIELDS = ("kinetic", "external", "hartree", "exchangeCorrelation", "total", "densityDifference",
+                     "energyDifference", "maxKSResidual", "iterations", "converged", "status", "mixing")
+
+    def radial_scf(self, Z, functional, points=2000, rmax=30.0, densities=False):
+        """solveAtom (AtomicDFT.cpp:6-41) for every (Z[j], functional[j]) at once, one workgroup
+        per atom.  Returns a list of dicts with the energy terms, the stop quantities, the iteration
+        count and the occupied orbitals' eigenvalues (alpha subshells, then beta), and with
+        densities=True the output spin densities as an array (njobs, 2, points)."""
+        Z = np.ascontiguousarray(np.atleast_1d(Z), dtype=np.int32)
+        fn = np.ascontiguousarray(np.broadcast_to(np.atleast_1d(functional), Z.shape), dtype=np.int32)
+        res = np.zeros((Z.size, 26))
+        dens = np.zeros((Z.size, 2, points)) if densities else None
+        rc = self.L.dft_radial_scf(self.ctx, C.c_int(Z.size), Z.ctypes.data_as(C.POINTER(C.c_int)),
+                                   fn.ctypes.data_as(C.POINTER(C.c_int)), C.c_int64(points), C.c_double(rmax),
+                                   res.ctypes.data_as(_dp), dens.ctypes.data_as(_dp) if densities else None)
+        self._chk(rc)
+        out = []
+        for j in range(Z.size):
+            d = {k: float(res[j, q]) for q, k in enumerate(self.RADIAL_FIELDS)}
+            norb = radial_orbital_count(int(Z[j]))
+            d["eigenvalues"] = [float(x) for x in res[j, 12:12 + norb]]
+            out.append(d)
+        return (out, dens) if densities else out
+
     def gram(self, mode, a, b, out, n, ma, mb, alpha=1.0, beta=0.0, lda=None, ldb=None, ldo=None):
         self._chk(self.L.dft_gram(self.ctx, C.c_int(mode), C.c_int64(n), C.c_int(ma), C.c_int(mb), C.c_double(alpha),
                                   _vp(a.ptr), C.c_int64(lda or n), _vp(b.ptr), C.c_int64(ldb or n), C.c_double(beta),
@@ -480,6 +504,22 @@ def _core_err(rc):
     L = core_lib()
     L.dft_core_last_error.restype = C.c_char_p
     raise DftError(rc, L.dft_core_last_error().decode())
+
+
+def radial_config(Z):
+    """getAtomicConfiguration (ElectronicConfiguration.cpp:50-145): the subshells of Z as
+    (n, l, alpha electrons, beta electrons) tuples."""
+    L = hip_lib()
+    a = [(C.c_int * 7)() for _ in range(4)]
+    k = L.dft_radial_config(C.c_int(Z), *a)
+    if k < 0:
+        raise ValueError(f"no configuration for Z = {Z}")
+    return [(a[0][i], a[1][i], a[2][i], a[3][i]) for i in range(k)]
+
+
+def radial_orbital_count(Z):
+    """Occupied radial orbitals of Z: one per subshell and spin with electrons."""
+    return sum((al > 0) + (be > 0) for _, _, al, be in radial_config(Z))
 
 
 def scf_run(grid, nuclei, functional=PZ81, charge=0, multiplicity=1, precision=SCF_FP64, nstates=0, max_subspace=0,

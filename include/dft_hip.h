@@ -207,6 +207,28 @@ int dft_add3(dft_ctx* ctx, int prec, int64_t n, const void* a, const void* b, co
  * {Z,x,y,z} per atom.  Returns -2 if a nucleus sits exactly on a grid point. */
 int dft_nuclear_potential(dft_ctx* ctx, const dft_grid* g, int prec, int natoms, const double* nuclei_host, void* v);
 
+/* ---- radial (atomic) stack ------------------------------------------------
+ * The reference's one-centre solver (AtomicDFT.cpp, SelfConsistentField.cpp,
+ * EigenvalueSolver.cpp, HartreePotential.cpp, ExchangeCorrelation.cpp,
+ * ElectronDensity.cpp, TotalEnergy.cpp) on the radial grid
+ * r_i = (i + 1) rmax / (points + 1), i < points (RadialGrid.cpp:5-20). */
+/* getAtomicConfiguration (ElectronicConfiguration.cpp:50-145): the subshells
+ * (n, l, alpha, beta electrons; at most 7) of Z = 1..30; returns their count
+ * or -1. */
+int dft_radial_config(int Z, int* n, int* l, int* alpha, int* beta);
+/* solveAtom (AtomicDFT.cpp:6-41) for njobs atoms at once, one workgroup
+ * each: Z[j] in 1..30, functional[j] DFT_XC_PZ81 or DFT_XC_PBE96, 2 <= points
+ * <= 2048 (the reference's default: 2000 points, rmax 30).  results: njobs
+ * rows of 26 doubles {kinetic, external, hartree, xc, total, dRho, |dE|, KS
+ * residual, iterations, converged (0/1), status, final mixing factor,
+ * eigenvalues of the occupied orbitals (alpha subshells, then beta; 14
+ * slots)}.  densities (optional, host): njobs x 2 x points, the last output
+ * alpha and beta densities.  The SCF stops as the reference's does (dRho <
+ * 1e-9, |dE| < 1e-11, KS < 1e-7, at most 250 iterations).  Returns -2 when a
+ * job met a singular tridiagonal pivot (its status is nonzero). */
+int dft_radial_scf(dft_ctx* ctx, int njobs, const int* Z, const int* functional, int64_t points, double rmax,
+                   double* results, double* densities);
+
 /* ---- (e) Gram / Rayleigh-Ritz products ------------------------------------
  * out (device, FP64, column-major ma x mb, leading dim ldo) =
  *   alpha * A^T B  (+ beta * out), A: N x ma (lda), B: N x mb (ldb).

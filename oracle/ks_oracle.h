@@ -140,4 +140,24 @@ int og_scf_step(const og_grid* g, int natoms, const double* nuclei, int function
 #ifdef __cplusplus
 }
 #endif
+
+/* ---- radial (atomic) stack, oracle/radial_oracle.c ------------------------
+ * Grid r_i = (i + 1) rmax / (points + 1), i < points (RadialGrid.cpp:5-20). */
+/* ElectronicConfiguration.cpp:16-145: subshells (n, l, alpha, beta) of Z = 1..30; returns their count */
+int og_radial_config(int Z, int* n, int* l, int* alpha, int* beta);
+/* EigenvalueSolver.cpp:314-403 solveOrbital: eigenvalue `index` of the tridiagonal H(l) with
+ * potential v, and its eigenvector u normalised to dr sum u^2 = 1 */
+int og_radial_orbital(long points, double rmax, const double* v, int l, long index, double* eig, double* u);
+/* HartreePotential.cpp:8-86 */
+int og_radial_hartree(long points, double rmax, const double* rho, double* vh);
+/* ExchangeCorrelation.cpp:167-279 (spin 0 or 1) and 281-370 */
+int og_radial_xc_potential(long points, double rmax, int functional, const double* na, const double* nb, int spin,
+                           double* v);
+double og_radial_xc_energy(long points, double rmax, int functional, const double* na, const double* nb);
+/* AtomicDFT.cpp:6-41 / SelfConsistentField.cpp: the atom's SCF.  res[8] = {kinetic, external,
+ * hartree, xc, total, dRho, |dE|, KS residual} of the last iteration; eig: orbital eigenvalues,
+ * alpha then beta (<= 14); the output spin densities if na_out / nb_out are given.  Returns the
+ * iteration count (negated if not converged in 250), < -1000 on an error. */
+int og_radial_scf(int Z, int functional, long points, double rmax, double* res, double* eig, int* norb_out,
+                  double* na_out, double* nb_out);
 #endif

@@ -408,3 +408,68 @@ class Reference:
         r["iterations"] = int(r["iterations"])
         r["eigenvalues"] = eig
         return r
+
+
+# ---- the radial (atomic) stack: oracle/radial_oracle.c and the reference's solveAtom
+RADIAL_KEYS = ("kinetic", "external", "hartree", "exchangeCorrelation", "total", "densityDifference",
+               "energyDifference", "maxKSResidual")
+
+
+def _radial_scf(self, Z, functional=1, points=2000, rmax=30.0, densities=False):
+    """og_radial_scf: the atom's SCF as the reference's solveAtom (AtomicDFT.cpp:6-41)."""
+    res = np.zeros(8)
+    eig = np.zeros(14)
+    no = C.c_int()
+    na = np.zeros(points) if densities else None
+    nb = np.zeros(points) if densities else None
+    self.L.og_radial_scf.restype = C.c_int
+    rc = self.L.og_radial_scf(C.c_int(Z), C.c_int(functional), C.c_long(points), C.c_double(rmax), _d(res), _d(eig),
+                              C.byref(no), None if na is None else _d(na), None if nb is None else _d(nb))
+    if rc < -1000 or rc == -1001:
+        raise OracleError(f"og_radial_scf rc={rc}")
+    out = dict(zip(RADIAL_KEYS, res.tolist()))
+    out.update(iterations=abs(rc), converged=rc > 0, eigenvalues=eig[:no.value].tolist())
+    if densities:
+        out["alpha"], out["beta"] = na, nb
+    return out
+
+
+def _radial_orbital(self, V, l, index, rmax=30.0):
+    """og_radial_orbital: eigenvalue `index` of the radial H(l) with potential V, and its vector."""
+    V = _f64(V)
+    u = np.zeros(V.size)
+    e = C.c_double()
+    self._chk(self.L.og_radial_orbital(C.c_long(V.size), C.c_double(rmax), _d(V), C.c_int(l), C.c_long(index),
+                                       C.byref(e), _d(u)))
+    return e.value, u
+
+
+def _radial_hartree(self, rho, rmax=30.0):
+    rho = _f64(rho)
+    out = np.zeros(rho.size)
+    self._chk(self.L.og_radial_hartree(C.c_long(rho.size), C.c_double(rmax), _d(rho), _d(out)))
+    return out
+
+
+Oracle.radial_scf = _radial_scf
+Oracle.radial_orbital = _radial_orbital
+Oracle.radial_hartree = _radial_hartree
+
+
+def _ref_radial_scf(self, Z, functional=1, points=2000, rmax=30.0):
+    """rf_radial_scf: the reference's own solveAtom(RadialGrid(points, rmax), getAtomicConfiguration(Z), f)."""
+    out = np.zeros(10)
+    eig = np.zeros(14)
+    no = C.c_int()
+    self._chk(self.L.rf_radial_scf(C.c_int(Z), C.c_int(functional), C.c_long(points), C.c_double(rmax), _d(out),
+                                   _d(eig), C.c_int(14), C.byref(no)))
+    keys = ["iterations", "converged", "kinetic", "external", "hartree", "exchangeCorrelation", "total",
+            "energyDifference", "densityDifference", "maxKSResidual"]
+    r = dict(zip(keys, out.tolist()))
+    r["iterations"] = int(r["iterations"])
+    r["converged"] = bool(r["converged"])
+    r["eigenvalues"] = eig[:no.value].tolist()
+    return r
+
+
+Reference.radial_scf = _ref_radial_scf

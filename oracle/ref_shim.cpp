@@ -4,8 +4,10 @@
 // fixtures under tests/golden and to time the reference CPU path.  Each
 // wrapper calls the reference's public function named in its comment and
 // turns exceptions into status codes (-1 invalid_argument, -2 other).
+#include "AtomicDFT.h"
 #include "CartesianGrid.h"
 #include "DFTData.h"
+#include "ElectronicConfiguration.h"
 #include "EigenvalueSolverMolecule.h"
 #include "ElectronDensityMolecule.h"
 #include "ExchangeCorrelationMolecule.h"
@@ -18,6 +20,7 @@
 #include "NuclearRepulsion.h"
 #include "PBE96.h"
 #include "PZ81.h"
+#include "RadialGrid.h"
 #include "SelfConsistentFieldMolecule.h"
 #include "TotalEnergyMolecule.h"
 
@@ -308,6 +311,24 @@ int rf_scf(const long* n, const double* b, int natoms, const double* nuc, int ch
             if (quiet) { std::cout.rdbuf(o1); std::cerr.rdbuf(o2); }
             throw;
         }
+    });
+}
+
+// solveAtom (AtomicDFT.cpp:6-41) on RadialGrid(points, rmax) with
+// getAtomicConfiguration(Z): out = {iterations, converged, kinetic, external,
+// hartree, xc, total, dE, dRho, KS residual}; eig: the orbitals' eigenvalues in
+// the result's order (up to neig)
+int rf_radial_scf(int Z, int fid, long points, double rmax, double* out, double* eig, int neig, int* norb) {
+    return guarded([&] {
+        const RadialGrid grid((std::size_t)points, rmax);
+        const AtomicResult r = solveAtom(grid, getAtomicConfiguration(Z), functional(fid));
+        const auto& s = r.scf;
+        double v[10] = {(double)s.iterations, s.converged ? 1.0 : 0.0, s.energy.kinetic, s.energy.external,
+                        s.energy.hartree, s.energy.exchangeCorrelation, s.energy.total, s.energyDifference,
+                        s.densityDifference, s.maxKSResidual};
+        std::memcpy(out, v, sizeof v);
+        *norb = (int)s.orbitals.size();
+        for (int i = 0; i < neig && i < (int)s.orbitals.size(); ++i) eig[i] = s.orbitals[i].eigenvalue;
     });
 }
 
