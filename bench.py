@@ -167,6 +167,29 @@ def time_dense(dft_amd, grid, w, reps=5):
     return rows
 
 
+def time_radial(dft_amd, reps=3):
+    """The reference's atom table (solveAtom for Z = 1..30 in PZ81 and PBE96 on 2000 radial points)
+    as one batched launch, against the CPU oracle (the reference's algorithm, one thread)."""
+    dev = dft_amd.Device(0)
+    Z = [z for _ in (1, 2) for z in range(1, 31)]
+    fn = [f for f in (1, 2) for _ in range(1, 31)]
+    dev.radial_scf(Z, fn)  # loads the code object
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = dev.radial_scf(Z, fn)
+    gpu = (time.perf_counter() - t) / reps
+    dev.close()
+    from oracle.oracle import Oracle  # the CPU baseline leg: the oracle times the reference's algorithm
+    O = Oracle()
+    t = time.perf_counter()
+    ref = [O.radial_scf(z, f) for z, f in zip(Z, fn)]
+    cpu = time.perf_counter() - t
+    worst = max(abs(g["total"] - r["total"]) / abs(r["total"]) for g, r in zip(out, ref))
+    return {"radial_atoms": len(Z), "gpu_seconds": gpu, "cpu_oracle_seconds": cpu, "cpu_threads": 1,
+            "speedup": cpu / gpu, "scf_iterations": int(sum(g["iterations"] for g in out)),
+            "worst_rel_energy_vs_oracle": worst}
+
+
 def kernel_table(dft_amd, grid, w, nb, reps=10):
     """Average launch time, achieved bandwidth and flop rate of each hot kernel
     at the workload's grid (HIP events on the library stream)."""
@@ -471,6 +494,9 @@ def main():
     ap.add_argument("--energy-tol", type=float, default=1e-5,
                     help="SCF stop rule |dE| < tol (Ha) for every leg; the metric is defined at 1e-5, a tighter value "
                          "compares the precisions at convergence")
+    ap.add_argument("--radial", action="store_true",
+                    help="only time the radial atom table (Z = 1..30, PZ81 and PBE96, 2000 points) as one GPU "
+                         "launch, beside the CPU oracle's sequential solves")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="with --hpsi-only: also run 5 device copies of this many MiB (counter calibration)")
     a = ap.parse_args()
@@ -508,6 +534,9 @@ def main():
         return
     if a.jacobi_only:
         print(json.dumps(time_jacobi(dft_amd, grid, reps=a.jacobi_only)))
+        return
+    if a.radial:
+        print(json.dumps(time_radial(dft_amd)))
         return
     if a.dense_only:
         print(json.dumps(time_dense(dft_amd, grid, w, reps=a.dense_only)))
