@@ -167,6 +167,37 @@ def time_dense(dft_amd, grid, w, reps=5):
     return rows
 
 
+def time_cnn(dft_amd, reps=10, dims=(128, 128, 128)):
+    """Inference of the learned correction's network (2 -> 16 -> 16 -> 16 -> 1 channels, 3^3
+    convolutions) over one box, HIP events on the library stream.  Useful work: 2 x 15 120 flops
+    per voxel (the four layers' multiply-adds)."""
+    dev = dft_amd.Device(0)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(1)
+    fan = [2 * 27, 16 * 27, 16 * 27, 16 * 27]
+    chans = [(2, 16), (16, 16), (16, 16), (16, 1)]
+    p = np.concatenate([np.concatenate([rng.standard_normal(co * ci * 27) * np.sqrt(2.0 / f), np.zeros(co)])
+                        for (ci, co), f in zip(chans, fan)]).astype(np.float32)
+    w32 = dev.array(p, dtype=np.float32)
+    wbf = dev.array(shape=(p.size,), dtype=np.uint16)
+    dev.cnn_to_bf16(w32, wbf)
+    x = dev.array(rng.standard_normal((1, 2 * n)).astype(np.float32), dtype=np.float32)
+    out = dev.array(shape=(1, n), dtype=np.float32)
+    act = dev.array(shape=(dev.cnn_act_count(dims, 1, 0),), dtype=np.float32)
+    dev.cnn_forward(dims, 1, wbf, x, out, act, 0)
+    dev.sync()
+    e0, e1 = dev.event(), dev.event()
+    dev.record(e0)
+    for _ in range(reps):
+        dev.cnn_forward(dims, 1, wbf, x, out, act, 0)
+    dev.record(e1)
+    dev.sync()
+    ms = dev.elapsed_ms(e0, e1) / reps
+    dev.close()
+    flops = 2.0 * (2 * 16 + 16 * 16 + 16 * 16 + 16) * 27 * n
+    return {"cnn_forward_box": list(dims), "ms": ms, "voxels_per_s": n / ms * 1e3, "useful_TFLOPs": flops / ms / 1e9}
+
+
 def time_radial(dft_amd, reps=3):
     """The reference's atom table (solveAtom for Z = 1..30 in PZ81 and PBE96 on 2000 radial points)
     as one batched launch, against the CPU oracle (the reference's algorithm, one thread)."""
@@ -494,6 +525,9 @@ def main():
     ap.add_argument("--energy-tol", type=float, default=1e-5,
                     help="SCF stop rule |dE| < tol (Ha) for every leg; the metric is defined at 1e-5, a tighter value "
                          "compares the precisions at convergence")
+    ap.add_argument("--cnn-only", type=int, default=0,
+                    help="only time N forward passes of the learned correction's network over one 128^3 box "
+                         "(inference, He-initialised weights)")
     ap.add_argument("--radial", action="store_true",
                     help="only time the radial atom table (Z = 1..30, PZ81 and PBE96, 2000 points) as one GPU "
                          "launch, beside the CPU oracle's sequential solves")
@@ -537,6 +571,9 @@ def main():
         return
     if a.radial:
         print(json.dumps(time_radial(dft_amd)))
+        return
+    if a.cnn_only:
+        print(json.dumps(time_cnn(dft_amd, reps=a.cnn_only)))
         return
     if a.dense_only:
         print(json.dumps(time_dense(dft_amd, grid, w, reps=a.dense_only)))

@@ -8,7 +8,9 @@
 // Activation layout: [item][channel][z][y][x], x fastest (the grid layout).
 //
 // Kernels:
-//   k_conv    -- direct convolution.  A workgroup owns a 16x4x4 output tile
+//   k_conv16_mfma -- the 16 -> 16 layers (forward and data gradient) as an
+//                implicit GEMM on the bf16 matrix cores (below).
+//   k_conv    -- direct convolution (the 2 -> 16 and 16 -> 1 layers).  A workgroup owns a 16x4x4 output tile
 //                (one voxel per thread) and stages the (18x6x6) x cin input
 //                halo tile in LDS; the FP32 copy of the BF16 weights is read
 //                with wave-uniform addresses (scalar loads), so each LDS read
@@ -90,7 +92,7 @@ __device__ inline void load_halo(float* xs, const float* __restrict__ inb, int n
 //   forward:   K(o,i,tap) = W[o][i][tap]            (W of shape [NO][NI][27])
 //   transpose: K(o,i,tap) = W[i][o][26 - tap]       (W of shape [NI][NO][27])
 // EPI_DSILU multiplies the result by SiLU'(zepi[item][o][v]).
-template <int NI, int NO, bool IN_SILU, bool TRANS, bool BIAS, bool EPI_DSILU>
+template <int NI, int NO, bool IN_SILU, bool TRANS, bool BIAS, bool EPI_DSILU, bool OUT_SILU = false>
 __global__ __launch_bounds__(256) void k_conv(int nx, int ny, int nz, int tiles_z, const float* __restrict__ w,
                                               const float* __restrict__ bias, const float* __restrict__ in,
                                               float* __restrict__ out, const float* __restrict__ zepi) {
@@ -129,7 +131,114 @@ __global__ __launch_bounds__(256) void k_conv(int nx, int ny, int nz, int tiles_
             const int64_t p = ((int64_t)item * NO + o) * N + v;
             float r = acc[o];
             if (EPI_DSILU) r *= dsilu(zepi[p]);
+            if (OUT_SILU) r = silu(r);
             out[p] = r;
+        }
+    }
+}
+
+// The 16 -> 16 convolutions (layers 1 and 2, 92% of the network's flops) as an
+// implicit GEMM on the bf16 matrix cores: out[o][v] = sum_K A[o][K] B[K][v] with
+// K = tap * 16 + i (27 taps x 16 channels = 432, padded to 14 steps of 32).
+//   A (v_mfma_f32_16x16x32_bf16 operand A, 16 x 32): the BF16 parameters as
+//     they are (exact); lane l holds output channel l % 16 and K = 32 s + 8 (l / 16)
+//     .. + 7, i.e. tap 2 s + (l / 16) / 2 and channels 8 ((l / 16) % 2) .. + 7,
+//     all 14 steps in registers for the whole tile.
+//   B (32 x 16): the input halo tile in LDS, channel-last, each FP32 value split
+//     into bf16 hi + lo (two MFMAs per step, lo first: ~2^-17 relative, the FP32
+//     kernel's 3e-5 bar with room); lane l reads the 8 channels of its K slice
+//     at voxel (x0 + l % 16 + kx, ...) with one 16-byte load per piece.
+//   D (16 x 16): lane l, register r = output channel 4 (l / 16) + r at voxel
+//     x0 + l % 16; wave w takes the output rows (ty, tz) 4w .. 4w + 3 of the
+//     16 x 4 x 4 tile.
+// Modes as k_conv: forward (bias, SiLU of the input on load) and the data
+// gradient (flipped, transposed kernel; times SiLU'(z) in the epilogue).
+typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+constexpr int kConvKSteps = 14;
+
+template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI_DSILU, bool OUT_SILU>
+__global__ __launch_bounds__(256) void k_conv16_mfma(int nx, int ny, int nz, int tiles_z,
+                                                     const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
+                                                     const float* __restrict__ in, float* __restrict__ out,
+                                                     const float* __restrict__ zepi) {
+    __shared__ __attribute__((aligned(16))) __bf16 xh[HV * 16];
+    __shared__ __attribute__((aligned(16))) __bf16 xl[HV * 16];
+    const int bz = blockIdx.z % tiles_z, item = blockIdx.z / tiles_z;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = bz * TZ;
+    const int64_t N = (int64_t)nx * ny * nz;
+    const float* inb = in + (int64_t)item * 16 * N;
+    // a thread stages whole halo voxels: 16 channel loads (each coalesced over the
+    // threads' consecutive voxels), then the voxel's 32 bytes of hi and of lo pieces
+    for (int r = threadIdx.x; r < HV; r += 256) {
+        const int lx = r % HX, ly = (r / HX) % HY, lz = r / (HX * HY);
+        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly, gz = z0 - 1 + lz;
+        const bool inside = gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz;
+        const int64_t v = inside ? ((int64_t)gz * ny + gy) * nx + gx : 0;
+        bfv8 h[2], l[2];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            float x = inside ? inb[(int64_t)i * N + v] : 0.0f;
+            if (IN_SILU) x = silu(x);
+            const __bf16 hb = (__bf16)x;
+            h[i >> 3][i & 7] = hb;
+            l[i >> 3][i & 7] = (__bf16)(x - (float)hb);
+        }
+        bfv8* dh = reinterpret_cast<bfv8*>(xh + r * 16);
+        bfv8* dl = reinterpret_cast<bfv8*>(xl + r * 16);
+        dh[0] = h[0];
+        dh[1] = h[1];
+        dl[0] = l[0];
+        dl[1] = l[1];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    bfv8 wa[kConvKSteps];
+#pragma unroll
+    for (int s = 0; s < kConvKSteps; ++s) {
+        const int tap = 2 * s + (kq >> 1), i0 = 8 * (kq & 1);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int i = i0 + e;
+            uint16_t bits = 0;  // the padding taps 27 carry zero weights
+            if (tap < 27) bits = TRANS ? wbf[(i * 16 + r16) * 27 + 26 - tap] : wbf[(r16 * 16 + i) * 27 + tap];
+            wa[s][e] = __builtin_bit_cast(__bf16, bits);
+        }
+    }
+    __syncthreads();
+    fv4 acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = fv4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < kConvKSteps; ++s) {
+        const int tap0 = 2 * s + (kq >> 1);
+        const int tap = tap0 < 27 ? tap0 : 0;  // padding: any in-tile voxel (zero weights)
+        const int kz = tap / 9, ky = (tap / 3) % 3, kx = tap % 3, i0 = 8 * (kq & 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = 4 * w + j, ty = row % TY, tz = row / TY;
+            const int vox = ((tz + kz) * HY + (ty + ky)) * HX + r16 + kx;
+            const bfv8 bh = *reinterpret_cast<const bfv8*>(xh + vox * 16 + i0);
+            const bfv8 bl = *reinterpret_cast<const bfv8*>(xl + vox * 16 + i0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[s], bl, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[s], bh, acc[j], 0, 0, 0);
+        }
+    }
+    const int gx = x0 + r16;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int row = 4 * w + j, gy = y0 + row % TY, gz = z0 + row / TY;
+        if (gx >= nx || gy >= ny || gz >= nz) continue;
+        const int64_t v = ((int64_t)gz * ny + gy) * nx + gx;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = 4 * kq + r;
+            const int64_t p = ((int64_t)item * 16 + o) * N + v;
+            float val = acc[j][r];
+            if (BIAS) val += bias[o];
+            if (EPI_DSILU) val *= dsilu(zepi[p]);
+            if (OUT_SILU) val = silu(val);
+            out[p] = val;
         }
     }
 }
@@ -385,13 +494,24 @@ int weights_to_float(dft_ctx* c, int64_t count, const uint16_t* w_bf16, float* d
     return DFT_OK;
 }
 
-template <int NI, int NO, bool IN_SILU, bool TRANS, bool BIAS, bool EPI>
+template <int NI, int NO, bool IN_SILU, bool TRANS, bool BIAS, bool EPI, bool OUT_SILU = false>
 int conv(dft_ctx* c, int64_t nx, int64_t ny, int64_t nz, int nbatch, const float* w, const float* b, const float* in,
          float* out, const float* zepi) {
     int tz = 0;
     const dim3 grid = conv_grid(nx, ny, nz, nbatch, &tz);
-    k_conv<NI, NO, IN_SILU, TRANS, BIAS, EPI>
+    k_conv<NI, NO, IN_SILU, TRANS, BIAS, EPI, OUT_SILU>
         <<<grid, 256, 0, c->stream>>>((int)nx, (int)ny, (int)nz, tz, w, b, in, out, zepi);
+    DFT_CHECK_LAUNCH(c);
+    return DFT_OK;
+}
+
+template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI, bool OUT_SILU = false>
+int conv16_mfma(dft_ctx* c, int64_t nx, int64_t ny, int64_t nz, int nbatch, const uint16_t* wbf, const float* b,
+                const float* in, float* out, const float* zepi) {
+    int tz = 0;
+    const dim3 grid = conv_grid(nx, ny, nz, nbatch, &tz);
+    k_conv16_mfma<IN_SILU, TRANS, BIAS, EPI, OUT_SILU>
+        <<<grid, 256, 0, c->stream>>>((int)nx, (int)ny, (int)nz, tz, wbf, b, in, out, zepi);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }
@@ -455,10 +575,17 @@ int dft_cnn_forward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t nz
     float* a1 = a0 + layer;
     float* a2 = training ? a1 + layer : a0;  // inference: ping-pong
     if (int rc = weights_to_float(c, L.total, w_bf16, wf)) return rc;
-    if (int rc = conv<2, 16, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
-    if (int rc = conv<16, 16, true, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[1], wf + L.b[1], a0, a1, nullptr)) return rc;
-    if (int rc = conv<16, 16, true, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[2], wf + L.b[2], a1, a2, nullptr)) return rc;
-    return conv<16, 1, true, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
+    if (training) {  // the pre-activations z are kept for the backward pass; SiLU is applied on load
+        if (int rc = conv<2, 16, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
+        if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr)) return rc;
+        if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr)) return rc;
+        return conv<16, 1, true, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
+    }
+    // inference: each layer stores SiLU(z) once, instead of its consumer applying it at every halo read
+    if (int rc = conv<2, 16, false, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
+    if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr)) return rc;
+    if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr)) return rc;
+    return conv<16, 1, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
 }
 
 int dft_cnn_backward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, const uint16_t* w_bf16,
@@ -487,10 +614,10 @@ int dft_cnn_backward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t n
     if (int rc = conv<1, 16, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[3], nullptr, A, B, z2)) return rc;
     // layer 2: dz2 in B
     if (int rc = wgrad<16, 16, true>(c, nx, ny, nz, nbatch, z1, B, part, grad + L.w[2])) return rc;
-    if (int rc = conv<16, 16, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[2], nullptr, B, A, z1)) return rc;
+    if (int rc = conv16_mfma<false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], nullptr, B, A, z1)) return rc;
     // layer 1: dz1 in A
     if (int rc = wgrad<16, 16, true>(c, nx, ny, nz, nbatch, z0, A, part, grad + L.w[1])) return rc;
-    if (int rc = conv<16, 16, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[1], nullptr, A, B, z0)) return rc;
+    if (int rc = conv16_mfma<false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], nullptr, A, B, z0)) return rc;
     // layer 0: dz0 in B, raw input
     if (int rc = wgrad<2, 16, false>(c, nx, ny, nz, nbatch, in, B, part, grad + L.w[0])) return rc;
     if (loss_host) {
