@@ -157,9 +157,29 @@ typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
 typedef float fv4 __attribute__((ext_vector_type(4)));
 constexpr int kConvKSteps = 14;
 
+// A operands of one 16 -> 16 layer in operand order, wp[s * 64 + lane] (built once per
+// layer and call by k_pack_w16 from the BF16 parameters; TRANS: the flipped, transposed
+// kernel of the data gradient).  The padding taps (27) are zero.
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_pack_w16(const uint16_t* __restrict__ wbf, bfv8* __restrict__ wp) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= kConvKSteps * 64) return;
+    const int s = q / 64, lane = q % 64, r16 = lane & 15, kq = lane >> 4;
+    const int tap = 2 * s + (kq >> 1), i0 = 8 * (kq & 1);
+    bfv8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = i0 + e;
+        uint16_t bits = 0;
+        if (tap < 27) bits = TRANS ? wbf[(i * 16 + r16) * 27 + 26 - tap] : wbf[(r16 * 16 + i) * 27 + tap];
+        v[e] = __builtin_bit_cast(__bf16, bits);
+    }
+    wp[q] = v;
+}
+
 template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI_DSILU, bool OUT_SILU>
 __global__ __launch_bounds__(256) void k_conv16_mfma(int nx, int ny, int nz, int tiles_z,
-                                                     const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
+                                                     const bfv8* __restrict__ wp, const float* __restrict__ bias,
                                                      const float* __restrict__ in, float* __restrict__ out,
                                                      const float* __restrict__ zepi) {
     __shared__ __attribute__((aligned(16))) __bf16 xh[HV * 16];
@@ -195,16 +215,7 @@ __global__ __launch_bounds__(256) void k_conv16_mfma(int nx, int ny, int nz, int
     const int r16 = lane & 15, kq = lane >> 4;
     bfv8 wa[kConvKSteps];
 #pragma unroll
-    for (int s = 0; s < kConvKSteps; ++s) {
-        const int tap = 2 * s + (kq >> 1), i0 = 8 * (kq & 1);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int i = i0 + e;
-            uint16_t bits = 0;  // the padding taps 27 carry zero weights
-            if (tap < 27) bits = TRANS ? wbf[(i * 16 + r16) * 27 + 26 - tap] : wbf[(r16 * 16 + i) * 27 + tap];
-            wa[s][e] = __builtin_bit_cast(__bf16, bits);
-        }
-    }
+    for (int s = 0; s < kConvKSteps; ++s) wa[s] = wp[s * 64 + lane];
     __syncthreads();
     fv4 acc[4];
 #pragma unroll
@@ -505,13 +516,20 @@ int conv(dft_ctx* c, int64_t nx, int64_t ny, int64_t nz, int nbatch, const float
     return DFT_OK;
 }
 
+// floats of one layer's packed A operands (14 steps x 64 lanes x 16 bytes)
+constexpr int64_t kPackFloats = kConvKSteps * 64 * 4;
+
+// wpack: kPackFloats floats of device memory for the layer's packed weights
 template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI, bool OUT_SILU = false>
 int conv16_mfma(dft_ctx* c, int64_t nx, int64_t ny, int64_t nz, int nbatch, const uint16_t* wbf, const float* b,
-                const float* in, float* out, const float* zepi) {
+                const float* in, float* out, const float* zepi, float* wpack) {
+    bfv8* wp = reinterpret_cast<bfv8*>(wpack);
+    k_pack_w16<TRANS><<<(kConvKSteps * 64 + 255) / 256, 256, 0, c->stream>>>(wbf, wp);
+    DFT_CHECK_LAUNCH(c);
     int tz = 0;
     const dim3 grid = conv_grid(nx, ny, nz, nbatch, &tz);
     k_conv16_mfma<IN_SILU, TRANS, BIAS, EPI, OUT_SILU>
-        <<<grid, 256, 0, c->stream>>>((int)nx, (int)ny, (int)nz, tz, wbf, b, in, out, zepi);
+        <<<grid, 256, 0, c->stream>>>((int)nx, (int)ny, (int)nz, tz, wp, b, in, out, zepi);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
 }
@@ -545,7 +563,8 @@ int dft_cnn_param_count(int channels, int64_t* count) {
 int dft_cnn_act_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int training, int64_t* count) {
     if (!count || channels < 1 || nx < 1 || ny < 1 || nz < 1 || nbatch < 1) return DFT_EINVAL;
     const int64_t layer = (int64_t)channels * nx * ny * nz * nbatch;
-    *count = align64(net_layout(channels).total) + (training ? 3 : 2) * layer;
+    // FP32 weights, the layers, then the packed A operands of the two 16 -> 16 layers
+    *count = align64(net_layout(channels).total) + align64((training ? 3 : 2) * layer) + 2 * kPackFloats;
     return DFT_OK;
 }
 
@@ -559,7 +578,7 @@ int dft_cnn_scratch_count(int channels, int64_t nx, int64_t ny, int64_t nz, int 
         const int64_t np = (int64_t)L.cout[l] * L.cin[l];
         rows = std::max<int64_t>(rows, std::max<int64_t>(1, 256 / np) * (np * 27 + L.cout[l]));
     }
-    *count = align64(L.total) + 2 * align64(layer) + (int64_t)kWgradBlocks * rows;
+    *count = align64(L.total) + 2 * align64(layer) + align64((int64_t)kWgradBlocks * rows) + 2 * kPackFloats;
     return DFT_OK;
 }
 
@@ -574,17 +593,18 @@ int dft_cnn_forward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t nz
     float* a0 = act + align64(L.total);
     float* a1 = a0 + layer;
     float* a2 = training ? a1 + layer : a0;  // inference: ping-pong
+    float* wpk = a0 + align64((training ? 3 : 2) * layer);  // packed A operands, layers 1 and 2
     if (int rc = weights_to_float(c, L.total, w_bf16, wf)) return rc;
     if (training) {  // the pre-activations z are kept for the backward pass; SiLU is applied on load
         if (int rc = conv<2, 16, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
-        if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr)) return rc;
-        if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr)) return rc;
+        if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr, wpk)) return rc;
+        if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr, wpk + kPackFloats)) return rc;
         return conv<16, 1, true, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
     }
     // inference: each layer stores SiLU(z) once, instead of its consumer applying it at every halo read
     if (int rc = conv<2, 16, false, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
-    if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr)) return rc;
-    if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr)) return rc;
+    if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr, wpk)) return rc;
+    if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr, wpk + kPackFloats)) return rc;
     return conv<16, 1, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
 }
 
@@ -603,6 +623,13 @@ int dft_cnn_backward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t n
     float* A = scratch + align64(L.total);
     float* B = A + align64(layer);
     float* part = B + align64(layer);
+    const NetLayout L0 = net_layout(16);
+    int64_t rows = 0;
+    for (int l = 0; l < 4; ++l) {
+        const int64_t np = (int64_t)L0.cout[l] * L0.cin[l];
+        rows = std::max<int64_t>(rows, std::max<int64_t>(1, 256 / np) * (np * 27 + L0.cout[l]));
+    }
+    float* wpk = part + align64((int64_t)kWgradBlocks * rows);  // packed A operands of the data gradients
     if (int rc = weights_to_float(c, L.total, w_bf16, wf)) return rc;
     // dL/dout and the loss
     const int nblk = (int)std::min<int64_t>((M + 255) / 256, 4 * (int64_t)c->num_cus);
@@ -614,10 +641,10 @@ int dft_cnn_backward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t n
     if (int rc = conv<1, 16, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[3], nullptr, A, B, z2)) return rc;
     // layer 2: dz2 in B
     if (int rc = wgrad<16, 16, true>(c, nx, ny, nz, nbatch, z1, B, part, grad + L.w[2])) return rc;
-    if (int rc = conv16_mfma<false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], nullptr, B, A, z1)) return rc;
+    if (int rc = conv16_mfma<false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], nullptr, B, A, z1, wpk)) return rc;
     // layer 1: dz1 in A
     if (int rc = wgrad<16, 16, true>(c, nx, ny, nz, nbatch, z0, A, part, grad + L.w[1])) return rc;
-    if (int rc = conv16_mfma<false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], nullptr, A, B, z0)) return rc;
+    if (int rc = conv16_mfma<false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], nullptr, A, B, z0, wpk + kPackFloats)) return rc;
     // layer 0: dz0 in B, raw input
     if (int rc = wgrad<2, 16, false>(c, nx, ny, nz, nbatch, in, B, part, grad + L.w[0])) return rc;
     if (loss_host) {
