@@ -10,7 +10,10 @@
 // Kernels:
 //   k_conv16_mfma -- the 16 -> 16 layers (forward and data gradient) as an
 //                implicit GEMM on the bf16 matrix cores (below).
-//   k_conv    -- direct convolution (the 2 -> 16 and 16 -> 1 layers).  A workgroup owns a 16x4x4 output tile
+//                The 16 -> 1 output layer runs on the same kernel with one
+//                non-zero row of A.
+//   k_conv    -- direct convolution (the 2 -> 16 layer and the 1 -> 16 data
+//                gradient).  A workgroup owns a 16x4x4 output tile
 //                (one voxel per thread) and stages the (18x6x6) x cin input
 //                halo tile in LDS; the FP32 copy of the BF16 weights is read
 //                with wave-uniform addresses (scalar loads), so each LDS read
@@ -160,7 +163,7 @@ constexpr int kConvKSteps = 14;
 // A operands of one 16 -> 16 layer in operand order, wp[s * 64 + lane] (built once per
 // layer and call by k_pack_w16 from the BF16 parameters; TRANS: the flipped, transposed
 // kernel of the data gradient).  The padding taps (27) are zero.
-template <bool TRANS>
+template <bool TRANS, int NOUT = 16>
 __global__ __launch_bounds__(256) void k_pack_w16(const uint16_t* __restrict__ wbf, bfv8* __restrict__ wp) {
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= kConvKSteps * 64) return;
@@ -171,13 +174,13 @@ __global__ __launch_bounds__(256) void k_pack_w16(const uint16_t* __restrict__ w
     for (int e = 0; e < 8; ++e) {
         const int i = i0 + e;
         uint16_t bits = 0;
-        if (tap < 27) bits = TRANS ? wbf[(i * 16 + r16) * 27 + 26 - tap] : wbf[(r16 * 16 + i) * 27 + tap];
+        if (tap < 27 && r16 < NOUT) bits = TRANS ? wbf[(i * 16 + r16) * 27 + 26 - tap] : wbf[(r16 * 16 + i) * 27 + tap];
         v[e] = __builtin_bit_cast(__bf16, bits);
     }
     wp[q] = v;
 }
 
-template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI_DSILU, bool OUT_SILU>
+template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI_DSILU, bool OUT_SILU, int NOUT = 16>
 __global__ __launch_bounds__(256) void k_conv16_mfma(int nx, int ny, int nz, int tiles_z,
                                                      const bfv8* __restrict__ wp, const float* __restrict__ bias,
                                                      const float* __restrict__ in, float* __restrict__ out,
@@ -244,7 +247,8 @@ __global__ __launch_bounds__(256) void k_conv16_mfma(int nx, int ny, int nz, int
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int o = 4 * kq + r;
-            const int64_t p = ((int64_t)item * 16 + o) * N + v;
+            if (o >= NOUT) continue;
+            const int64_t p = ((int64_t)item * NOUT + o) * N + v;
             float val = acc[j][r];
             if (BIAS) val += bias[o];
             if (EPI_DSILU) val *= dsilu(zepi[p]);
@@ -520,15 +524,15 @@ int conv(dft_ctx* c, int64_t nx, int64_t ny, int64_t nz, int nbatch, const float
 constexpr int64_t kPackFloats = kConvKSteps * 64 * 4;
 
 // wpack: kPackFloats floats of device memory for the layer's packed weights
-template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI, bool OUT_SILU = false>
+template <bool IN_SILU, bool TRANS, bool BIAS, bool EPI, bool OUT_SILU = false, int NOUT = 16>
 int conv16_mfma(dft_ctx* c, int64_t nx, int64_t ny, int64_t nz, int nbatch, const uint16_t* wbf, const float* b,
                 const float* in, float* out, const float* zepi, float* wpack) {
     bfv8* wp = reinterpret_cast<bfv8*>(wpack);
-    k_pack_w16<TRANS><<<(kConvKSteps * 64 + 255) / 256, 256, 0, c->stream>>>(wbf, wp);
+    k_pack_w16<TRANS, NOUT><<<(kConvKSteps * 64 + 255) / 256, 256, 0, c->stream>>>(wbf, wp);
     DFT_CHECK_LAUNCH(c);
     int tz = 0;
     const dim3 grid = conv_grid(nx, ny, nz, nbatch, &tz);
-    k_conv16_mfma<IN_SILU, TRANS, BIAS, EPI, OUT_SILU>
+    k_conv16_mfma<IN_SILU, TRANS, BIAS, EPI, OUT_SILU, NOUT>
         <<<grid, 256, 0, c->stream>>>((int)nx, (int)ny, (int)nz, tz, wp, b, in, out, zepi);
     DFT_CHECK_LAUNCH(c);
     return DFT_OK;
@@ -563,8 +567,8 @@ int dft_cnn_param_count(int channels, int64_t* count) {
 int dft_cnn_act_count(int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, int training, int64_t* count) {
     if (!count || channels < 1 || nx < 1 || ny < 1 || nz < 1 || nbatch < 1) return DFT_EINVAL;
     const int64_t layer = (int64_t)channels * nx * ny * nz * nbatch;
-    // FP32 weights, the layers, then the packed A operands of the two 16 -> 16 layers
-    *count = align64(net_layout(channels).total) + align64((training ? 3 : 2) * layer) + 2 * kPackFloats;
+    // FP32 weights, the layers, then the packed A operands of layers 1, 2 and 3
+    *count = align64(net_layout(channels).total) + align64((training ? 3 : 2) * layer) + 3 * kPackFloats;
     return DFT_OK;
 }
 
@@ -593,19 +597,21 @@ int dft_cnn_forward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t nz
     float* a0 = act + align64(L.total);
     float* a1 = a0 + layer;
     float* a2 = training ? a1 + layer : a0;  // inference: ping-pong
-    float* wpk = a0 + align64((training ? 3 : 2) * layer);  // packed A operands, layers 1 and 2
+    float* wpk = a0 + align64((training ? 3 : 2) * layer);  // packed A operands, layers 1, 2 and 3
     if (int rc = weights_to_float(c, L.total, w_bf16, wf)) return rc;
     if (training) {  // the pre-activations z are kept for the backward pass; SiLU is applied on load
         if (int rc = conv<2, 16, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
         if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr, wpk)) return rc;
         if (int rc = conv16_mfma<true, false, true, false>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr, wpk + kPackFloats)) return rc;
-        return conv<16, 1, true, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
+        return conv16_mfma<true, false, true, false, false, 1>(c, nx, ny, nz, nbatch, w_bf16 + L.w[3], wf + L.b[3], a2, out,
+                                                              nullptr, wpk + 2 * kPackFloats);
     }
     // inference: each layer stores SiLU(z) once, instead of its consumer applying it at every halo read
     if (int rc = conv<2, 16, false, false, true, false, true>(c, nx, ny, nz, nbatch, wf + L.w[0], wf + L.b[0], in, a0, nullptr)) return rc;
     if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[1], wf + L.b[1], a0, a1, nullptr, wpk)) return rc;
     if (int rc = conv16_mfma<false, false, true, false, true>(c, nx, ny, nz, nbatch, w_bf16 + L.w[2], wf + L.b[2], a1, a2, nullptr, wpk + kPackFloats)) return rc;
-    return conv<16, 1, false, false, true, false>(c, nx, ny, nz, nbatch, wf + L.w[3], wf + L.b[3], a2, out, nullptr);
+    return conv16_mfma<false, false, true, false, false, 1>(c, nx, ny, nz, nbatch, w_bf16 + L.w[3], wf + L.b[3], a2, out,
+                                                             nullptr, wpk + 2 * kPackFloats);
 }
 
 int dft_cnn_backward(dft_ctx* c, int channels, int64_t nx, int64_t ny, int64_t nz, int nbatch, const uint16_t* w_bf16,
