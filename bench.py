@@ -703,6 +703,13 @@ def main():
         "hpsi_f32_GBps": ach, "hpsi_f64_GBps": by64 / ms64 / 1e6,
         "hpsi_f32_gpt_orb_per_s": grid.size * nb / (ms * 1e-3), "hpsi_f64_gpt_orb_per_s": grid.size * nb / (ms64 * 1e-3),
         "dense_products": dense,
+        # the timed run's largest kernel (the fused Ritz residual at the subspace limit, bf16x6 split)
+        "roofline_dominant": ({"kernel": "Ritz residual [W | V][Q ; -Q theta] with norms, bf16x6 split, "
+                                         f"{dense['ritz_residual_bf16x6']['shape']}",
+                               "bound": "hbm", "achieved": dense["ritz_residual_bf16x6"]["GBps"], "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": dense["ritz_residual_bf16x6"]["hbm_frac"],
+                               "avg_launch_ms": dense["ritz_residual_bf16x6"]["ms"],
+                               "mfma_frac": dense["ritz_residual_bf16x6"]["mfma_frac"]} if dense else None),
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(f"hpsi_f32_{w['n']}x{w['n']}x{w['n']}_nb{nb}"),
                      "kernel": f"H psi FP32, {w['n']}^3 x {nb} orbitals",
