@@ -261,28 +261,44 @@ int update_mfma(dft_ctx* c, int64_t n, int m1, int m, int k, T alpha, const T* A
 // ---- FP32 block update on the bf16 matrix cores (three-piece split) -------
 // Y = [A | A2] (alpha C) + beta Y for FP32 blocks: N grid points, m inner
 // rows (m1 from A, the rest from A2), k <= 96 output columns.  Every FP32
-// operand value x is split as x1 + x2 + x3 with x1 = bf16(x), x2 =
-// bf16(x - x1), x3 = bf16(x - x1 - x2) (24 significand bits: exact for
-// normal values), and a product keeps the six terms of order <= 2^-18,
-// a3b1 + a1b3 + a2b2 + a2b1 + a1b2 + a1b1 (smallest first; the dropped
-// terms are ~2^-27 relative), each a v_mfma_f32_16x16x32_bf16 with FP32
-// accumulation: FP32-level products at the bf16 MFMA rate, so the pass is
-// bound by its HBM stream (N (m + k) floats) rather than by FP32 MFMA.
+// operand value x is split as x1 + x2 + x3 (24 significand bits: exact for
+// normal values), and a product keeps the six largest terms, a3b1 + a1b3 +
+// a2b2 + a2b1 + a1b2 + a1b1 (smallest first), each a
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation: FP32-level products at
+// the bf16 MFMA rate, so the pass is bound by its HBM stream (N (m + k)
+// floats) rather than by FP32 MFMA.  Both operands are split by
+// rounding, x1 = bf16(x), x2 = bf16(x - x1), x3 = bf16(x - x1 - x2), so
+// |x2| <= 2^-8 |x|, |x3| <= 2^-16 |x| and the dropped terms a2b3 + a3b2 +
+// a3b3 stay below 2^-23 |ab|.
 // Layout: MFMA operand A = (alpha C)^T, row j = output column, K = inner
 // row; operand B = the grid columns, K = inner row, column = grid point.
-// Lane l of a wave: points p0 + 4 (l % 16) .. + 3 (one 16-byte load per
-// inner row), K slice 8 (l / 16) .. + 7 of each 32-row step; result
-// register r of tile jt = output column jt 16 + 4 (l / 16) + r at those
-// four points (one 16-byte store).  The split coefficients are prepared once
-// per call in operand order ([32-row step][jt][piece][lane][8]) and staged
-// through LDS 64 inner rows at a time for the 8 waves (512 points) of a
-// workgroup.  Y may alias columns of A or A2: a wave reads every inner row
-// at its points before it writes there.  NORM: part[j * gridDim.x + b] =
-// the workgroup's sum of squares of stored column j.
+// Lane l of a wave: points p0 + 4 (l % 16) .. + 3 (16 bytes per inner row),
+// K slice 8 (l / 16) .. + 7 of each 32-row step; result register r of tile
+// jt = output column jt 16 + 4 (l / 16) + r at those four points (one
+// 16-byte store).  The split coefficients are prepared once per call in
+// operand order ([32-row step][jt][piece][lane][8]) and staged through LDS
+// KSC steps at a time for the 8 waves (512 points) of a workgroup.
+// The grid values reach LDS by direct loads (global_load_lds_dwordx4), one
+// 32-row step ahead, into a per-wave staging buffer: no registers hold them
+// in flight, so two waves per SIMD fit the 256-register budget while the
+// next step's loads overlap this step's products.  Lane l's 16 bytes of
+// direct load e land at stage[w][64 e + l] (the instruction writes M0 +
+// 16 x lane), which is where lane l reads them back.
+// Y may alias columns of A or A2: a wave reads every inner row at its
+// points before it writes there.  NORM: part[j * gridDim.x + b] = the
+// workgroup's sum of squares of stored column j.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kS3Waves = 8, kS3Chunk = 64;
+// bf16(lo) | bf16(hi) << 16, rounded to nearest even (one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+
+constexpr int kS3Waves = 8, kS3Steps = 4;  // waves per workgroup, 32-row steps per coefficient chunk
 
 __global__ __launch_bounds__(256) void k_split_coef(int m, int k, int njt, int nks, double alpha,
                                                     const double* __restrict__ cm, int64_t ldc,
@@ -311,28 +327,60 @@ __global__ __launch_bounds__(256) void k_split_coef(int m, int k, int njt, int n
     o[128] = p3;
 }
 
-template <int NJT, bool NORM>
+template <int NJT, bool NORM, int KSC = kS3Steps>
 __global__ __launch_bounds__(512) void k_update_split3(int64_t n, int m1, int m, int k, const float* A, int64_t lda,
-                                                       const float* A2, int64_t lda2,
-                                                       const bf16x8* __restrict__ cs, float beta, float* Y,
-                                                       int64_t ldy, double* __restrict__ part) {
-    constexpr int KSC = kS3Chunk / 32;
+                                                        const float* A2, int64_t lda2,
+                                                        const bf16x8* __restrict__ cs, float beta, float* Y,
+                                                        int64_t ldy, double* __restrict__ part) {
+    constexpr int W = kS3Waves;
     __shared__ bf16x8 lds[KSC * NJT * 3 * 64];
-    __shared__ double red[NORM ? kS3Waves : 1][NORM ? NJT * 16 : 1];  // per-wave FP64 column sums of squares
+    __shared__ f32x4 stage[W][8 * 64];
+    __shared__ double red[NORM ? W : 1][NORM ? NJT * 16 : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r16 = lane & 15, qd = lane >> 4;
     const int nks = (m + 31) / 32;
     const int64_t ntile = (n + 63) / 64;
     if constexpr (NORM)
-        for (int q = threadIdx.x; q < kS3Waves * NJT * 16; q += 512) (&red[0][0])[q] = 0.0;
-    for (int64_t t0 = (int64_t)blockIdx.x * kS3Waves; t0 < ntile; t0 += (int64_t)gridDim.x * kS3Waves) {
+        for (int q = threadIdx.x; q < W * NJT * 16; q += 512) (&red[0][0])[q] = 0.0;
+    for (int64_t t0 = (int64_t)blockIdx.x * W; t0 < ntile; t0 += (int64_t)gridDim.x * W) {
         const int64_t p = (t0 + w) * 64 + 4 * r16;
-        const bool pin = p < n;  // n % 4 == 0: all four points or none
+        const bool pin = p < n;
+        const int64_t pc = pin ? p : 0;  // lanes past the grid load point 0 and store nothing
+        // rows past m read row m - 1, whose coefficients are zero
+        // per-lane byte offsets of this lane's first row of a step, from the
+        // step's first row in A and in A2 (32-bit: the 8 rows of a lane group
+        // and the point lie within 4 GiB of the step's row base)
+        const uint32_t offA = (uint32_t)(((int64_t)8 * qd * lda + pc) * 4);
+        const uint32_t offA2 = (uint32_t)(((int64_t)8 * qd * lda2 + pc) * 4);
+        auto issue = [&](int ks) {
+            const int r0 = ks * 32;
+            if (r0 + 31 < m && (r0 + 31 < m1 || r0 >= m1)) {
+                // the step's 32 rows lie in one of A, A2: a uniform row base
+                // and stride, and one per-lane offset
+                const bool inA = r0 < m1;
+                const char* base = inA ? (const char*)(A + (int64_t)r0 * lda) : (const char*)(A2 + (int64_t)(r0 - m1) * lda2);
+                const int64_t stride = (inA ? lda : lda2) * 4;
+                const uint32_t off = inA ? offA : offA2;
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    __builtin_amdgcn_global_load_lds((const void*)(base + e * stride + off),
+                                                     (__attribute__((address_space(3))) void*)&stage[w][64 * e], 16, 0, 0);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = min(r0 + 8 * qd + e, m - 1);
+                    const float* s = i < m1 ? A + (int64_t)i * lda : A2 + (int64_t)(i - m1) * lda2;
+                    __builtin_amdgcn_global_load_lds((const void*)(s + pc),
+                                                     (__attribute__((address_space(3))) void*)&stage[w][64 * e], 16, 0, 0);
+                }
+            }
+        };
         f32x4 acc[4][NJT];
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
             for (int jt = 0; jt < NJT; ++jt) acc[c][jt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        issue(0);
         for (int ks0 = 0; ks0 < nks; ks0 += KSC) {
             __syncthreads();  // the previous chunk's operands are consumed
             const int nsteps = min(KSC, nks - ks0);
@@ -340,30 +388,29 @@ __global__ __launch_bounds__(512) void k_update_split3(int64_t n, int m1, int m,
             for (int q = threadIdx.x; q < nsteps * NJT * 3 * 64; q += 512) lds[q] = src[q];
             __syncthreads();
             for (int kk = 0; kk < nsteps; ++kk) {
-                const int ib = (ks0 + kk) * 32 + 8 * qd;
+                // this step's direct loads have landed; read them, and let the
+                // reads finish before the next step's loads overwrite the buffer
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 f32x4 x[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int i = ib + e;
-                    if (pin && i < m) {
-                        const float* s = i < m1 ? A + (int64_t)i * lda : A2 + (int64_t)(i - m1) * lda2;
-                        x[e] = *reinterpret_cast<const f32x4*>(s + p);
-                    } else {
-                        x[e] = f32x4{0.f, 0.f, 0.f, 0.f};
-                    }
-                }
-                bf16x8 b1[4], b2[4], b3[4];
+                for (int e = 0; e < 8; ++e) x[e] = stage[w][64 * e + lane];
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                if (ks0 + kk + 1 < nks) issue(ks0 + kk + 1);
+                // pieces by rounding, two values per conversion: the packed
+                // pair's halves are the rounded values' top bits
+                u32x4 b1[4], b2[4], b3[4];
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float v = x[e][c];
-                        const __bf16 h1 = (__bf16)v;
-                        const float r1 = v - (float)h1;
-                        const __bf16 h2 = (__bf16)r1;
-                        b1[c][e] = h1;
-                        b2[c][e] = h2;
-                        b3[c][e] = (__bf16)(r1 - (float)h2);
+                    for (int e = 0; e < 8; e += 2) {
+                        const float v0 = x[e][c], v1 = x[e + 1][c];
+                        const uint32_t h = pack_bf16x2(v0, v1);
+                        const float r0 = v0 - __uint_as_float(h << 16), r1 = v1 - __uint_as_float(h & 0xffff0000u);
+                        const uint32_t g = pack_bf16x2(r0, r1);
+                        const float s0 = r0 - __uint_as_float(g << 16), s1 = r1 - __uint_as_float(g & 0xffff0000u);
+                        b1[c][e / 2] = h;
+                        b2[c][e / 2] = g;
+                        b3[c][e / 2] = pack_bf16x2(s0, s1);
                     }
 #pragma unroll
                 for (int jt = 0; jt < NJT; ++jt) {
@@ -371,18 +418,19 @@ __global__ __launch_bounds__(512) void k_update_split3(int64_t n, int m1, int m,
                     const bf16x8 a1 = blk[lane], a2 = blk[64 + lane], a3 = blk[128 + lane];
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
+                        const bf16x8 p1 = __builtin_bit_cast(bf16x8, b1[c]), p2 = __builtin_bit_cast(bf16x8, b2[c]),
+                                     p3 = __builtin_bit_cast(bf16x8, b3[c]);
                         f32x4 s = acc[c][jt];
-                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1[c], s, 0, 0, 0);
-                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3[c], s, 0, 0, 0);
-                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2[c], s, 0, 0, 0);
-                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1[c], s, 0, 0, 0);
-                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2[c], s, 0, 0, 0);
-                        acc[c][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[c], s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, p1, s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, p3, s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, p2, s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, p1, s, 0, 0, 0);
+                        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, p2, s, 0, 0, 0);
+                        acc[c][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, p1, s, 0, 0, 0);
                     }
                 }
             }
         }
-        // epilogue; lanes past the grid keep to the shuffles of the norm sums
 #pragma unroll
         for (int jt = 0; jt < NJT; ++jt)
 #pragma unroll
@@ -409,7 +457,7 @@ __global__ __launch_bounds__(512) void k_update_split3(int64_t n, int m1, int m,
         for (int j = threadIdx.x; j < k; j += 512) {
             double s = 0.0;
 #pragma unroll
-            for (int q = 0; q < kS3Waves; ++q) s += red[q][j];
+            for (int q = 0; q < W; ++q) s += red[q][j];
             part[(int64_t)j * gridDim.x + blockIdx.x] = s;
         }
     }
@@ -419,7 +467,9 @@ __global__ __launch_bounds__(512) void k_update_split3(int64_t n, int m1, int m,
 // aligned columns)
 bool split3_fits(int64_t n, int k, int64_t lda, int64_t lda2, int64_t ldy, const void* a, const void* a2,
                  const void* y) {
-    return k >= 1 && k <= 96 && n % 4 == 0 && lda % 4 == 0 && lda2 % 4 == 0 && ldy % 4 == 0 &&
+    // the kernel's per-lane offsets from a step's first row are 32-bit
+    const bool off32 = 4 * (24 * std::max(lda, lda2) + n) < (int64_t)1 << 32;
+    return k >= 1 && k <= 96 && n % 4 == 0 && lda % 4 == 0 && lda2 % 4 == 0 && ldy % 4 == 0 && off32 &&
            reinterpret_cast<uintptr_t>(a) % 16 == 0 && reinterpret_cast<uintptr_t>(a2) % 16 == 0 &&
            reinterpret_cast<uintptr_t>(y) % 16 == 0;
 }
