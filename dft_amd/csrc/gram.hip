@@ -33,6 +33,13 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
 __device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }  // round to nearest even
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+// bf16(lo) | bf16(hi) << 16, rounded to nearest even (one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
 
 // result row of register r held by quarter-wave q (see the layout note above)
 template <bool F64>
@@ -471,21 +478,24 @@ __global__ __launch_bounds__(64 * kWideWaves) void k_gram_wide(int64_t n, int ma
     auto store = [&](const float (&raw)[kWideChunks][4], int buf) {  // split a step's values into LDS buffer buf
 #pragma unroll
         for (int u = 0; u < kWideChunks; ++u) {
-            bf4 x1, x2, x3;
+            // two values per conversion; the packed pair's halves are the
+            // rounded values' top bits
+            u2 x1, x2, x3;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float v = colv[u] ? raw[u][e] : 0.0f;
-                const __bf16 h1 = to_bf16(v);
-                const float r1 = v - (float)h1;
-                const __bf16 h2 = to_bf16(r1);
-                x1[e] = h1;
-                x2[e] = h2;
-                if constexpr (P == 3) x3[e] = to_bf16(r1 - (float)h2);
+            for (int e = 0; e < 4; e += 2) {
+                const float v0 = colv[u] ? raw[u][e] : 0.0f, v1 = colv[u] ? raw[u][e + 1] : 0.0f;
+                const uint32_t h = pack_bf16x2(v0, v1);
+                const float r0 = v0 - __uint_as_float(h << 16), r1 = v1 - __uint_as_float(h & 0xffff0000u);
+                const uint32_t g = pack_bf16x2(r0, r1);
+                x1[e / 2] = h;
+                x2[e / 2] = g;
+                if constexpr (P == 3)
+                    x3[e / 2] = pack_bf16x2(r0 - __uint_as_float(g << 16), r1 - __uint_as_float(g & 0xffff0000u));
             }
             __bf16* d = sl + buf * BUF + dst0 + DSTEP * u;
-            *reinterpret_cast<bf4*>(d) = x1;
-            *reinterpret_cast<bf4*>(d + PIECE) = x2;
-            if constexpr (P == 3) *reinterpret_cast<bf4*>(d + 2 * PIECE) = x3;
+            *reinterpret_cast<u2*>(d) = x1;
+            *reinterpret_cast<u2*>(d + PIECE) = x2;
+            if constexpr (P == 3) *reinterpret_cast<u2*>(d + 2 * PIECE) = x3;
         }
     };
     auto products = [&](int buf) {  // this wave's tiles over the step in LDS buffer buf
