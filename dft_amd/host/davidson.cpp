@@ -245,7 +245,8 @@ int BlockDavidson::restart_in_place(int m, int kept, const std::vector<double>& 
           "dft_block_update");
     check(c_, dft_block_update(c_, prec_, n_, m, kn, 1.0, W_.get(), n_, Cr_.as<double>(), m, 0.0, W_.get(), n_),
           "dft_block_update");
-    check(c_, dft_copy(c_, X_.get(), V_.get(), (size_t)n_ * es_ * k), "dft_copy");
+    // the Ritz block is now V's first k columns; it is copied to X_ only if
+    // the iteration ends before the next Rayleigh-Ritz step
     gram(kn, kn, V_.get(), W_.get(), G_.as<double>(), mmax_);
     return kn;
 }
@@ -340,8 +341,9 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
     P.reset();
     int mprev = 0;  // basis size when Qprev was taken (0: none yet)
     const int nact = nconv;
-    // X_ holds V Q_k of the latest Rayleigh-Ritz step only when x_fresh
-    bool x_fresh = false;
+    // X_ holds V Q_k of the latest Rayleigh-Ritz step only when x_fresh; the
+    // Ritz block is V's first k columns when x_in_v (after an in-place restart)
+    bool x_fresh = false, x_in_v = false;
     int m_rr = m;
     auto ritz_block = [&](bool with_hx) {
         check(c_, dft_block_update(c_, prec_, n_, m_rr, k, 1.0, V, n_, Q, mmax_, 0.0, X_.get(), n_), "dft_block_update");
@@ -372,7 +374,7 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
                                      std::to_string(asym) + "): " + std::string(dft_last_error(c_)));
         }
         m_rr = m;
-        x_fresh = false;
+        x_fresh = x_in_v = false;
         if (px) ritz_block(false);
         check(c_, dft_ritz_residual(c_, prec_, n_, m, k, V, n_, W, n_, Q, mmax_, theta, R_.get(), n_, rn_.as<double>()),
               "dft_ritz_residual");
@@ -410,8 +412,9 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
             std::vector<double> Qh, Pc;
             const int kept = restart_coefficients(m, mprev, std::max(0, mmax_ - k - nt), Qh, Pc);
             if (dft_block_update_in_place(prec_, m, k + kept)) {
+                // X_, if fresh, already holds the same V Q_k for the correction below
                 m = restart_in_place(m, kept, Qh, Pc);
-                x_fresh = true;
+                x_in_v = true;
             } else {
                 ritz_block(true);
                 m = restart(m, kept, Pc);
@@ -449,8 +452,9 @@ DavidsonStats BlockDavidson::solve(const void* veff, bool warm, const DavidsonOp
         m += kept;
     }
     // the Ritz block of the last Rayleigh-Ritz step (V's first m_rr columns
-    // are unchanged unless a restart formed X_ itself)
-    if (!x_fresh) ritz_block(false);
+    // are unchanged unless a restart formed X_ itself or left it in V)
+    if (x_in_v) check(c_, dft_copy(c_, X_.get(), V_.get(), (size_t)n_ * es_ * k), "dft_copy");
+    else if (!x_fresh) ritz_block(false);
     return st;
 }
 
