@@ -653,9 +653,10 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             const auto& ea = f.da->eigenvalues();
             const double homo = ea[nal - 1], lumo = (int)ea.size() > nal ? ea[nal] : homo;
             std::fprintf(stderr,
-                         "[scf] %3d %s E=%.12f dE=%.3e dRho=%.3e KS=%.3e mix=%.3f eps0=%.8f homo=%.5f lumo=%.5f dav=%d/%d\n",
+                         "[scf] %3d %s E=%.12f dE=%.3e dRho=%.3e KS=%.3e mix=%.3f eps0=%.8f homo=%.5f lumo=%.5f dav=%d/%d "
+                         "t=%.2fs\n",
                          it, p == DFT_F64 ? "f64" : "f32", E[5], de, dd, ks, imix, out.eig[0], homo, lumo, sa.iterations,
-                         closed ? 0 : sb.iterations);
+                         closed ? 0 : sb.iterations, secs(t0, Clock::now()));
         }
         bool conv = cfg_.stop_rule == DFT_STOP_DE ? de < etol : (de < kEnergyTol && dd < kDensityTol && ks < kKsTol);
         if (refining) ++refine_steps;
