@@ -499,12 +499,13 @@ def test_block_update(dev, n, m, k):
 
 
 @pytest.mark.parametrize("n,m,k", [(64000, 87, 29), (70001, 40, 29), (4096, 61, 96), (30002, 150, 40), (5000, 3, 1),
-                                   (20000, 288, 96), (20001, 97, 70), (20003, 201, 33)])
+                                   (20000, 288, 96), (20001, 97, 70), (20003, 201, 33), (20000, 150, 96)])
 def test_ritz_residual(dev, n, m, k):
     """R = W Q - V Q diag(theta) and its column norms^2, against numpy: one
     fused pass (vector and scalar grid sizes); with the 2m x k coefficients
     beyond the LDS budget, FP32 as one bf16 split pass over [W | V] with its
-    norms when the columns are 16-byte aligned (m = 150, 288), else with
+    norms when the columns are 16-byte aligned (m = 288; m = 150 at k = 96,
+    where one 32-row step straddles W and V and the last one passes 2m), else with
     k <= 48 as chunks of the [W | V] rows, one of them straddling W and V,
     the norms from the last (m = 201; FP64 m = 150), and wider blocks
     (m = 97 at k = 70; FP64 m = 288) as two library passes and the norms."""
