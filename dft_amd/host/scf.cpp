@@ -19,8 +19,16 @@ double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::dura
 // DFTConstants.h:14-19 and SelfConsistentFieldMolecule.cpp:27-31
 constexpr double kMixing0 = 0.25, kMinMix = 0.10, kMaxMix = 0.50, kMixUp = 1.10, kMixDown = 0.75, kOsc = 1.05;
 constexpr double kEnergyTol = 1.0e-11, kDensityTol = 1.0e-9, kKsTol = 1.0e-7;
-// mixed mode: where the FP32 phase hands over to FP64
-constexpr double kF32EnergyFloor = 3.0e-4, kF32DensityFloor = 1.0e-5;
+// mixed mode: where the FP32 phase hands over to FP64 (|dE| below the floor
+// or the stop tolerance, whichever is larger).  Near the hand-over an FP32
+// step costs a few Davidson iterations and an FP64 step some twenty, so the
+// FP32 phase runs on while the density still settles: on the 40-atom
+// cluster at 256^3 (PBE96, smeared) a floor of 3e-4 handed over at dRho
+// 2.8e-5 and took 4 FP64 steps; 1e-5 hands over at dRho 8.9e-6 after three
+// more FP32 steps and takes 3.  (Run on to |dE| < 1e-7 the deeper FP32 phase
+// costs FP64 steps instead: its last Pulay entries hold FP32 noise, which
+// the refinement drops when it stalls; see below.)
+constexpr double kF32EnergyFloor = 1.0e-5, kF32DensityFloor = 1.0e-5;
 // ... or, below this relative density change, once three FP32 steps in a row
 // have not lowered it by 10% (the FP32 phase has reached its own noise floor)
 constexpr double kF32StallDensity = 1.0e-3;
@@ -115,6 +123,37 @@ class PulayMixer {
         A_ = o.A_;
         count_ = o.count_;
         head_ = o.head_;
+        inherited_.assign(depth_, false);
+        for (int i = 0; i < count_; ++i) inherited_[i] = true;
+    }
+
+    // Forget the entries taken over from another mixer that this one has not
+    // yet overwritten: the kept entries move to slots 0.. in slot order (a
+    // kept slot only ever moves down, onto a dropped or already-moved one)
+    // and the DIIS matrix follows.  Returns the number of entries dropped.
+    int drop_inherited() {
+        if (inherited_.empty()) return 0;
+        std::vector<int> keep;
+        for (int i = 0; i < count_; ++i)
+            if (!inherited_[i]) keep.push_back(i);
+        const int dropped = count_ - (int)keep.size();
+        if (dropped == 0) return 0;
+        const size_t cb = (size_t)L_ * esize(prec_);
+        char* h = static_cast<char*>(hist_.get());
+        std::vector<double> A2((size_t)depth_ * depth_, 0.0);
+        for (int t = 0; t < (int)keep.size(); ++t) {
+            const int s = keep[t];
+            if (s != t) {
+                check(c_, dft_copy(c_, h + (size_t)t * cb, h + (size_t)s * cb, cb), "dft_copy");
+                check(c_, dft_copy(c_, h + (size_t)(depth_ + t) * cb, h + (size_t)(depth_ + s) * cb, cb), "dft_copy");
+            }
+            for (int u = 0; u < (int)keep.size(); ++u) A2[(size_t)t * depth_ + u] = A_[(size_t)s * depth_ + keep[u]];
+        }
+        A_ = A2;
+        count_ = (int)keep.size();
+        head_ = count_ % depth_;
+        inherited_.assign(depth_, false);
+        return dropped;
     }
 
     void mix(void* ra, void* rb, const void* ora, const void* orb) {
@@ -132,6 +171,7 @@ class PulayMixer {
             check(c_, dft_axpby(c_, prec_, n_, -1.0, rb, 1.0, res + fb), "dft_axpby");
         }
         count_ = std::min(count_ + 1, depth_);
+        if (!inherited_.empty()) inherited_[head_] = false;
         // new row: <R_i, R_head> for every stored i (slots 0..count-1 hold the history)
         const int mode = prec_ == DFT_F64 ? DFT_GRAM_F64 : DFT_GRAM_F32;
         check(c_, dft_gram(c_, mode, L_, count_, 1, 1.0, h + (size_t)depth_ * cb, L_, res, L_, 0.0,
@@ -251,6 +291,7 @@ class PulayMixer {
     Buf hist_, coef_, dots_, rbar_, kr_;
     std::vector<double> A_;
     int count_ = 0, head_ = 0;
+    std::vector<bool> inherited_;  // slots still holding another mixer's entries
 };
 
 struct ScfDriver::Fields {
@@ -551,6 +592,12 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
     double best_dd = std::numeric_limits<double>::infinity();  // FP32 phase: lowest dRho so far
     int stall = 0;
     int refine_steps = 0;
+    // FP64 refinement: three steps in a row without a 10% lower dRho drop the
+    // FP32 steps' entries from the Pulay history (near the FP32 noise floor
+    // they hold noise at the size of the remaining residual)
+    double best_dd64 = std::numeric_limits<double>::infinity();
+    int stall64 = 0;
+    bool dropped32 = false;
     for (int it = 1; it <= maxit + (mixed ? std::max(cfg_.refine_iterations, 0) : 0); ++it) {
         if (!refining && it > maxit) break;
         Fields& f = *F;
@@ -660,9 +707,10 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         }
         bool conv = cfg_.stop_rule == DFT_STOP_DE ? de < etol : (de < kEnergyTol && dd < kDensityTol && ks < kKsTol);
         if (refining) ++refine_steps;
-        // The FP32 phase stops at its own noise floor (FP32 fields carry ~1e-4 Ha
-        // of energy noise and ~1e-6 of relative density noise) and hands over to
-        // the FP64 refinement, which then applies the configured rule.
+        // The FP32 phase stops when |dE| reaches the floor above or when dRho
+        // stops falling (FP32 fields carry ~1e-4 Ha of energy noise and ~1e-6
+        // of relative density noise) and hands over to the FP64 refinement,
+        // which then applies the configured rule.
         if (mixed && !refining && f.prec == DFT_F32) {
             conv = cfg_.stop_rule == DFT_STOP_DE ? de < std::max(etol, kF32EnergyFloor) : dd < kF32DensityFloor;
             if (dd < 0.9 * best_dd) {
@@ -676,6 +724,15 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
         if (conv && !hand_over) {
             rep.converged = 1;
             break;
+        }
+        if (refining && !dropped32) {
+            if (dd < 0.9 * best_dd64) {
+                best_dd64 = dd;
+                stall64 = 0;
+            } else if (++stall64 >= 3 && pulay && pulay_prec == p) {
+                pulay->drop_inherited();
+                dropped32 = true;
+            }
         }
         if (cfg_.mixer == DFT_MIX_PULAY) {
             if (!pulay || pulay_prec != p) {
