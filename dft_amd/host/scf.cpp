@@ -496,13 +496,7 @@ void ScfDriver::run(dft_scf_report& rep, ScfOutputs& out) {
             // With smeared occupations a residual r turns orbitals within ~kT of
             // each other by ~r / gap while their occupations differ by ~gap / 4kT:
             // the density moves by ~r / 4kT, so r must stay below ~0.4 kT of dRho
-            // (0.4 kT keeps that motion below 0.1 dRho.  The FP64 refinement of a mixed
-            // run allows 0.3 dRho, 1.2 kT: it starts within the FP32 noise floor of the
-            // fixed point, its Davidson iterations cost 2.5x an FP32 one, and its energy
-            // error still goes with r^2.  On the 40-atom cluster at 256^3 this took the
-            // 3 refinement steps from 56 to 42 Davidson iterations.)
-            const double kfac = mixed && f.prec == DFT_F64 ? 1.2 : 0.4;
-            const double dd_tol = smear ? std::min(0.1, kfac * kT) * dd_prev : 0.1 * dd_prev;
+            const double dd_tol = smear ? std::min(0.1, 0.4 * kT) * dd_prev : 0.1 * dd_prev;
             // Whole-orbital runs: while the energy still moves by |dE| the orbitals need
             // only be converged to ~0.3 sqrt(0.1 |dE|) (capped at 0.1): early steps,
             // whose density changes by O(1), take a few iterations each.  Smeared runs
